@@ -1,0 +1,191 @@
+/*
+ * monkeygl_hip.h -- C ABI of the MI355X (gfx950) render path.
+ *
+ * Plain C, POD structs, host or device pointers plus sizes. No HIP or torch
+ * types appear in any signature, so the library can be bound from C++, from
+ * ctypes or from any FFI.
+ *
+ * Every entry point below replaces one extern "C" launcher or device-data
+ * call of the reference (MonkeyGL, core/kernel.cu and core/CuDataManager.cu);
+ * the replaced symbol is cited beside each declaration. Texture objects and
+ * cudaArrays of the reference are replaced by an opaque context that owns a
+ * bricked int16 volume, a bricked uint8 label mask, per-label transfer
+ * function LUTs and per-brick statistics, all resident in HBM.
+ *
+ * Error convention: every call returns mk_status (MK_OK == 0). A failing call
+ * leaves a message retrievable with mk_last_error(ctx). Nothing ever exits
+ * the process (the reference calls exit() from checkCudaErrors).
+ *
+ * Threading: a context is used by one host thread at a time. Calls on one
+ * context are ordered on the context's stream. Different contexts are
+ * independent.
+ */
+#ifndef MONKEYGL_HIP_H
+#define MONKEYGL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* core/Defines.h:34 MAXOBJECTCOUNT; labels 0..MK_MAX_OBJECTS are valid. */
+#define MK_MAX_OBJECTS 64
+#define MK_NUM_LABELS (MK_MAX_OBJECTS + 1)
+
+typedef enum {
+    MK_OK = 0,
+    MK_ERR_ARG = 1,      /* null pointer, bad size, bad enum            */
+    MK_ERR_STATE = 2,    /* e.g. render before any volume was uploaded  */
+    MK_ERR_HIP = 3,      /* a HIP runtime call failed                   */
+    MK_ERR_NOMEM = 4,    /* device allocation failed                    */
+    MK_ERR_RANGE = 5     /* data out of range (label > MK_MAX_OBJECTS)  */
+} mk_status;
+
+/* core/Defines.h RenderType (RenderTypeVR, RenderTypeMIP, RenderTypeSurface) */
+typedef enum { MK_RENDER_VR = 0, MK_RENDER_MIP = 1, MK_RENDER_SURFACE = 2 } mk_render_type;
+
+/* core/Defines.h MPRType (MPRTypeAverage, MPRTypeMIP, MPRTypeMinIP) */
+typedef enum { MK_MPR_AVERAGE = 0, MK_MPR_MIP = 1, MK_MPR_MINIP = 2 } mk_mpr_type;
+
+/* Volume sampling for MPR/CPR. LINEAR is the reference's only mode
+ * (core/CuDataManager.cu:64). NEAREST has no reference counterpart: it reads
+ * the voxel floor(f * dim) (clamped) and is bit-exact across host and device. */
+typedef enum { MK_SAMPLE_LINEAR = 0, MK_SAMPLE_NEAREST = 1 } mk_sampling;
+
+/* core/Defines.h AlphaAndWWWL: per-label opacity cap and window. */
+typedef struct { float alpha, ww, wl; } mk_alpha_wwwl;
+
+/* core/Defines.h VOI: inclusive voxel index box. */
+typedef struct { int left, right, posterior, anterior, head, foot; } mk_voi;
+
+/*
+ * Arguments of one VR / MIP / Surface frame: the by-value arguments of
+ * cu_render (core/kernel.cu:471-492) minus the texture handles, which the
+ * context owns. transform is the row-major float[9] that the reference copies
+ * into float3x3 (core/CuDataManager.cpp:85-88).
+ *
+ * step_scale and exit_alpha parametrise the marcher: the reference rule is
+ * step_scale = 1 (base step 1/depth, core/kernel.cu:169) and
+ * exit_alpha = 0.995 (core/kernel.cu:248).
+ */
+typedef struct {
+    int width;
+    int height;
+    int render_type;           /* mk_render_type */
+    int invert_z;              /* VolumeInfo::Need2InvertZ */
+    float max_len_spacing[3];  /* DataManager::InitCommon f3maxLenSpacing */
+    float spacing[3];
+    float spacing_voxel[3];    /* 1/dim per axis */
+    float x_translate;
+    float y_translate;
+    float scale;
+    float transform[9];
+    mk_voi voi;
+    float background[4];       /* RGBA background, core/kernel.cu:500 */
+    mk_alpha_wwwl alpha_wwwl[MK_NUM_LABELS];
+    float step_scale;
+    float exit_alpha;
+} mk_render_params;
+
+/*
+ * Arguments of one slab-MPR image: cu_renderPlane_{MIP,MinIP,Average}
+ * (core/kernel.cu:616-631, 706-721, 795-810) minus the texture handle, plus
+ * mpr_type (which launcher) and sampling (new: nearest).
+ */
+typedef struct {
+    int width;
+    int height;
+    int mpr_type;        /* mk_mpr_type */
+    int sampling;        /* mk_sampling */
+    int invert_z;
+    float spacing[3];
+    float dir_h[3];
+    float dir_v[3];
+    float dir_n[3];
+    float pt_left_top[3];
+    float pixel_spacing;
+    float half_num;
+} mk_plane_params;
+
+/* One marching-cubes triangle: three vertices, x,y,z each, voxel units. */
+typedef struct { float v[9]; } mk_triangle;
+
+typedef struct mk_context mk_context;
+
+/* ---- device and context ------------------------------------------------ */
+
+/* DeviceInfo::GetCount (core/DeviceInfo.cpp:91-97). */
+int mk_device_count(void);
+
+/* DeviceInfo + CuDataManager construction (core/DeviceInfo.cpp:28-70,
+ * core/CuDataManager.cpp:34-44). device < 0 picks the device with the most
+ * memory, as the reference does. */
+mk_status mk_context_create(int device, mk_context** out);
+
+/* CuDataManager::~CuDataManager (core/CuDataManager.cpp:46-68). */
+void mk_context_destroy(mk_context* ctx);
+
+/* Message of the last failing call on ctx (or of the last failing
+ * mk_context_create when ctx is NULL). Never NULL. */
+const char* mk_last_error(const mk_context* ctx);
+
+/* The context's HIP stream as an opaque handle (hipStream_t). */
+void* mk_context_stream(mk_context* ctx);
+
+/* Blocks until all work queued on the context's stream is done. */
+mk_status mk_synchronize(mk_context* ctx);
+
+/* ---- resident data ----------------------------------------------------- */
+
+/* cu_copyVolumeData (core/CuDataManager.cu:28-73). host is int16, x fastest
+ * (index z*nx*ny + y*nx + x), as the reference's make_cudaPitchedPtr layout.
+ * Builds 8^3 bricks and per-brick min/max on the device. Drops any mask. */
+mk_status mk_copy_volume_data(mk_context* ctx, const int16_t* host, int nx, int ny, int nz);
+
+/* Same, from a device buffer of the same layout (already in HBM). */
+mk_status mk_copy_volume_data_device(mk_context* ctx, const int16_t* dev, int nx, int ny, int nz);
+
+/* cu_copyMaskData (core/CuDataManager.cu:75-120). host is uint8 labels in
+ * the volume's layout and dims; every value must be <= MK_MAX_OBJECTS. */
+mk_status mk_copy_mask_data(mk_context* ctx, const uint8_t* host, int nx, int ny, int nz);
+mk_status mk_copy_mask_data_device(mk_context* ctx, const uint8_t* dev, int nx, int ny, int nz);
+
+/* Removes the mask (reference: mask texture 0 => every sample has label 0). */
+mk_status mk_clear_mask(mk_context* ctx);
+
+/* cu_setTransferFunc (core/CuDataManager.cu:122-163) with the label of
+ * CuDataManager::SetTransferFunction (core/CuDataManager.cpp:80-83).
+ * rgba holds n RGBA float entries. A label with no LUT reads as all zero. */
+mk_status mk_set_transfer_func(mk_context* ctx, int label, const float* rgba, int n);
+
+/* ---- rendering ---------------------------------------------------------- */
+
+/* cu_render (core/kernel.cu:471-568): renders into caller-owned host memory
+ * of width*height*3 bytes (RGB24), synchronously. */
+mk_status mk_render(mk_context* ctx, const mk_render_params* p, uint8_t* host_rgb);
+
+/* Same frame into device memory (width*height*3 bytes) on the given stream
+ * (NULL = the context's stream); returns after enqueueing. */
+mk_status mk_render_async(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb, void* stream);
+
+/* cu_renderPlane_MIP / _MinIP / _Average (core/kernel.cu:616-837): int16
+ * width*height output in caller-owned host memory, synchronously. */
+mk_status mk_render_plane(mk_context* ctx, const mk_plane_params* p, int16_t* host_out);
+mk_status mk_render_plane_async(mk_context* ctx, const mk_plane_params* p, int16_t* dev_out, void* stream);
+
+/* cu_renderCPR (core/kernel.cu:875-918): row y samples points[y] + x*dirs[y]
+ * (voxel units, 3 doubles each, height rows). Host pointers, synchronous. */
+mk_status mk_render_cpr(mk_context* ctx, int width, int height, const double* points,
+                        const double* dirs, int invert_z, int sampling, int16_t* host_out);
+/* Same with device-resident points/dirs/output, on the given stream. */
+mk_status mk_render_cpr_async(mk_context* ctx, int width, int height, const double* dev_points,
+                              const double* dev_dirs, int invert_z, int sampling,
+                              int16_t* dev_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MONKEYGL_HIP_H */

@@ -1,0 +1,202 @@
+"""ctypes binding of the C ABI in ``include/monkeygl_hip.h``.
+
+The structs mirror the header field for field. ``load()`` raises if the HIP
+library has not been built: there is no CPU fallback anywhere in the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libmonkeygl_hip.so"
+
+MK_MAX_OBJECTS = 64
+MK_NUM_LABELS = MK_MAX_OBJECTS + 1
+
+MK_OK, MK_ERR_ARG, MK_ERR_STATE, MK_ERR_HIP, MK_ERR_NOMEM, MK_ERR_RANGE = range(6)
+MK_RENDER_VR, MK_RENDER_MIP, MK_RENDER_SURFACE = range(3)
+MK_MPR_AVERAGE, MK_MPR_MIP, MK_MPR_MINIP = range(3)
+MK_SAMPLE_LINEAR, MK_SAMPLE_NEAREST = range(2)
+
+# exported symbols, in header order
+SYMBOLS = [
+    "mk_device_count", "mk_context_create", "mk_context_destroy", "mk_last_error",
+    "mk_context_stream", "mk_synchronize", "mk_copy_volume_data", "mk_copy_volume_data_device",
+    "mk_copy_mask_data", "mk_copy_mask_data_device", "mk_clear_mask", "mk_set_transfer_func",
+    "mk_render", "mk_render_async", "mk_render_plane", "mk_render_plane_async",
+    "mk_render_cpr", "mk_render_cpr_async",
+]
+
+
+class AlphaWWWL(C.Structure):
+    _fields_ = [("alpha", C.c_float), ("ww", C.c_float), ("wl", C.c_float)]
+
+
+class VOI(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("left", "right", "posterior", "anterior", "head", "foot")]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("render_type", C.c_int), ("invert_z", C.c_int),
+        ("max_len_spacing", C.c_float * 3), ("spacing", C.c_float * 3), ("spacing_voxel", C.c_float * 3),
+        ("x_translate", C.c_float), ("y_translate", C.c_float), ("scale", C.c_float),
+        ("transform", C.c_float * 9), ("voi", VOI), ("background", C.c_float * 4),
+        ("alpha_wwwl", AlphaWWWL * MK_NUM_LABELS), ("step_scale", C.c_float), ("exit_alpha", C.c_float),
+    ]
+
+
+class PlaneParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("mpr_type", C.c_int), ("sampling", C.c_int),
+        ("invert_z", C.c_int), ("spacing", C.c_float * 3), ("dir_h", C.c_float * 3), ("dir_v", C.c_float * 3),
+        ("dir_n", C.c_float * 3), ("pt_left_top", C.c_float * 3), ("pixel_spacing", C.c_float),
+        ("half_num", C.c_float),
+    ]
+
+
+class MkError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"mk status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads the HIP library; raises when it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m monkeygl_amd.build`")
+    lib = C.CDLL(str(LIB_PATH))
+    vp, i, st = C.c_void_p, C.c_int, C.c_int
+    sig = {
+        "mk_device_count": (i, []),
+        "mk_context_create": (st, [i, C.POINTER(vp)]),
+        "mk_context_destroy": (None, [vp]),
+        "mk_last_error": (C.c_char_p, [vp]),
+        "mk_context_stream": (vp, [vp]),
+        "mk_synchronize": (st, [vp]),
+        "mk_copy_volume_data": (st, [vp, vp, i, i, i]),
+        "mk_copy_volume_data_device": (st, [vp, vp, i, i, i]),
+        "mk_copy_mask_data": (st, [vp, vp, i, i, i]),
+        "mk_copy_mask_data_device": (st, [vp, vp, i, i, i]),
+        "mk_clear_mask": (st, [vp]),
+        "mk_set_transfer_func": (st, [vp, i, vp, i]),
+        "mk_render": (st, [vp, C.POINTER(RenderParams), vp]),
+        "mk_render_async": (st, [vp, C.POINTER(RenderParams), vp, vp]),
+        "mk_render_plane": (st, [vp, C.POINTER(PlaneParams), vp]),
+        "mk_render_plane_async": (st, [vp, C.POINTER(PlaneParams), vp, vp]),
+        "mk_render_cpr": (st, [vp, i, i, vp, vp, i, i, vp]),
+        "mk_render_cpr_async": (st, [vp, i, i, vp, vp, i, i, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a) -> int:
+    if a is None:
+        return 0
+    if isinstance(a, int):
+        return a
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return int(a.data_ptr())  # torch tensor on the device
+
+
+class Context:
+    """One render context: resident volume, mask and LUTs on one device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        st = self.lib.mk_context_create(device, C.byref(h))
+        if st != MK_OK:
+            raise MkError(st, self.lib.mk_last_error(None).decode())
+        self.h = h
+        self.shape = None
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.mk_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int) -> None:
+        if st != MK_OK:
+            raise MkError(st, self.lib.mk_last_error(self.h).decode())
+
+    @property
+    def stream(self) -> int:
+        return self.lib.mk_context_stream(self.h) or 0
+
+    def synchronize(self) -> None:
+        self._check(self.lib.mk_synchronize(self.h))
+
+    def set_volume(self, vol_zyx: np.ndarray) -> None:
+        """vol_zyx: int16 array indexed [z, y, x] (x fastest in memory)."""
+        v = np.ascontiguousarray(vol_zyx, dtype=np.int16)
+        nz, ny, nx = v.shape
+        self._check(self.lib.mk_copy_volume_data(self.h, _ptr(v), nx, ny, nz))
+        self.shape = (nz, ny, nx)
+
+    def set_volume_device(self, t, shape_zyx) -> None:
+        nz, ny, nx = shape_zyx
+        self._check(self.lib.mk_copy_volume_data_device(self.h, _ptr(t), nx, ny, nz))
+        self.shape = (nz, ny, nx)
+
+    def set_mask(self, mask_zyx: np.ndarray) -> None:
+        m = np.ascontiguousarray(mask_zyx, dtype=np.uint8)
+        nz, ny, nx = m.shape
+        self._check(self.lib.mk_copy_mask_data(self.h, _ptr(m), nx, ny, nz))
+
+    def clear_mask(self) -> None:
+        self._check(self.lib.mk_clear_mask(self.h))
+
+    def set_lut(self, label: int, lut: np.ndarray) -> None:
+        a = np.ascontiguousarray(lut, dtype=np.float32).reshape(-1, 4)
+        self._check(self.lib.mk_set_transfer_func(self.h, label, _ptr(a), a.shape[0]))
+
+    def render(self, p: RenderParams) -> np.ndarray:
+        out = np.empty((p.height, p.width, 3), np.uint8)
+        self._check(self.lib.mk_render(self.h, C.byref(p), _ptr(out)))
+        return out
+
+    def render_async(self, p: RenderParams, dev_out, stream: int = 0) -> None:
+        self._check(self.lib.mk_render_async(self.h, C.byref(p), _ptr(dev_out), stream or None))
+
+    def render_plane(self, p: PlaneParams) -> np.ndarray:
+        out = np.empty((p.height, p.width), np.int16)
+        self._check(self.lib.mk_render_plane(self.h, C.byref(p), _ptr(out)))
+        return out
+
+    def render_plane_async(self, p: PlaneParams, dev_out, stream: int = 0) -> None:
+        self._check(self.lib.mk_render_plane_async(self.h, C.byref(p), _ptr(dev_out), stream or None))
+
+    def render_cpr(self, width: int, points: np.ndarray, dirs: np.ndarray, invert_z: bool,
+                   sampling: int = MK_SAMPLE_LINEAR) -> np.ndarray:
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        drs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        h = pts.shape[0]
+        out = np.empty((h, width), np.int16)
+        self._check(self.lib.mk_render_cpr(self.h, width, h, _ptr(pts), _ptr(drs), int(invert_z), sampling, _ptr(out)))
+        return out
+
+    def render_cpr_async(self, width: int, height: int, dev_points, dev_dirs, invert_z: bool, sampling: int,
+                         dev_out, stream: int = 0) -> None:
+        self._check(self.lib.mk_render_cpr_async(self.h, width, height, _ptr(dev_points), _ptr(dev_dirs),
+                                                 int(invert_z), sampling, _ptr(dev_out), stream or None))

@@ -1,0 +1,102 @@
+// mk_context.h -- host-side state of one render context (internal).
+//
+// One context replaces the reference's CuDataManager (core/CuDataManager.h:42-83):
+// it owns the resident volume, mask and LUTs on one device and one HIP stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/monkeygl_hip.h"
+#include "mk_device.h"
+
+// Per-brick statistics used for empty-space skipping. vmin/vmax cover the
+// brick plus a one-voxel apron on every side (the trilinear footprint of any
+// sample whose truncated index lies in the brick); labels covers the brick's
+// own voxels (bit l = label l present, bit 64 in lab_hi).
+struct MkBrickStats {
+    int16_t* vmin = nullptr;
+    int16_t* vmax = nullptr;
+    uint64_t* lab_lo = nullptr;
+    uint8_t* lab_hi = nullptr;
+};
+
+// LUT layout on the device: all labels' RGBA entries packed as float4, label l
+// at [off[l], off[l]+len[l]); nzp holds, per label, len+1 prefix counts of
+// entries with nonzero alpha (at offset off[l] + l).
+struct MkLutTable {
+    int off[MK_NUM_LABELS];
+    int len[MK_NUM_LABELS];
+};
+
+struct mk_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // volume
+    int nx = 0, ny = 0, nz = 0;
+    int nbx = 0, nby = 0, nbz = 0;
+    int16_t* d_vox = nullptr;
+    uint8_t* d_mask = nullptr;
+    MkBrickStats stats;
+    uint64_t used_lo = 1;   // labels present in the mask (label 0 without a mask)
+    bool used_hi = false;
+
+    // transfer functions
+    std::vector<float> h_lut[MK_NUM_LABELS];
+    bool lut_dirty = true;
+    float4* d_lut = nullptr;
+    int* d_lutnz = nullptr;
+    size_t lut_cap = 0, lutnz_cap = 0;
+    MkLutTable lut_tab{};
+    uint32_t lut_version = 1;
+
+    // per-frame skip tables (cached on their inputs)
+    uint8_t* d_occ = nullptr;     // VR / Surface: brick may contribute
+    float* d_mipb = nullptr;      // MIP: upper bound of the windowed value
+    bool occ_valid = false;
+    int occ_type = -1;
+    uint32_t occ_lut_version = 0;
+    uint32_t occ_vol_version = 0;
+    mk_alpha_wwwl occ_aw[MK_NUM_LABELS];
+    uint32_t vol_version = 0;
+
+    // exact float step lattice a_{k+1} = a_k + L1 from a_0 = 0
+    float* d_steps = nullptr;
+    size_t steps_cap = 0;
+    float steps_l1 = -1.0f;
+    int steps_n = 0;
+
+    // staging for synchronous host-output calls
+    void* d_stage = nullptr;
+    size_t stage_cap = 0;
+    void* d_stage2 = nullptr;
+    size_t stage2_cap = 0;
+
+    size_t nbricks() const { return (size_t)nbx * nby * nbz; }
+    MkVolView view() const {
+        MkVolView v;
+        v.vox = d_vox;
+        v.mask = d_mask;
+        v.nx = nx; v.ny = ny; v.nz = nz;
+        v.nbx = nbx; v.nby = nby; v.nbz = nbz;
+        return v;
+    }
+};
+
+// error helpers shared by the translation units
+mk_status mk_fail(mk_context* ctx, mk_status st, const char* fmt, ...);
+mk_status mk_hip_fail(mk_context* ctx, hipError_t e, const char* what);
+mk_status mk_ensure_buffer(mk_context* ctx, void** buf, size_t* cap, size_t bytes);
+mk_status mk_upload_luts(mk_context* ctx);
+
+#define MK_HIP(ctx, call)                                          \
+    do {                                                           \
+        hipError_t _e = (call);                                    \
+        if (_e != hipSuccess) return mk_hip_fail((ctx), _e, #call); \
+    } while (0)

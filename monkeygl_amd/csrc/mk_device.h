@@ -1,0 +1,104 @@
+// mk_device.h -- device-side layout and sampling of the bricked volume.
+//
+// HBM layout. The int16 volume (x fastest on the host, index z*nx*ny+y*nx+x)
+// is stored as 8x8x8 bricks of 1 KiB: bricks are ordered x, then y, then z,
+// and voxels inside a brick x, then y, then z. One brick is eight 128-byte
+// cache lines, so a trilinear footprint (2x2x2) touches at most 8 bricks and a
+// wave's 8x8 ray tile stepping through the volume reuses the same lines for
+// several steps whatever the ray direction. The uint8 label mask uses the same
+// brick order (512 B per brick). Offsets are 32-bit: a volume may hold up to
+// 2^32 padded voxels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MK_BRICK_SHIFT 3
+#define MK_BRICK 8
+#define MK_BRICK_VOXELS 512
+
+struct MkVolView {
+    const int16_t* vox;   // bricked int16 voxels
+    const uint8_t* mask;  // bricked uint8 labels, or null
+    int nx, ny, nz;       // voxel dims
+    int nbx, nby, nbz;    // brick dims
+};
+
+__host__ __device__ inline uint32_t mk_brick_index(const MkVolView& v, int x, int y, int z) {
+    return ((uint32_t)(z >> MK_BRICK_SHIFT) * (uint32_t)v.nby + (uint32_t)(y >> MK_BRICK_SHIFT)) * (uint32_t)v.nbx +
+           (uint32_t)(x >> MK_BRICK_SHIFT);
+}
+
+__host__ __device__ inline uint32_t mk_voxel_offset(const MkVolView& v, int x, int y, int z) {
+    return (mk_brick_index(v, x, y, z) << 9) | ((uint32_t)(z & 7) << 6) | ((uint32_t)(y & 7) << 3) | (uint32_t)(x & 7);
+}
+
+__device__ inline int mk_clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// NaN-safe saturate: NaN maps to 0.
+__device__ inline float mk_sat(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
+
+// Raw voxel as float with the normalized-float clamp of int16 (-32768 reads
+// as -32767): multiplying by 32768/32767 afterwards gives the reference's
+// 32768 * tex3D<float>() value.
+__device__ inline float mk_raw(const MkVolView& v, int x, int y, int z) {
+    int s = v.vox[mk_voxel_offset(v, x, y, z)];
+    return (float)(s < -32767 ? -32767 : s);
+}
+
+// Split of one normalized coordinate into texel indices and weight:
+// x_t = f*N - 0.5, i = floor(x_t), a = x_t - i, neighbours clamped.
+__device__ inline void mk_split(float f, int n, int& i0, int& i1, float& a) {
+    float xt = f * (float)n - 0.5f;
+    float fl = floorf(xt);
+    a = xt - fl;
+    fl = fminf(fmaxf(fl, -2.0f), (float)(n + 1));
+    int i = (int)fl;
+    i0 = mk_clampi(i, 0, n - 1);
+    i1 = mk_clampi(i + 1, 0, n - 1);
+}
+
+// Trilinear sample in HU' units (32768 * normalized-float read).
+__device__ inline float mk_sample_hu(const MkVolView& v, float fx, float fy, float fz) {
+    int x0, x1, y0, y1, z0, z1;
+    float ax, ay, az;
+    mk_split(fx, v.nx, x0, x1, ax);
+    mk_split(fy, v.ny, y0, y1, ay);
+    mk_split(fz, v.nz, z0, z1, az);
+    float c000 = mk_raw(v, x0, y0, z0), c100 = mk_raw(v, x1, y0, z0);
+    float c010 = mk_raw(v, x0, y1, z0), c110 = mk_raw(v, x1, y1, z0);
+    float c001 = mk_raw(v, x0, y0, z1), c101 = mk_raw(v, x1, y0, z1);
+    float c011 = mk_raw(v, x0, y1, z1), c111 = mk_raw(v, x1, y1, z1);
+    float c00 = c000 + ax * (c100 - c000);
+    float c10 = c010 + ax * (c110 - c010);
+    float c01 = c001 + ax * (c101 - c001);
+    float c11 = c011 + ax * (c111 - c011);
+    float c0 = c00 + ay * (c10 - c00);
+    float c1 = c01 + ay * (c11 - c01);
+    float c = c0 + az * (c1 - c0);
+    return c * (32768.0f / 32767.0f);
+}
+
+// Point-sampled HU' for nearest MPR/CPR: voxel floor(f*N) clamped, then the
+// normalized-float read v/32767 times 32768, in that op order (bit-exact with
+// the host restatement).
+__device__ inline float mk_sample_nearest_hu(const MkVolView& v, float fx, float fy, float fz) {
+    float tx = floorf(fx * (float)v.nx), ty = floorf(fy * (float)v.ny), tz = floorf(fz * (float)v.nz);
+    int x = mk_clampi((int)fminf(fmaxf(tx, -1.0f), (float)v.nx), 0, v.nx - 1);
+    int y = mk_clampi((int)fminf(fmaxf(ty, -1.0f), (float)v.ny), 0, v.ny - 1);
+    int z = mk_clampi((int)fminf(fmaxf(tz, -1.0f), (float)v.nz), 0, v.nz - 1);
+    int s = v.vox[mk_voxel_offset(v, x, y, z)];
+    float r = (float)s / 32767.0f;
+    r = r < -1.0f ? -1.0f : r;
+    return 32768.0f * r;
+}
+
+// Maps a linear workgroup id to a tile id so that consecutive tiles land on
+// the same XCD (workgroups are dispatched round-robin over the 8 XCDs, each
+// with its own L2). Bijective on [0, n).
+__device__ inline uint32_t mk_xcd_tile(uint32_t b, uint32_t n) {
+    const uint32_t X = 8;
+    uint32_t q = n / X, r = n % X;
+    uint32_t x = b % X, i = b / X;
+    return x * q + (x < r ? x : r) + i;
+}

@@ -1,0 +1,433 @@
+// mk_volume.hip -- context lifetime and the resident data of the render path:
+// bricked int16 volume, bricked uint8 label mask, per-brick statistics and the
+// transfer-function LUTs. Replaces the reference's CuDataManager
+// (core/CuDataManager.cu:28-163, core/CuDataManager.cpp:34-88) and the device
+// pick of DeviceInfo (core/DeviceInfo.cpp:28-70).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "mk_context.h"
+
+static std::mutex g_err_mu;
+static std::string g_create_err = "no error";
+
+mk_status mk_fail(mk_context* ctx, mk_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) {
+        ctx->err = buf;
+    } else {
+        std::lock_guard<std::mutex> lk(g_err_mu);
+        g_create_err = buf;
+    }
+    return st;
+}
+
+mk_status mk_hip_fail(mk_context* ctx, hipError_t e, const char* what) {
+    mk_status st = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? MK_ERR_NOMEM : MK_ERR_HIP;
+    return mk_fail(ctx, st, "%s failed: %s", what, hipGetErrorString(e));
+}
+
+mk_status mk_ensure_buffer(mk_context* ctx, void** buf, size_t* cap, size_t bytes) {
+    if (*cap >= bytes && *buf) return MK_OK;
+    if (*buf) {
+        MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        MK_HIP(ctx, hipFree(*buf));
+        *buf = nullptr;
+        *cap = 0;
+    }
+    if (bytes == 0) return MK_OK;
+    MK_HIP(ctx, hipMalloc(buf, bytes));
+    *cap = bytes;
+    return MK_OK;
+}
+
+template <typename P> static void mk_free(P*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+extern "C" int mk_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" mk_status mk_context_create(int device, mk_context** out) {
+    if (!out) return mk_fail(nullptr, MK_ERR_ARG, "mk_context_create: out is null");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return mk_fail(nullptr, MK_ERR_HIP, "mk_context_create: no HIP device (%s)", hipGetErrorString(e));
+    if (device < 0) {
+        // DeviceInfo: the device with the largest global memory
+        size_t best = 0;
+        device = 0;
+        for (int i = 0; i < n; ++i) {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, i) == hipSuccess && prop.totalGlobalMem > best) {
+                best = prop.totalGlobalMem;
+                device = i;
+            }
+        }
+    }
+    if (device >= n) return mk_fail(nullptr, MK_ERR_ARG, "mk_context_create: device %d of %d", device, n);
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return mk_fail(nullptr, MK_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+    mk_context* ctx = new mk_context();
+    ctx->device = device;
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return mk_fail(nullptr, MK_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    for (int l = 0; l < MK_NUM_LABELS; ++l) ctx->occ_aw[l] = mk_alpha_wwwl{0, 0, 0};
+    *out = ctx;
+    return MK_OK;
+}
+
+static void mk_free_volume(mk_context* ctx) {
+    mk_free(ctx->d_vox);
+    mk_free(ctx->d_mask);
+    mk_free(ctx->stats.vmin);
+    mk_free(ctx->stats.vmax);
+    mk_free(ctx->stats.lab_lo);
+    mk_free(ctx->stats.lab_hi);
+    mk_free(ctx->d_occ);
+    mk_free(ctx->d_mipb);
+    ctx->nx = ctx->ny = ctx->nz = 0;
+    ctx->occ_valid = false;
+}
+
+extern "C" void mk_context_destroy(mk_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    mk_free_volume(ctx);
+    mk_free(ctx->d_lut);
+    mk_free(ctx->d_lutnz);
+    mk_free(ctx->d_steps);
+    if (ctx->d_stage) (void)hipFree(ctx->d_stage);
+    if (ctx->d_stage2) (void)hipFree(ctx->d_stage2);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char* mk_last_error(const mk_context* ctx) {
+    if (ctx) return ctx->err.c_str();
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_create_err.c_str();
+}
+
+extern "C" void* mk_context_stream(mk_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+extern "C" mk_status mk_synchronize(mk_context* ctx) {
+    if (!ctx) return MK_ERR_ARG;
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MK_OK;
+}
+
+// ---- bricking ------------------------------------------------------------
+
+// One workgroup of 512 threads per brick: thread o writes voxel o of brick b.
+// Voxels past the volume edge are zero (the samplers clamp indices, so they
+// are never read).
+template <typename E>
+__global__ void __launch_bounds__(512) k_brickify(const E* __restrict__ lin, E* __restrict__ out, int nx, int ny, int nz,
+                                                  int nbx, int nby) {
+    uint32_t b = blockIdx.x;
+    uint32_t o = threadIdx.x;
+    int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    int x = bx * 8 + (o & 7), y = by * 8 + ((o >> 3) & 7), z = bz * 8 + (o >> 6);
+    E v = 0;
+    if (x < nx && y < ny && z < nz) v = lin[((size_t)z * ny + y) * nx + x];
+    out[((size_t)b << 9) | o] = v;
+}
+
+// One wave per brick: min/max over the brick plus a one-voxel apron.
+__global__ void __launch_bounds__(256) k_brick_minmax(MkVolView v, int16_t* __restrict__ vmin, int16_t* __restrict__ vmax,
+                                                       uint32_t nbricks) {
+    uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (b >= nbricks) return;
+    int bx = b % v.nbx, by = (b / v.nbx) % v.nby, bz = b / (v.nbx * v.nby);
+    int lo = 32767, hi = -32768;
+    // 10x10x10 apron box, clamped to the volume
+    for (int i = lane; i < 1000; i += 64) {
+        int dx = i % 10, dy = (i / 10) % 10, dz = i / 100;
+        int x = mk_clampi(bx * 8 - 1 + dx, 0, v.nx - 1);
+        int y = mk_clampi(by * 8 - 1 + dy, 0, v.ny - 1);
+        int z = mk_clampi(bz * 8 - 1 + dz, 0, v.nz - 1);
+        int s = v.vox[mk_voxel_offset(v, x, y, z)];
+        lo = min(lo, s);
+        hi = max(hi, s);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off));
+        hi = max(hi, __shfl_xor(hi, off));
+    }
+    if (lane == 0) {
+        vmin[b] = (int16_t)lo;
+        vmax[b] = (int16_t)hi;
+    }
+}
+
+// One wave per brick: label bits of the brick's own voxels. flags[0..1]
+// accumulate the union (lo 64 bits, bit 0 of flags[2] = label 64) and
+// flags[3] != 0 flags a label above MK_MAX_OBJECTS.
+__global__ void __launch_bounds__(256) k_brick_labels(MkVolView v, uint64_t* __restrict__ lab_lo, uint8_t* __restrict__ lab_hi,
+                                                       uint32_t nbricks, unsigned long long* __restrict__ flags) {
+    uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (b >= nbricks) return;
+    int bx = b % v.nbx, by = (b / v.nbx) % v.nby, bz = b / (v.nbx * v.nby);
+    uint32_t lo0 = 0, lo1 = 0, hi = 0, bad = 0;
+    for (int o = lane; o < 512; o += 64) {
+        int x = bx * 8 + (o & 7), y = by * 8 + ((o >> 3) & 7), z = bz * 8 + (o >> 6);
+        if (x >= v.nx || y >= v.ny || z >= v.nz) continue;
+        uint32_t l = v.mask[((size_t)b << 9) | o];
+        if (l < 32) lo0 |= 1u << l;
+        else if (l < 64) lo1 |= 1u << (l - 32);
+        else if (l == 64) hi = 1;
+        else bad = 1;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo0 |= __shfl_xor(lo0, off);
+        lo1 |= __shfl_xor(lo1, off);
+        hi |= __shfl_xor(hi, off);
+        bad |= __shfl_xor(bad, off);
+    }
+    if (lane == 0) {
+        uint64_t lo = ((uint64_t)lo1 << 32) | lo0;
+        lab_lo[b] = lo;
+        lab_hi[b] = (uint8_t)hi;
+        if (lo) atomicOr(&flags[0], (unsigned long long)lo);
+        if (hi) atomicOr(&flags[2], 1ull);
+        if (bad) atomicOr(&flags[3], 1ull);
+    }
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t val, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = val;
+}
+__global__ void k_fill_u8(uint8_t* p, uint8_t val, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = val;
+}
+
+static mk_status mk_check_dims(mk_context* ctx, int nx, int ny, int nz) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) return mk_fail(ctx, MK_ERR_ARG, "volume dims must be positive (%d,%d,%d)", nx, ny, nz);
+    size_t nb = (size_t)((nx + 7) / 8) * ((ny + 7) / 8) * ((nz + 7) / 8);
+    if (nb * 512 > 0xFFFFFFFFull) return mk_fail(ctx, MK_ERR_ARG, "volume too large: %zu padded voxels (limit 2^32)", nb * 512);
+    return MK_OK;
+}
+
+static mk_status mk_volume_from_device(mk_context* ctx, const int16_t* d_lin, int nx, int ny, int nz) {
+    mk_free_volume(ctx);
+    ctx->nx = nx; ctx->ny = ny; ctx->nz = nz;
+    ctx->nbx = (nx + 7) / 8; ctx->nby = (ny + 7) / 8; ctx->nbz = (nz + 7) / 8;
+    size_t nb = ctx->nbricks();
+    MK_HIP(ctx, hipMalloc((void**)&ctx->d_vox, nb * 512 * sizeof(int16_t)));
+    MK_HIP(ctx, hipMalloc((void**)&ctx->stats.vmin, nb * sizeof(int16_t)));
+    MK_HIP(ctx, hipMalloc((void**)&ctx->stats.vmax, nb * sizeof(int16_t)));
+    MK_HIP(ctx, hipMalloc((void**)&ctx->stats.lab_lo, nb * sizeof(uint64_t)));
+    MK_HIP(ctx, hipMalloc((void**)&ctx->stats.lab_hi, nb * sizeof(uint8_t)));
+    MK_HIP(ctx, hipMalloc((void**)&ctx->d_occ, nb * sizeof(uint8_t)));
+    MK_HIP(ctx, hipMalloc((void**)&ctx->d_mipb, nb * sizeof(float)));
+    hipLaunchKernelGGL(k_brickify<int16_t>, dim3((uint32_t)nb), dim3(512), 0, ctx->stream, d_lin, ctx->d_vox, nx, ny, nz,
+                       ctx->nbx, ctx->nby);
+    MK_HIP(ctx, hipGetLastError());
+    MkVolView v = ctx->view();
+    hipLaunchKernelGGL(k_brick_minmax, dim3((uint32_t)((nb + 3) / 4)), dim3(256), 0, ctx->stream, v, ctx->stats.vmin,
+                       ctx->stats.vmax, (uint32_t)nb);
+    MK_HIP(ctx, hipGetLastError());
+    // no mask: every brick holds label 0 only
+    hipLaunchKernelGGL(k_fill_u64, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.lab_lo,
+                       (uint64_t)1, (uint32_t)nb);
+    hipLaunchKernelGGL(k_fill_u8, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.lab_hi,
+                       (uint8_t)0, (uint32_t)nb);
+    MK_HIP(ctx, hipGetLastError());
+    ctx->used_lo = 1;
+    ctx->used_hi = false;
+    ctx->vol_version++;
+    ctx->occ_valid = false;
+    return MK_OK;
+}
+
+extern "C" mk_status mk_copy_volume_data(mk_context* ctx, const int16_t* host, int nx, int ny, int nz) {
+    if (!ctx) return MK_ERR_ARG;
+    if (!host) return mk_fail(ctx, MK_ERR_ARG, "mk_copy_volume_data: null data");
+    mk_status st = mk_check_dims(ctx, nx, ny, nz);
+    if (st) return st;
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    size_t bytes = (size_t)nx * ny * nz * sizeof(int16_t);
+    void* tmp = nullptr;
+    MK_HIP(ctx, hipMalloc(&tmp, bytes));
+    hipError_t e = hipMemcpyAsync(tmp, host, bytes, hipMemcpyHostToDevice, ctx->stream);
+    st = e == hipSuccess ? mk_volume_from_device(ctx, (const int16_t*)tmp, nx, ny, nz) : mk_hip_fail(ctx, e, "hipMemcpyAsync");
+    e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp);
+    if (st) return st;
+    if (e != hipSuccess) return mk_hip_fail(ctx, e, "hipStreamSynchronize");
+    return MK_OK;
+}
+
+extern "C" mk_status mk_copy_volume_data_device(mk_context* ctx, const int16_t* dev, int nx, int ny, int nz) {
+    if (!ctx) return MK_ERR_ARG;
+    if (!dev) return mk_fail(ctx, MK_ERR_ARG, "mk_copy_volume_data_device: null data");
+    mk_status st = mk_check_dims(ctx, nx, ny, nz);
+    if (st) return st;
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    st = mk_volume_from_device(ctx, dev, nx, ny, nz);
+    if (st) return st;
+    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MK_OK;
+}
+
+static mk_status mk_mask_from_device(mk_context* ctx, const uint8_t* d_lin, int nx, int ny, int nz) {
+    if (!ctx->d_vox) return mk_fail(ctx, MK_ERR_STATE, "mask upload before any volume");
+    if (nx != ctx->nx || ny != ctx->ny || nz != ctx->nz)
+        return mk_fail(ctx, MK_ERR_ARG, "mask dims (%d,%d,%d) differ from volume (%d,%d,%d)", nx, ny, nz, ctx->nx, ctx->ny, ctx->nz);
+    size_t nb = ctx->nbricks();
+    uint8_t* d_mask = nullptr;
+    MK_HIP(ctx, hipMalloc((void**)&d_mask, nb * 512));
+    hipLaunchKernelGGL(k_brickify<uint8_t>, dim3((uint32_t)nb), dim3(512), 0, ctx->stream, d_lin, d_mask, nx, ny, nz,
+                       ctx->nbx, ctx->nby);
+    MK_HIP(ctx, hipGetLastError());
+    unsigned long long* d_flags = nullptr;
+    MK_HIP(ctx, hipMalloc((void**)&d_flags, 4 * sizeof(unsigned long long)));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, (uint64_t*)d_flags, (uint64_t)0, 4u);
+    MkVolView v = ctx->view();
+    v.mask = d_mask;
+    hipLaunchKernelGGL(k_brick_labels, dim3((uint32_t)((nb + 3) / 4)), dim3(256), 0, ctx->stream, v, ctx->stats.lab_lo,
+                       ctx->stats.lab_hi, (uint32_t)nb, d_flags);
+    MK_HIP(ctx, hipGetLastError());
+    unsigned long long flags[4];
+    MK_HIP(ctx, hipMemcpyAsync(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
+    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(d_flags);
+    if (flags[3]) {
+        (void)hipFree(d_mask);
+        // restore the no-mask label statistics
+        hipLaunchKernelGGL(k_fill_u64, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.lab_lo,
+                           (uint64_t)1, (uint32_t)nb);
+        hipLaunchKernelGGL(k_fill_u8, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.lab_hi,
+                           (uint8_t)0, (uint32_t)nb);
+        if (ctx->d_mask) {
+            // a previous mask stays active: recompute its statistics
+            MkVolView v2 = ctx->view();
+            unsigned long long* d_f2 = nullptr;
+            MK_HIP(ctx, hipMalloc((void**)&d_f2, 4 * sizeof(unsigned long long)));
+            hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, (uint64_t*)d_f2, (uint64_t)0, 4u);
+            hipLaunchKernelGGL(k_brick_labels, dim3((uint32_t)((nb + 3) / 4)), dim3(256), 0, ctx->stream, v2,
+                               ctx->stats.lab_lo, ctx->stats.lab_hi, (uint32_t)nb, d_f2);
+            MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            (void)hipFree(d_f2);
+        }
+        MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return mk_fail(ctx, MK_ERR_RANGE, "mask holds a label above %d", MK_MAX_OBJECTS);
+    }
+    if (ctx->d_mask) (void)hipFree(ctx->d_mask);
+    ctx->d_mask = d_mask;
+    ctx->used_lo = flags[0];
+    ctx->used_hi = flags[2] != 0;
+    ctx->vol_version++;
+    ctx->occ_valid = false;
+    return MK_OK;
+}
+
+extern "C" mk_status mk_copy_mask_data(mk_context* ctx, const uint8_t* host, int nx, int ny, int nz) {
+    if (!ctx) return MK_ERR_ARG;
+    if (!host) return mk_fail(ctx, MK_ERR_ARG, "mk_copy_mask_data: null data");
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    if (nx <= 0 || ny <= 0 || nz <= 0) return mk_fail(ctx, MK_ERR_ARG, "mask dims must be positive");
+    size_t bytes = (size_t)nx * ny * nz;
+    void* tmp = nullptr;
+    MK_HIP(ctx, hipMalloc(&tmp, bytes));
+    hipError_t e = hipMemcpyAsync(tmp, host, bytes, hipMemcpyHostToDevice, ctx->stream);
+    mk_status st = e == hipSuccess ? mk_mask_from_device(ctx, (const uint8_t*)tmp, nx, ny, nz) : mk_hip_fail(ctx, e, "hipMemcpyAsync");
+    e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp);
+    if (st) return st;
+    if (e != hipSuccess) return mk_hip_fail(ctx, e, "hipStreamSynchronize");
+    return MK_OK;
+}
+
+extern "C" mk_status mk_copy_mask_data_device(mk_context* ctx, const uint8_t* dev, int nx, int ny, int nz) {
+    if (!ctx) return MK_ERR_ARG;
+    if (!dev) return mk_fail(ctx, MK_ERR_ARG, "mk_copy_mask_data_device: null data");
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    return mk_mask_from_device(ctx, dev, nx, ny, nz);
+}
+
+extern "C" mk_status mk_clear_mask(mk_context* ctx) {
+    if (!ctx) return MK_ERR_ARG;
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    if (!ctx->d_mask) return MK_OK;
+    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    mk_free(ctx->d_mask);
+    size_t nb = ctx->nbricks();
+    hipLaunchKernelGGL(k_fill_u64, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.lab_lo,
+                       (uint64_t)1, (uint32_t)nb);
+    hipLaunchKernelGGL(k_fill_u8, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.lab_hi,
+                       (uint8_t)0, (uint32_t)nb);
+    MK_HIP(ctx, hipGetLastError());
+    ctx->used_lo = 1;
+    ctx->used_hi = false;
+    ctx->vol_version++;
+    ctx->occ_valid = false;
+    return MK_OK;
+}
+
+// ---- transfer functions ---------------------------------------------------
+
+extern "C" mk_status mk_set_transfer_func(mk_context* ctx, int label, const float* rgba, int n) {
+    if (!ctx) return MK_ERR_ARG;
+    if (label < 0 || label > MK_MAX_OBJECTS) return mk_fail(ctx, MK_ERR_ARG, "label %d out of range", label);
+    if (n < 0 || (n > 0 && !rgba)) return mk_fail(ctx, MK_ERR_ARG, "bad LUT (n=%d)", n);
+    if (n > (1 << 20)) return mk_fail(ctx, MK_ERR_ARG, "LUT too long (%d entries)", n);
+    ctx->h_lut[label].assign(rgba, rgba + 4 * (size_t)n);
+    ctx->lut_dirty = true;
+    return MK_OK;
+}
+
+mk_status mk_upload_luts(mk_context* ctx) {
+    if (!ctx->lut_dirty && ctx->d_lut) return MK_OK;
+    size_t total = 0, total_nz = 0;
+    for (int l = 0; l < MK_NUM_LABELS; ++l) {
+        int len = (int)(ctx->h_lut[l].size() / 4);
+        ctx->lut_tab.off[l] = (int)total;
+        ctx->lut_tab.len[l] = len;
+        total += len;
+        total_nz += len + 1;
+    }
+    std::vector<float> packed(4 * (total ? total : 1), 0.0f);
+    std::vector<int> nz(total_nz ? total_nz : 1, 0);
+    for (int l = 0; l < MK_NUM_LABELS; ++l) {
+        int len = ctx->lut_tab.len[l];
+        if (len) memcpy(&packed[4 * (size_t)ctx->lut_tab.off[l]], ctx->h_lut[l].data(), 16 * (size_t)len);
+        int* p = &nz[(size_t)ctx->lut_tab.off[l] + l];
+        p[0] = 0;
+        for (int i = 0; i < len; ++i) p[i + 1] = p[i] + (ctx->h_lut[l][4 * (size_t)i + 3] != 0.0f ? 1 : 0);
+    }
+    mk_status st = mk_ensure_buffer(ctx, (void**)&ctx->d_lut, &ctx->lut_cap, packed.size() * sizeof(float));
+    if (st) return st;
+    st = mk_ensure_buffer(ctx, (void**)&ctx->d_lutnz, &ctx->lutnz_cap, nz.size() * sizeof(int));
+    if (st) return st;
+    MK_HIP(ctx, hipMemcpyAsync(ctx->d_lut, packed.data(), packed.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    MK_HIP(ctx, hipMemcpyAsync(ctx->d_lutnz, nz.data(), nz.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    // host vectors die at return: wait for the copies
+    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->lut_dirty = false;
+    ctx->lut_version++;
+    return MK_OK;
+}

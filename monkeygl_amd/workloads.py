@@ -1,0 +1,158 @@
+"""Seeded synthetic workloads of the render path (no datasets are available).
+
+Volumes are int16 arrays indexed [z, y, x] (x fastest in memory, the layout
+the reference's SetVolumeArray hands to the device) with the reference's
+default orientation (direction z = (0, 0, -1), so VolumeInfo::Need2InvertZ
+is false).
+
+* ``blob_volume``    -- Gaussian blobs on a -1024 background plus integer
+                        uniform noise, clamped to [-1024, 3071].
+* ``body_volume``    -- blob "chambers" plus a network of cubic-spline vessels
+                        (radius in mm, constant HU per vessel).
+* ``ellipsoid_mask`` -- uint8 label mask of seeded ellipsoids.
+* ``tf_points``      -- seeded transfer-function control points.
+* ``centerline``     -- seeded cubic-spline centreline for CPR.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+BACKGROUND = -1024
+HU_MIN, HU_MAX = -1024, 3071
+
+
+@dataclass
+class VolumeSpec:
+    name: str
+    dims: tuple          # (nx, ny, nz)
+    spacing: tuple       # mm (x, y, z)
+    n_blobs: int
+    sigma: tuple         # voxel units (lo, hi)
+    amplitude: tuple
+    n_vessels: int = 0
+    seed: int = 0
+
+
+SMALL = VolumeSpec("blobs128", (128, 128, 128), (1.0, 1.0, 1.0), 8, (4, 20), (200, 2000))
+CARDIAC = VolumeSpec("cardiac512x512x361", (512, 512, 361), (0.351, 0.351, 0.3), 12, (8, 40), (800, 1400), 40)
+BODY = VolumeSpec("body512x512x1559", (512, 512, 1559), (0.7422, 0.7422, 1.0), 24, (8, 40), (800, 1400), 40)
+TF_FIT = VolumeSpec("blobs256", (256, 256, 256), (1.0, 1.0, 1.0), 16, (4, 20), (200, 2000))
+
+
+def _add_blob(vol: np.ndarray, c, sigma: float, amp: float) -> None:
+    nz, ny, nx = vol.shape
+    r = int(np.ceil(3.5 * sigma))
+    lo = [max(0, int(np.floor(c[i])) - r) for i in range(3)]
+    hi = [min((nx, ny, nz)[i], int(np.floor(c[i])) + r + 1) for i in range(3)]
+    if any(lo[i] >= hi[i] for i in range(3)):
+        return
+    x = (np.arange(lo[0], hi[0], dtype=np.float32) - np.float32(c[0])) ** 2
+    y = (np.arange(lo[1], hi[1], dtype=np.float32) - np.float32(c[1])) ** 2
+    z = (np.arange(lo[2], hi[2], dtype=np.float32) - np.float32(c[2])) ** 2
+    inv = np.float32(-0.5 / (sigma * sigma))
+    g = np.exp(z[:, None, None] * inv) * np.exp(y[None, :, None] * inv) * np.exp(x[None, None, :] * inv)
+    vol[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] += np.float32(amp) * g
+
+
+def _spline(ctrl: np.ndarray, n: int) -> np.ndarray:
+    """Catmull-Rom spline through the control points, n samples."""
+    P = np.vstack([ctrl[:1], ctrl, ctrl[-1:]])
+    segs = len(ctrl) - 1
+    t = np.linspace(0, segs, n, endpoint=False)
+    i = np.minimum(t.astype(int), segs - 1)
+    u = (t - i)[:, None]
+    p0, p1, p2, p3 = P[i], P[i + 1], P[i + 2], P[i + 3]
+    out = 0.5 * ((2 * p1) + (-p0 + p2) * u + (2 * p0 - 5 * p1 + 4 * p2 - p3) * u ** 2 + (-p0 + 3 * p1 - 3 * p2 + p3) * u ** 3)
+    return np.vstack([out, ctrl[-1:]])
+
+
+def _add_vessel(vol: np.ndarray, pts_vox: np.ndarray, radius_mm: float, hu: float, spacing) -> None:
+    nz, ny, nx = vol.shape
+    sp = np.asarray(spacing, np.float32)
+    rv = np.ceil(radius_mm / sp).astype(int)
+    for p in pts_vox:
+        c = np.floor(p).astype(int)
+        lo = np.maximum(c - rv, 0)
+        hi = np.minimum(c + rv + 2, (nx, ny, nz))
+        if np.any(lo >= hi):
+            continue
+        x = ((np.arange(lo[0], hi[0]) - p[0]) * sp[0]) ** 2
+        y = ((np.arange(lo[1], hi[1]) - p[1]) * sp[1]) ** 2
+        z = ((np.arange(lo[2], hi[2]) - p[2]) * sp[2]) ** 2
+        inside = (z[:, None, None] + y[None, :, None] + x[None, None, :]) <= radius_mm * radius_mm
+        sub = vol[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]
+        np.maximum(sub, np.where(inside, np.float32(hu), np.float32(-np.inf)), out=sub)
+
+
+def make_volume(spec: VolumeSpec, seed: int | None = None) -> np.ndarray:
+    rng = np.random.default_rng(spec.seed if seed is None else seed)
+    nx, ny, nz = spec.dims
+    vol = np.full((nz, ny, nx), np.float32(BACKGROUND), np.float32)
+    margin = np.array([nx, ny, nz], float) / 8.0
+    for _ in range(spec.n_blobs):
+        c = rng.uniform(margin, np.array([nx, ny, nz]) - margin)
+        sigma = rng.uniform(*spec.sigma)
+        amp = rng.uniform(*spec.amplitude)
+        _add_blob(vol, c, sigma, amp)
+    for _ in range(spec.n_vessels):
+        k = rng.integers(5, 9)
+        ctrl = rng.uniform(margin, np.array([nx, ny, nz]) - margin, size=(k, 3))
+        # sample densely enough that consecutive points are < 0.5 voxel apart
+        length = np.sum(np.linalg.norm(np.diff(ctrl, axis=0), axis=1))
+        pts = _spline(ctrl, int(length * 2.5) + 8)
+        _add_vessel(vol, pts, rng.uniform(1.0, 4.0), rng.uniform(300.0, 600.0), spec.spacing)
+    noise = rng.integers(-20, 21, size=vol.shape, dtype=np.int16)
+    out = np.rint(vol)
+    np.clip(out, HU_MIN, HU_MAX, out=out)
+    out = out.astype(np.int16)
+    out += noise
+    np.clip(out, HU_MIN, HU_MAX, out=out)
+    return out
+
+
+def ellipsoid_mask(dims, n_labels: int = 4, seed: int = 1) -> np.ndarray:
+    """Labels 1..n_labels as seeded axis-aligned ellipsoids (later labels win)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dims
+    m = np.zeros((nz, ny, nx), np.uint8)
+    for lab in range(1, n_labels + 1):
+        c = rng.uniform(0.25, 0.75, 3) * np.array([nx, ny, nz])
+        r = rng.uniform(0.08, 0.22, 3) * np.array([nx, ny, nz])
+        lo = np.maximum(np.floor(c - r).astype(int), 0)
+        hi = np.minimum(np.ceil(c + r).astype(int) + 1, [nx, ny, nz])
+        x = ((np.arange(lo[0], hi[0]) - c[0]) / r[0]) ** 2
+        y = ((np.arange(lo[1], hi[1]) - c[1]) / r[1]) ** 2
+        z = ((np.arange(lo[2], hi[2]) - c[2]) / r[2]) ** 2
+        inside = (z[:, None, None] + y[None, :, None] + x[None, None, :]) <= 1.0
+        sub = m[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]
+        sub[inside] = lab
+    return m
+
+
+def tf_points(seed: int = 0, n_points: int = 4) -> dict:
+    """Seeded control points on the reference's [0, 99] TF domain: the first
+    is transparent, the rest partly opaque."""
+    rng = np.random.default_rng(seed)
+    pos = np.sort(rng.choice(np.arange(15, 99), size=n_points, replace=False))
+    pts = {}
+    for i, p in enumerate(pos):
+        r, g, b = rng.uniform(0.2, 1.0, 3)
+        a = 0.0 if i == 0 else float(rng.uniform(0.05, 0.8))
+        pts[int(p)] = (float(r), float(g), float(b), a)
+    return pts
+
+
+def centerline(dims, n: int = 200, seed: int = 2) -> np.ndarray:
+    """Seeded cubic-spline centreline of n points (voxel units) through the
+    middle of the volume, roughly along z."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dims
+    k = 6
+    z = np.linspace(0.1 * nz, 0.9 * nz, k)
+    x = nx / 2 + rng.uniform(-0.15, 0.15, k) * nx
+    y = ny / 2 + rng.uniform(-0.15, 0.15, k) * ny
+    ctrl = np.stack([x, y, z], axis=1)
+    pts = _spline(ctrl, n - 1)
+    return pts[:n]

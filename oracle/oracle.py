@@ -1,0 +1,108 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the CPU oracle (mk_oracle.cpp).
+
+Imported by tests/, ``__graft_entry__.smoke()`` and the cpu_baseline leg of
+bench.py, always as the checker. The product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from monkeygl_amd._capi import MK_NUM_LABELS, PlaneParams, RenderParams
+
+LIB = Path(__file__).resolve().parent / "_build" / "libmk_oracle.so"
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} is not built; run `python -m monkeygl_amd.build`")
+        L = C.CDLL(str(LIB))
+        vp, i = C.c_void_p, C.c_int
+        L.mko_set_threads.restype = i
+        L.mko_set_threads.argtypes = [i]
+        L.mko_render.restype = i
+        L.mko_render.argtypes = [C.POINTER(RenderParams), vp, i, i, i, vp, vp, vp, i, i, vp, vp]
+        L.mko_render_plane.restype = i
+        L.mko_render_plane.argtypes = [C.POINTER(PlaneParams), vp, i, i, i, i, i, vp, vp]
+        L.mko_render_cpr.restype = i
+        L.mko_render_cpr.argtypes = [i, i, vp, vp, i, i, vp, i, i, i, i, i, vp, vp]
+        L.mko_tex3_linear.restype = C.c_double
+        L.mko_tex3_linear.argtypes = [vp, i, i, i, C.c_double, C.c_double, C.c_double]
+        L.mko_tex1_linear.restype = i
+        L.mko_tex1_linear.argtypes = [vp, i, C.c_double, vp]
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int) -> int:
+    return lib().mko_set_threads(n)
+
+
+def _luts(luts: dict):
+    """luts: {label: float32 array (n,4)} -> (pointer array, length array, keepalive)."""
+    keep = []
+    ptrs = (C.c_void_p * MK_NUM_LABELS)()
+    lens = (C.c_int * MK_NUM_LABELS)()
+    for l, a in luts.items():
+        arr = np.ascontiguousarray(a, dtype=np.float32).reshape(-1, 4)
+        keep.append(arr)
+        ptrs[l] = arr.ctypes.data
+        lens[l] = arr.shape[0]
+    return ptrs, lens, keep
+
+
+def render(p: RenderParams, vol_zyx: np.ndarray, luts: dict, mask_zyx=None, precision: str = "f64",
+           weights8: bool = False, touched: bool = False):
+    v = np.ascontiguousarray(vol_zyx, dtype=np.int16)
+    nz, ny, nx = v.shape
+    m = None if mask_zyx is None else np.ascontiguousarray(mask_zyx, dtype=np.uint8)
+    ptrs, lens, keep = _luts(luts)
+    out = np.zeros((p.height, p.width, 3), np.uint8)
+    t = np.zeros(v.shape, np.uint8) if touched else None
+    rc = lib().mko_render(C.byref(p), v.ctypes.data, nx, ny, nz, None if m is None else m.ctypes.data,
+                          C.cast(ptrs, C.c_void_p), C.cast(lens, C.c_void_p), 0 if precision == "f64" else 1,
+                          int(weights8), out.ctypes.data, None if t is None else t.ctypes.data)
+    assert rc == 0
+    return (out, t) if touched else out
+
+
+def render_plane(p: PlaneParams, vol_zyx: np.ndarray, precision: str = "f64", weights8: bool = False):
+    v = np.ascontiguousarray(vol_zyx, dtype=np.int16)
+    nz, ny, nx = v.shape
+    out = np.zeros((p.height, p.width), np.int16)
+    rc = lib().mko_render_plane(C.byref(p), v.ctypes.data, nx, ny, nz, 0 if precision == "f64" else 1,
+                                int(weights8), out.ctypes.data, None)
+    assert rc == 0
+    return out
+
+
+def render_cpr(width: int, points, dirs, invert_z: bool, vol_zyx, sampling: int = 0, precision: str = "f64",
+               weights8: bool = False):
+    v = np.ascontiguousarray(vol_zyx, dtype=np.int16)
+    nz, ny, nx = v.shape
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    drs = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((pts.shape[0], width), np.int16)
+    rc = lib().mko_render_cpr(width, pts.shape[0], pts.ctypes.data, drs.ctypes.data, int(invert_z), sampling,
+                              v.ctypes.data, nx, ny, nz, 0 if precision == "f64" else 1, int(weights8),
+                              out.ctypes.data, None)
+    assert rc == 0
+    return out
+
+
+def tex3_linear(vol_zyx, fx: float, fy: float, fz: float) -> float:
+    v = np.ascontiguousarray(vol_zyx, dtype=np.int16)
+    nz, ny, nx = v.shape
+    return lib().mko_tex3_linear(v.ctypes.data, nx, ny, nz, fx, fy, fz)
+
+
+def tex1_linear(lut: np.ndarray, s: float) -> np.ndarray:
+    a = np.ascontiguousarray(lut, dtype=np.float32).reshape(-1, 4)
+    out = np.zeros(4, np.float64)
+    lib().mko_tex1_linear(a.ctypes.data, a.shape[0], s, out.ctypes.data)
+    return out

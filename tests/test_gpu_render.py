@@ -1,0 +1,144 @@
+"""GPU parity of VR / MIP / Surface against the FP64 oracle, through the C ABI.
+
+Bar: every RGB channel within 1/255 of the FP64 oracle on at least 99.9% of
+pixels (the tolerance absorbs float-vs-double rounding and the truncating
+x255 conversion of core/kernel.cu:40-47).
+"""
+import numpy as np
+import pytest
+
+from monkeygl_amd import params as P
+from monkeygl_amd import workloads as W
+from monkeygl_amd._capi import MK_RENDER_MIP, MK_RENDER_SURFACE, MK_RENDER_VR, MkError
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1          # 1/255 per channel
+MIN_FRAC = 0.999
+
+VIEWS = [(0, 0), (30, 0), (0, 45), (20, 35), (-75, 110)]
+
+
+def frac_within(a, b, tol=TOL):
+    d = np.abs(a.astype(np.int32) - b.astype(np.int32)).max(axis=-1)
+    return float((d <= tol).mean()), int(d.max())
+
+
+@pytest.fixture(scope="module")
+def small():
+    vol = W.make_volume(W.SMALL)
+    lut = P.transfer_function(W.tf_points(0))
+    return vol, lut
+
+
+def _params(rt, ang, size=256, zoom=None, pan=None, labels=None, invert_z=False, **kw):
+    info = P.RenderInfo()
+    info.set_vr_size(size, size)
+    info.rotate(*ang)
+    if zoom:
+        info.zoom(zoom)
+    if pan:
+        info.pan(*pan)
+    labels = labels or {0: P.LabelState(1.0, 2000.0, 0.0)}
+    return P.render_params((128, 128, 128), (1.0, 1.0, 1.0), info, labels, render_type=rt, invert_z=invert_z, **kw)
+
+
+@pytest.mark.parametrize("rt", [MK_RENDER_VR, MK_RENDER_MIP, MK_RENDER_SURFACE])
+@pytest.mark.parametrize("ang", VIEWS)
+def test_render_small_matches_oracle(gpu_ctx, small, rt, ang):
+    vol, lut = small
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_lut(0, lut)
+    p = _params(rt, ang)
+    got = gpu_ctx.render(p)
+    ref = O.render(p, vol, {0: lut}, precision="f64")
+    frac, dmax = frac_within(got, ref)
+    print(f"rt={rt} ang={ang} frac={frac:.6f} max={dmax}")
+    assert frac >= MIN_FRAC, (frac, dmax)
+
+
+@pytest.mark.parametrize("rt", [MK_RENDER_VR, MK_RENDER_MIP])
+def test_zoom_pan_invert_z(gpu_ctx, small, rt):
+    vol, lut = small
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_lut(0, lut)
+    p = _params(rt, (15, 25), size=192, zoom=1.7, pan=(13.0, -21.0), invert_z=True)
+    got = gpu_ctx.render(p)
+    ref = O.render(p, vol, {0: lut})
+    frac, dmax = frac_within(got, ref)
+    assert frac >= MIN_FRAC, (frac, dmax)
+
+
+def test_nonsquare_and_odd_sizes(gpu_ctx, small):
+    vol, lut = small
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_lut(0, lut)
+    for w, h in [(1, 1), (17, 3), (130, 77), (300, 128)]:
+        p = _params(MK_RENDER_VR, (10, 20), width=w, height=h)
+        got = gpu_ctx.render(p)
+        ref = O.render(p, vol, {0: lut})
+        frac, dmax = frac_within(got, ref)
+        assert got.shape == (h, w, 3)
+        assert frac >= MIN_FRAC or dmax <= 1, (w, h, frac, dmax)
+
+
+def test_build_mode_step_exit_4096_lut(gpu_ctx, small):
+    """Config-0 build parameters: half-voxel step, exit at 0.99, 4096-entry LUT."""
+    vol, _ = small
+    pts = W.tf_points(0)
+    lut = P.transfer_function({int(round(k * 4095 / 99)): v for k, v in pts.items()}, n=4096)
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_lut(0, lut)
+    for ang in [(0, 0), (33, 12)]:
+        p = _params(MK_RENDER_VR, ang, step_scale=0.5, exit_alpha=0.99)
+        got = gpu_ctx.render(p)
+        ref = O.render(p, vol, {0: lut})
+        frac, dmax = frac_within(got, ref)
+        assert frac >= MIN_FRAC, (ang, frac, dmax)
+
+
+def test_multilabel_mask(gpu_ctx, small):
+    """Four ellipsoid labels, each with its own LUT, window and alpha cap."""
+    vol, lut0 = small
+    mask = W.ellipsoid_mask((128, 128, 128), 4, seed=3)
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_mask(mask)
+    luts = {0: lut0}
+    labels = {0: P.LabelState(0.4, 2000.0, 0.0)}
+    for l in range(1, 5):
+        luts[l] = P.transfer_function(W.tf_points(10 + l))
+        labels[l] = P.LabelState([1.0, 0.8, 0.3, 0.6][l - 1], 1500.0 + 100 * l, -100.0 * l)
+    for l, a in luts.items():
+        gpu_ctx.set_lut(l, a)
+    for rt in (MK_RENDER_VR, MK_RENDER_MIP):
+        for ang in [(0, 0), (40, 10)]:
+            p = _params(rt, ang, labels=labels)
+            got = gpu_ctx.render(p)
+            ref = O.render(p, vol, luts, mask_zyx=mask)
+            frac, dmax = frac_within(got, ref)
+            assert frac >= MIN_FRAC, (rt, ang, frac, dmax)
+    gpu_ctx.clear_mask()
+
+
+def test_empty_tf_gives_background(gpu_ctx, small):
+    vol, _ = small
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_lut(0, np.zeros((100, 4), np.float32))
+    p = _params(MK_RENDER_VR, (0, 0), background=(0.2, 0.4, 0.6, 1.0))
+    got = gpu_ctx.render(p)
+    assert (got[..., 0] == int(0.2 * 255)).all() and (got[..., 2] == int(np.float32(0.6) * 255)).all()
+
+
+def test_errors_are_reported(gpu_ctx, small):
+    vol, lut = small
+    gpu_ctx.set_volume(vol)
+    p = _params(MK_RENDER_VR, (0, 0))
+    p.render_type = 7
+    with pytest.raises(MkError):
+        gpu_ctx.render(p)
+    bad = np.full((128, 128, 128), 65, np.uint8)
+    with pytest.raises(MkError):
+        gpu_ctx.set_mask(bad)
+    with pytest.raises(MkError):
+        gpu_ctx.set_lut(65, lut)
