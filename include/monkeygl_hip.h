@@ -170,6 +170,15 @@ mk_status mk_render(mk_context* ctx, const mk_render_params* p, uint8_t* host_rg
  * (NULL = the context's stream); returns after enqueueing. */
 mk_status mk_render_async(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb, void* stream);
 
+/* Measurement variant of mk_render_async (no reference counterpart): renders
+ * the same frame without empty-space skipping and sets to 1 every voxel the
+ * reference rule reads, in dense x-fastest byte maps of the volume's dims
+ * (dev_touch_vol: volume voxels incl. gradient taps; dev_touch_mask: mask
+ * voxels, may be NULL, ignored without a mask). The maps must be zeroed by the
+ * caller. Their sums give the algorithmic bytes of the frame. */
+mk_status mk_render_footprint(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb,
+                              uint8_t* dev_touch_vol, uint8_t* dev_touch_mask, void* stream);
+
 /* cu_renderPlane_MIP / _MinIP / _Average (core/kernel.cu:616-837): int16
  * width*height output in caller-owned host memory, synchronously. */
 mk_status mk_render_plane(mk_context* ctx, const mk_plane_params* p, int16_t* host_out);

@@ -25,7 +25,7 @@ SYMBOLS = [
     "mk_device_count", "mk_context_create", "mk_context_destroy", "mk_last_error",
     "mk_context_stream", "mk_synchronize", "mk_copy_volume_data", "mk_copy_volume_data_device",
     "mk_copy_mask_data", "mk_copy_mask_data_device", "mk_clear_mask", "mk_set_transfer_func",
-    "mk_render", "mk_render_async", "mk_render_plane", "mk_render_plane_async",
+    "mk_render", "mk_render_async", "mk_render_footprint", "mk_render_plane", "mk_render_plane_async",
     "mk_render_cpr", "mk_render_cpr_async",
 ]
 
@@ -73,6 +73,15 @@ def load() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is not built; run `python -m monkeygl_amd.build`")
+    # One HIP runtime per process: torch's bundled runtime has the same SONAME
+    # (libamdhip64.so.7) as ROCm's, but torch asks for it as libamdhip64.so. If
+    # this library loaded ROCm's copy first, torch would load a second runtime
+    # that cannot see the GPU. Importing torch first makes the loader hand our
+    # library the runtime torch already holds.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     vp, i, st = C.c_void_p, C.c_int, C.c_int
     sig = {
@@ -90,6 +99,7 @@ def load() -> C.CDLL:
         "mk_set_transfer_func": (st, [vp, i, vp, i]),
         "mk_render": (st, [vp, C.POINTER(RenderParams), vp]),
         "mk_render_async": (st, [vp, C.POINTER(RenderParams), vp, vp]),
+        "mk_render_footprint": (st, [vp, C.POINTER(RenderParams), vp, vp, vp, vp]),
         "mk_render_plane": (st, [vp, C.POINTER(PlaneParams), vp]),
         "mk_render_plane_async": (st, [vp, C.POINTER(PlaneParams), vp, vp]),
         "mk_render_cpr": (st, [vp, i, i, vp, vp, i, i, vp]),
@@ -178,6 +188,10 @@ class Context:
 
     def render_async(self, p: RenderParams, dev_out, stream: int = 0) -> None:
         self._check(self.lib.mk_render_async(self.h, C.byref(p), _ptr(dev_out), stream or None))
+
+    def render_footprint(self, p: RenderParams, dev_out, dev_touch_vol, dev_touch_mask=None, stream: int = 0) -> None:
+        self._check(self.lib.mk_render_footprint(self.h, C.byref(p), _ptr(dev_out), _ptr(dev_touch_vol),
+                                                 _ptr(dev_touch_mask), stream or None))
 
     def render_plane(self, p: PlaneParams) -> np.ndarray:
         out = np.empty((p.height, p.width), np.int16)

@@ -19,6 +19,7 @@
 //   * 16x16 pixel tiles of four 8x8 waves, mapped XCD-aware.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -42,6 +43,8 @@ struct RenderArgs {
     int nslot;         // per-label accumulator slots (masked VR)
     uint8_t slot_of[MK_NUM_LABELS + 3];
     uint8_t* out;
+    uint8_t* touch_v;  // footprint maps (measurement variant only)
+    uint8_t* touch_m;
     int width, height;
     int invert_z;
     float mls[3];
@@ -148,6 +151,32 @@ __device__ inline float3 gradient(const RenderArgs& A, const float p[3]) {
     return make_float3(N.x * r, N.y * r, N.z * r);
 }
 
+// Footprint marking for the measurement variant: every voxel the reference
+// rule reads is set to 1 in a dense x-fastest byte map.
+__device__ inline void touch_footprint(const RenderArgs& A, float fx, float fy, float fz) {
+    int x0, x1, y0, y1, z0, z1;
+    float a;
+    mk_split(fx, A.v.nx, x0, x1, a);
+    mk_split(fy, A.v.ny, y0, y1, a);
+    mk_split(fz, A.v.nz, z0, z1, a);
+    const size_t nx = A.v.nx, nxy = (size_t)A.v.nx * A.v.ny;
+    const int xs[2] = {x0, x1}, ys[2] = {y0, y1}, zs[2] = {z0, z1};
+    for (int k = 0; k < 8; ++k) A.touch_v[zs[k >> 2] * nxy + ys[(k >> 1) & 1] * nx + xs[k & 1]] = 1;
+}
+
+__device__ inline void touch_gradient(const RenderArgs& A, const float p[3]) {
+    touch_footprint(A, p[0] + A.sv[0], p[1], p[2]);
+    touch_footprint(A, p[0] - A.sv[0], p[1], p[2]);
+    touch_footprint(A, p[0], p[1] + A.sv[1], p[2]);
+    touch_footprint(A, p[0], p[1] - A.sv[1], p[2]);
+    touch_footprint(A, p[0], p[1], p[2] + A.sv[2]);
+    touch_footprint(A, p[0], p[1], p[2] - A.sv[2]);
+}
+
+__device__ inline void touch_mask(const RenderArgs& A, int cx, int cy, int cz) {
+    if (A.touch_m) A.touch_m[((size_t)cz * A.v.ny + cy) * A.v.nx + cx] = 1;
+}
+
 __device__ inline void store_rgb(const RenderArgs& A, int x, int y, float4 c) {
     uint8_t* o = A.out + 3 * ((size_t)y * A.width + x);
     o[0] = (uint8_t)(unsigned)(mk_sat(c.x) * 255.0f);
@@ -189,8 +218,9 @@ __device__ inline Smem stage_shared(const RenderArgs& A, const AwTable& T, bool 
     return s;
 }
 
-// d_render (core/kernel.cu:133-264).
-template <int TW, bool MASK, bool LDS_LUT>
+// d_render (core/kernel.cu:133-264). TOUCH: measurement variant that skips
+// nothing and marks every voxel the reference rule reads.
+template <int TW, bool MASK, bool LDS_LUT, bool TOUCH>
 __global__ void __launch_bounds__(TW * TW) k_vr(RenderArgs A, AwTable T) {
     Smem s = stage_shared(A, T, LDS_LUT);
     int x, y;
@@ -220,8 +250,12 @@ __global__ void __launch_bounds__(TW * TW) k_vr(RenderArgs A, AwTable T) {
             float a = 0.f;
             float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
             int label = 0;
-            if (A.occ[mk_brick_index(A.v, cx, cy, cz)]) {
+            if (TOUCH || A.occ[mk_brick_index(A.v, cx, cy, cz)]) {
                 if (MASK) label = A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)];
+                if (TOUCH) {
+                    if (MASK) touch_mask(A, cx, cy, cz);
+                    touch_footprint(A, p[0], p[1], p[2]);
+                }
                 const float4 aw = s.aw[label];
                 float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
                 t = (t - aw.z) / aw.y + 0.5f;
@@ -246,6 +280,7 @@ __global__ void __launch_bounds__(TW * TW) k_vr(RenderArgs A, AwTable T) {
                 float* slot = MASK ? &s.aobj[A.slot_of[label] * BS + tid] : &aobj0;
                 if (*slot < cap) {
                     // tracing (kernel.cu:49-81)
+                    if (TOUCH) touch_gradient(A, p);
                     const float3 N = gradient(A, p);
                     const float d = N.x * L.x + N.y * L.y + N.z * L.z;
                     float k = 0.35f;
@@ -270,7 +305,7 @@ __global__ void __launch_bounds__(TW * TW) k_vr(RenderArgs A, AwTable T) {
 }
 
 // d_renderMIP (core/kernel.cu:266-364).
-template <int TW, bool MASK>
+template <int TW, bool MASK, bool TOUCH>
 __global__ void __launch_bounds__(TW * TW) k_mip(RenderArgs A, AwTable T) {
     Smem s = stage_shared(A, T, false);
     int x, y;
@@ -291,10 +326,12 @@ __global__ void __launch_bounds__(TW * TW) k_mip(RenderArgs A, AwTable T) {
             if (in_voi(A, p, ix, iy, iz)) {
                 const int cx = mk_clampi(ix, 0, A.v.nx - 1), cy = mk_clampi(iy, 0, A.v.ny - 1),
                           cz = mk_clampi(iz, 0, A.v.nz - 1);
-                if (A.mipb[mk_brick_index(A.v, cx, cy, cz)] > aAcc) {
+                if (TOUCH || A.mipb[mk_brick_index(A.v, cx, cy, cz)] > aAcc) {
                     int label = MASK ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : 0;
+                    if (TOUCH && MASK) touch_mask(A, cx, cy, cz);
                     const float4 aw = s.aw[label];
                     if (aw.x > 0.f) {
+                        if (TOUCH) touch_footprint(A, p[0], p[1], p[2]);
                         float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
                         t = (t - aw.z) / aw.y + 0.5f;
                         if (aAcc < t) aAcc = t;
@@ -309,7 +346,7 @@ __global__ void __launch_bounds__(TW * TW) k_mip(RenderArgs A, AwTable T) {
 }
 
 // d_renderSurface (core/kernel.cu:366-469).
-template <int TW>
+template <int TW, bool TOUCH>
 __global__ void __launch_bounds__(TW * TW) k_surface(RenderArgs A, AwTable T) {
     (void)stage_shared(A, T, false);
     int x, y;
@@ -331,9 +368,11 @@ __global__ void __launch_bounds__(TW * TW) k_surface(RenderArgs A, AwTable T) {
             if (in_voi(A, p, ix, iy, iz)) {
                 const int cx = mk_clampi(ix, 0, A.v.nx - 1), cy = mk_clampi(iy, 0, A.v.ny - 1),
                           cz = mk_clampi(iz, 0, A.v.nz - 1);
-                if (A.occ[mk_brick_index(A.v, cx, cy, cz)]) {
+                if (TOUCH || A.occ[mk_brick_index(A.v, cx, cy, cz)]) {
+                    if (TOUCH) touch_footprint(A, p[0], p[1], p[2]);
                     float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
                     if (t > 400.0f) {
+                        if (TOUCH) touch_gradient(A, p);
                         const float3 N = gradient(A, p);
                         aAcc = N.x * L.x + N.y * L.y + N.z * L.z;
                         if (aAcc > 0.f) break;
@@ -460,10 +499,8 @@ mk_status validate(mk_context* ctx, const mk_render_params* p) {
     return MK_OK;
 }
 
-}  // namespace
-
-extern "C" mk_status mk_render_async(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb, void* stream) {
-    if (!ctx) return MK_ERR_ARG;
+mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb, void* stream,
+                        uint8_t* touch_v, uint8_t* touch_m) {
     mk_status st = validate(ctx, p);
     if (st) return st;
     if (!dev_rgb) return mk_fail(ctx, MK_ERR_ARG, "null output");
@@ -492,6 +529,8 @@ extern "C" mk_status mk_render_async(mk_context* ctx, const mk_render_params* p,
     A.steps = ctx->d_steps;
     A.nsteps = ctx->steps_n;
     A.out = dev_rgb;
+    A.touch_v = touch_v;
+    A.touch_m = touch_m;
     A.width = p->width;
     A.height = p->height;
     A.invert_z = p->invert_z != 0;
@@ -520,7 +559,8 @@ extern "C" mk_status mk_render_async(mk_context* ctx, const mk_render_params* p,
     }
     A.nslot = mask ? nslot : 0;
     int total_lut = 0;
-    for (int l = 0; l < MK_NUM_LABELS; ++l) total_lut = ctx->lut_tab.off[l] + ctx->lut_tab.len[l] > total_lut ? ctx->lut_tab.off[l] + ctx->lut_tab.len[l] : total_lut;
+    for (int l = 0; l < MK_NUM_LABELS; ++l)
+        total_lut = std::max(total_lut, ctx->lut_tab.off[l] + ctx->lut_tab.len[l]);
     const bool lds_lut = p->render_type == MK_RENDER_VR && total_lut <= kLutLdsEntries;
     A.nlut_lds = lds_lut ? total_lut : 0;
 
@@ -534,6 +574,7 @@ extern "C" mk_status mk_render_async(mk_context* ctx, const mk_render_params* p,
     const uint32_t tiles = (uint32_t)(((p->width + tw - 1) / tw) * ((p->height + tw - 1) / tw));
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     if (s != ctx->stream) {
+        // the skip tables were (re)built on the context stream
         hipEvent_t ev;
         MK_HIP(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         MK_HIP(ctx, hipEventRecord(ev, ctx->stream));
@@ -542,21 +583,40 @@ extern "C" mk_status mk_render_async(mk_context* ctx, const mk_render_params* p,
     }
     dim3 grid(tiles), block(tw * tw);
 #define MK_LAUNCH(K) hipLaunchKernelGGL(K, grid, block, lds, s, A, T)
-    if (p->render_type == MK_RENDER_VR) {
-        if (tw == 16) {
-            if (mask) { if (lds_lut) MK_LAUNCH((k_vr<16, true, true>)); else MK_LAUNCH((k_vr<16, true, false>)); }
-            else { if (lds_lut) MK_LAUNCH((k_vr<16, false, true>)); else MK_LAUNCH((k_vr<16, false, false>)); }
-        } else {
-            if (lds_lut) MK_LAUNCH((k_vr<8, true, true>)); else MK_LAUNCH((k_vr<8, true, false>));
-        }
-    } else if (p->render_type == MK_RENDER_MIP) {
-        if (mask) MK_LAUNCH((k_mip<16, true>)); else MK_LAUNCH((k_mip<16, false>));
-    } else {
-        MK_LAUNCH((k_surface<16>));
+#define MK_VR(TOUCH)                                                                              \
+    if (tw == 16) {                                                                               \
+        if (mask) { if (lds_lut) MK_LAUNCH((k_vr<16, true, true, TOUCH>)); else MK_LAUNCH((k_vr<16, true, false, TOUCH>)); } \
+        else { if (lds_lut) MK_LAUNCH((k_vr<16, false, true, TOUCH>)); else MK_LAUNCH((k_vr<16, false, false, TOUCH>)); } \
+    } else {                                                                                      \
+        if (lds_lut) MK_LAUNCH((k_vr<8, true, true, TOUCH>)); else MK_LAUNCH((k_vr<8, true, false, TOUCH>)); \
     }
+    const bool touch = touch_v != nullptr;
+    if (p->render_type == MK_RENDER_VR) {
+        if (touch) { MK_VR(true) } else { MK_VR(false) }
+    } else if (p->render_type == MK_RENDER_MIP) {
+        if (touch) { if (mask) MK_LAUNCH((k_mip<16, true, true>)); else MK_LAUNCH((k_mip<16, false, true>)); }
+        else { if (mask) MK_LAUNCH((k_mip<16, true, false>)); else MK_LAUNCH((k_mip<16, false, false>)); }
+    } else {
+        if (touch) MK_LAUNCH((k_surface<16, true>)); else MK_LAUNCH((k_surface<16, false>));
+    }
+#undef MK_VR
 #undef MK_LAUNCH
     MK_HIP(ctx, hipGetLastError());
     return MK_OK;
+}
+
+}  // namespace
+
+extern "C" mk_status mk_render_async(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb, void* stream) {
+    if (!ctx) return MK_ERR_ARG;
+    return launch_render(ctx, p, dev_rgb, stream, nullptr, nullptr);
+}
+
+extern "C" mk_status mk_render_footprint(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb,
+                                         uint8_t* dev_touch_vol, uint8_t* dev_touch_mask, void* stream) {
+    if (!ctx) return MK_ERR_ARG;
+    if (!dev_touch_vol) return mk_fail(ctx, MK_ERR_ARG, "null footprint map");
+    return launch_render(ctx, p, dev_rgb, stream, dev_touch_vol, ctx->d_mask ? dev_touch_mask : nullptr);
 }
 
 extern "C" mk_status mk_render(mk_context* ctx, const mk_render_params* p, uint8_t* host_rgb) {

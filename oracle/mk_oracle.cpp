@@ -149,7 +149,10 @@ template <typename T> struct Ray {
     T mls[3];
     int invz;
     // core/kernel.cu:197-210 position of the sample at accumulated length acc.
-    inline void pos(T acc, double scale, T p[3]) const {
+    // acc is float in every instance: the reference defines its sample lattice
+    // by float accumulation (kernel.cu:169-172, 213, 238), so the FP64 oracle
+    // marches on that lattice and evaluates positions and values in double.
+    inline void pos(float acc, double scale, T p[3]) const {
         T fy = (T)(((double)acc - 0.866) * scale);
         T x = u * M[0] + fy * M[1] + v * M[2];
         T y = u * M[3] + fy * M[4] + v * M[5];
@@ -220,14 +223,15 @@ static void vr_pixel(const mk_render_params& P, const Volume& vol, const Luts& L
     T sv[3] = {T(P.spacing_voxel[0]), T(P.spacing_voxel[1]), T(P.spacing_voxel[2])};
     const double scale = (double)P.scale;
 
-    const T L1 = T(1) / T(vol.nz) * T(P.step_scale);
-    const T L4 = L1 / T(4);
-    const T L8 = L1 / T(8);
+    const float L1 = 1.0f / (float)vol.nz * P.step_scale;
+    const float L4 = L1 / 4.0f;
+    const float L8 = L1 / 8.0f;
     const T exitA = T(P.exit_alpha);
-    T step = L1;
+    float step = L1;
     T aObj[MK_NUM_LABELS];
     for (int i = 0; i < MK_NUM_LABELS; ++i) aObj[i] = T(0);
-    T aAcc = T(0), acc = T(0), aPre = T(0);
+    T aAcc = T(0), aPre = T(0);
+    float acc = 0.0f;
     T sum[4] = {T(0), T(0), T(0), T(0)};
 
     while ((double)acc < 1.732) {
@@ -246,7 +250,7 @@ static void vr_pixel(const mk_render_params& P, const Volume& vol, const Luts& L
         // getNextStep
         if (step == L4) a = T(1) - std::pow(T(1) - a, T(0.25f));
         else if (step == L8) a = T(1) - std::pow(T(1) - a, T(0.125f));
-        if (acc > T(0)) {
+        if (acc > 0.0f) {
             if (std::max(a, aPre) > T(0.001f)) {
                 if (step == L1) { acc -= (L1 - L4); step = L4; continue; }
                 else if (step == L4) { acc -= (L4 - L8); step = L8; continue; }
@@ -285,8 +289,9 @@ template <typename T, bool Q8>
 static void mip_pixel(const mk_render_params& P, const Volume& vol, int x, int y, uint8_t* out) {
     Ray<T> ray = make_ray<T>(P, x, y);
     const double scale = (double)P.scale;
-    const T step = T(1) / T(vol.nz) * T(P.step_scale);
-    T aAcc = T(0), acc = T(0);
+    const float step = 1.0f / (float)vol.nz * P.step_scale;
+    T aAcc = T(0);
+    float acc = 0.0f;
     while ((double)acc < 1.732) {
         T p[3];
         ray.pos(acc, scale, p);
@@ -315,8 +320,9 @@ static void surface_pixel(const mk_render_params& P, const Volume& vol, int x, i
     light_dir<T>(P, dirL);
     T sv[3] = {T(P.spacing_voxel[0]), T(P.spacing_voxel[1]), T(P.spacing_voxel[2])};
     const double scale = (double)P.scale;
-    const T step = T(1) / T(vol.nz) * T(P.step_scale);
-    T aAcc = T(0), acc = T(0);
+    const float step = 1.0f / (float)vol.nz * P.step_scale;
+    T aAcc = T(0);
+    float acc = 0.0f;
     while ((double)acc < 1.732) {
         T p[3];
         ray.pos(acc, scale, p);
@@ -388,11 +394,11 @@ static int16_t cpr_pixel(const double* pts, const double* dirs, int invz, int sa
 }
 
 template <typename T, bool Q8>
-static void render_all(const mk_render_params& P, const Volume& vol, const Luts& L, uint8_t* out) {
+static void render_all(const mk_render_params& P, const Volume& vol, const Luts& L, uint8_t* out, int y0, int y1) {
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int y = 0; y < P.height; ++y) {
+    for (int y = y0; y < y1; ++y) {
         for (int x = 0; x < P.width; ++x) {
-            uint8_t* o = out + 3 * ((size_t)y * P.width + x);
+            uint8_t* o = out + 3 * ((size_t)(y - y0) * P.width + x);
             if (P.render_type == MK_RENDER_VR) vr_pixel<T, Q8>(P, vol, L, x, y, o);
             else if (P.render_type == MK_RENDER_MIP) mip_pixel<T, Q8>(P, vol, x, y, o);
             else surface_pixel<T, Q8>(P, vol, x, y, o);
@@ -425,15 +431,29 @@ int mko_set_threads(int n) {
     return omp_get_max_threads();
 }
 
+int mko_render_rows(const mk_render_params* P, const int16_t* v, int nx, int ny, int nz, const uint8_t* mask,
+                    const float* const* luts, const int* lut_len, int precision, int weights8, int y0, int y1,
+                    uint8_t* out, uint8_t* touched);
+
 int mko_render(const mk_render_params* P, const int16_t* v, int nx, int ny, int nz, const uint8_t* mask,
                const float* const* luts, const int* lut_len, int precision, int weights8, uint8_t* out,
                uint8_t* touched) {
     if (!P || !v || !out || nx <= 0 || ny <= 0 || nz <= 0 || !luts || !lut_len) return 1;
+    return mko_render_rows(P, v, nx, ny, nz, mask, luts, lut_len, precision, weights8, 0, P->height, out, touched);
+}
+
+// Rows [y0, y1) of the frame P describes (rays unchanged), into out of
+// (y1 - y0) * width * 3 bytes.
+int mko_render_rows(const mk_render_params* P, const int16_t* v, int nx, int ny, int nz, const uint8_t* mask,
+                    const float* const* luts, const int* lut_len, int precision, int weights8, int y0, int y1,
+                    uint8_t* out, uint8_t* touched) {
+    if (!P || !v || !out || nx <= 0 || ny <= 0 || nz <= 0 || !luts || !lut_len) return 1;
+    if (y0 < 0 || y1 > P->height || y0 > y1) return 1;
     Volume vol{v, mask, nx, ny, nz, touched};
     Luts L{luts, lut_len};
-    if (precision == 0) render_all<double, false>(*P, vol, L, out);
-    else if (weights8) render_all<float, true>(*P, vol, L, out);
-    else render_all<float, false>(*P, vol, L, out);
+    if (precision == 0) render_all<double, false>(*P, vol, L, out, y0, y1);
+    else if (weights8) render_all<float, true>(*P, vol, L, out, y0, y1);
+    else render_all<float, false>(*P, vol, L, out, y0, y1);
     return 0;
 }
 

@@ -27,6 +27,8 @@ def lib() -> C.CDLL:
         L.mko_set_threads.argtypes = [i]
         L.mko_render.restype = i
         L.mko_render.argtypes = [C.POINTER(RenderParams), vp, i, i, i, vp, vp, vp, i, i, vp, vp]
+        L.mko_render_rows.restype = i
+        L.mko_render_rows.argtypes = [C.POINTER(RenderParams), vp, i, i, i, vp, vp, vp, i, i, i, i, vp, vp]
         L.mko_render_plane.restype = i
         L.mko_render_plane.argtypes = [C.POINTER(PlaneParams), vp, i, i, i, i, i, vp, vp]
         L.mko_render_cpr.restype = i
@@ -69,6 +71,21 @@ def render(p: RenderParams, vol_zyx: np.ndarray, luts: dict, mask_zyx=None, prec
                           int(weights8), out.ctypes.data, None if t is None else t.ctypes.data)
     assert rc == 0
     return (out, t) if touched else out
+
+
+def render_rows(p: RenderParams, vol_zyx: np.ndarray, luts: dict, y0: int, y1: int, mask_zyx=None,
+                precision: str = "f64", weights8: bool = False):
+    """Rows [y0, y1) of the frame p describes (same rays as the full frame)."""
+    v = np.ascontiguousarray(vol_zyx, dtype=np.int16)
+    nz, ny, nx = v.shape
+    m = None if mask_zyx is None else np.ascontiguousarray(mask_zyx, dtype=np.uint8)
+    ptrs, lens, keep = _luts(luts)
+    out = np.zeros((y1 - y0, p.width, 3), np.uint8)
+    rc = lib().mko_render_rows(C.byref(p), v.ctypes.data, nx, ny, nz, None if m is None else m.ctypes.data,
+                               C.cast(ptrs, C.c_void_p), C.cast(lens, C.c_void_p), 0 if precision == "f64" else 1,
+                               int(weights8), y0, y1, out.ctypes.data, None)
+    assert rc == 0
+    return out
 
 
 def render_plane(p: PlaneParams, vol_zyx: np.ndarray, precision: str = "f64", weights8: bool = False):

@@ -142,3 +142,28 @@ def test_errors_are_reported(gpu_ctx, small):
         gpu_ctx.set_mask(bad)
     with pytest.raises(MkError):
         gpu_ctx.set_lut(65, lut)
+
+
+@pytest.mark.parametrize("rt", [MK_RENDER_VR, MK_RENDER_MIP, MK_RENDER_SURFACE])
+def test_footprint_matches_oracle(gpu_ctx, small, rt):
+    """The measurement variant marks the voxels the reference rule reads; the
+    oracle marks them independently. The sets agree up to samples whose
+    footprint flips at a texel boundary between float and double positions."""
+    import torch
+
+    vol, lut = small
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_lut(0, lut)
+    p = _params(rt, (20, 35))
+    out = torch.empty(p.width * p.height * 3, dtype=torch.uint8, device="cuda")
+    tv = torch.zeros(vol.size, dtype=torch.uint8, device="cuda")
+    gpu_ctx.render_footprint(p, out, tv, None, stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    g = tv.cpu().numpy().reshape(vol.shape).astype(bool)
+    img, t = O.render(p, vol, {0: lut}, precision="f32", touched=True)
+    r = t.astype(bool)
+    assert r.sum() > 0
+    assert np.logical_xor(g, r).sum() <= 0.002 * r.sum(), (g.sum(), r.sum(), np.logical_xor(g, r).sum())
+    # the measurement variant renders the same frame
+    frac, dmax = frac_within(out.cpu().numpy().reshape(p.height, p.width, 3), img)
+    assert frac >= MIN_FRAC, (frac, dmax)
