@@ -1,0 +1,154 @@
+"""Writes golden vectors for the kernel-argument producers.
+
+Runs the reference's own TransferFunction.cpp, Methods.cpp and RenderInfo.cpp,
+compiled unmodified by oracle/build_ref.sh into oracle/_ref/libmkref.so, on a
+fixed set of inputs and stores inputs and outputs as JSON next to this script:
+
+  tf_luts.json        control points (+ optional alpha points) -> 100-entry LUT
+  render_info.json    RenderInfo operation sequences -> matrices after each op
+
+Usage: sh oracle/build_ref.sh && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REF_LIB = HERE.parent.parent / "oracle" / "_ref" / "libmkref.so"
+
+
+def load():
+    L = C.CDLL(str(REF_LIB))
+    vp, i, f = C.c_void_p, C.c_int, C.c_float
+    L.ref_transfer_function.restype = i
+    L.ref_transfer_function.argtypes = [vp, vp, i, vp, vp, i, vp, vp]
+    L.ref_render_info_new.restype = vp
+    L.ref_render_info_free.argtypes = [vp]
+    L.ref_render_info_rotate.argtypes = [vp, f, f]
+    L.ref_render_info_zoom.restype = f
+    L.ref_render_info_zoom.argtypes = [vp, f]
+    L.ref_render_info_pan.argtypes = [vp, f, f]
+    L.ref_render_info_preset.argtypes = [vp, i]
+    L.ref_render_info_set_vr_size.argtypes = [vp, i, i, vp, vp]
+    L.ref_render_info_get.argtypes = [vp, vp, vp, vp, vp, vp]
+    return L
+
+
+def ref_tf(L, pts: dict, alpha: dict | None = None):
+    pos = np.array(sorted(pts), np.int32)
+    rgba = np.array([pts[k] for k in sorted(pts)], np.float32).reshape(-1)
+    alpha = alpha or {}
+    apos = np.array(sorted(alpha), np.int32) if alpha else np.zeros(1, np.int32)
+    aval = np.array([alpha[k] for k in sorted(alpha)], np.float32) if alpha else np.zeros(1, np.float32)
+    out = np.zeros(4 * 1024, np.float32)
+    n = C.c_int(0)
+    ok = L.ref_transfer_function(pos.ctypes.data, rgba.ctypes.data, len(pos), apos.ctypes.data, aval.ctypes.data,
+                                 len(alpha), out.ctypes.data, C.byref(n))
+    return out[: 4 * n.value].reshape(-1, 4) if ok else None
+
+
+def ref_render_info_run(L, ops):
+    h = L.ref_render_info_new()
+    states = []
+    for op in ops:
+        kind = op[0]
+        if kind == "rotate":
+            L.ref_render_info_rotate(h, op[1], op[2])
+        elif kind == "zoom":
+            L.ref_render_info_zoom(h, op[1])
+        elif kind == "pan":
+            L.ref_render_info_pan(h, op[1], op[2])
+        elif kind == "preset":
+            L.ref_render_info_preset(h, op[1])
+        elif kind == "size":
+            ow, oh = C.c_int(0), C.c_int(0)
+            L.ref_render_info_set_vr_size(h, op[1], op[2], C.byref(ow), C.byref(oh))
+        bufs = [np.zeros(9, np.float32) for _ in range(4)]
+        xyz = np.zeros(3, np.float32)
+        L.ref_render_info_get(h, *[b.ctypes.data for b in bufs], xyz.ctypes.data)
+        states.append({"m": bufs[0].tolist(), "m_t": bufs[1].tolist(), "rot": bufs[2].tolist(),
+                       "rot_t": bufs[3].tolist(), "x_translate": float(xyz[0]), "y_translate": float(xyz[1]),
+                       "scale": float(xyz[2])})
+    L.ref_render_info_free(h)
+    return states
+
+
+def ref_vr_size(L, w, h):
+    r = L.ref_render_info_new()
+    ow, oh = C.c_int(0), C.c_int(0)
+    L.ref_render_info_set_vr_size(r, w, h, C.byref(ow), C.byref(oh))
+    L.ref_render_info_free(r)
+    return ow.value, oh.value
+
+
+def tf_cases():
+    rng = np.random.default_rng(1234)
+    cases = [
+        {"name": "object_default", "pts": {5: (0.8, 0.8, 0.8, 0.0), 95: (0.8, 0.8, 0.8, 0.8)}},
+        # trfns/coro.txt control points
+        {"name": "coro", "pts": {5: (0.8, 0.0, 0.0, 0.0), 90: (0.8, 0.8, 0.8, 0.8)}},
+        {"name": "ends_exact", "pts": {0: (0.0, 0.1, 0.2, 0.0), 99: (1.0, 0.9, 0.8, 1.0)}},
+        {"name": "one_point", "pts": {40: (0.5, 0.5, 0.5, 0.5)}},
+        {"name": "all_out_of_range", "pts": {-5: (1, 1, 1, 1), 120: (0, 0, 0, 0)}},
+        {"name": "partly_out_of_range", "pts": {-3: (1, 0, 0, 1), 30: (0, 1, 0, 0.3), 70: (0, 0, 1, 0.6), 140: (1, 1, 1, 1)}},
+        {"name": "adjacent", "pts": {10: (1, 0, 0, 0.1), 11: (0, 1, 0, 0.9), 12: (0, 0, 1, 0.2)}},
+        {"name": "alpha_map", "pts": {0: (0.9, 0.2, 0.1, 1.0), 99: (0.1, 0.2, 0.9, 1.0)},
+         "alpha": {20: 0.0, 50: 0.7, 80: 0.2}},
+        {"name": "alpha_map_single", "pts": {0: (0.9, 0.2, 0.1, 1.0), 99: (0.1, 0.2, 0.9, 1.0)}, "alpha": {50: 0.7}},
+        {"name": "alpha_map_out_of_range", "pts": {10: (0.3, 0.3, 0.3, 0.3), 60: (0.6, 0.6, 0.6, 0.6)},
+         "alpha": {-10: 0.5, 40: 0.1, 99: 0.9}},
+    ]
+    for k in range(12):
+        n = int(rng.integers(2, 9))
+        pos = rng.choice(np.arange(-10, 110), size=n, replace=False)
+        pts = {int(p): tuple(float(x) for x in np.round(rng.uniform(0, 1, 4), 4)) for p in pos}
+        case = {"name": f"random_{k}", "pts": pts}
+        if k % 3 == 0:
+            apos = rng.choice(np.arange(0, 100), size=int(rng.integers(2, 5)), replace=False)
+            case["alpha"] = {int(p): float(np.round(rng.uniform(0, 1), 4)) for p in apos}
+        cases.append(case)
+    # the control points the tests and the benchmark use
+    import sys
+    sys.path.insert(0, str(HERE.parent.parent))
+    from monkeygl_amd import workloads as W
+    for s in (0, 11, 12, 13, 14, 100, 101, 102, 103, 104):
+        cases.append({"name": f"workload_seed_{s}", "pts": W.tf_points(s)})
+    return cases
+
+
+def render_info_cases():
+    return [
+        {"name": "rotate_z_steps", "ops": [("rotate", 30.0, 0.0)] * 12},
+        {"name": "rotate_x_steps", "ops": [("rotate", 0.0, 45.0)] * 8},
+        {"name": "mixed", "ops": [("rotate", 20.0, 35.0), ("zoom", 1.7), ("pan", 13.0, -21.0), ("rotate", -75.0, 110.0),
+                                  ("zoom", 0.6), ("rotate", 5.5, -3.25)]},
+        {"name": "presets", "ops": [("zoom", 2.0), ("preset", 0), ("preset", 1), ("zoom", 1.5), ("preset", 2),
+                                    ("preset", 3), ("preset", 4), ("preset", 5), ("rotate", 10.0, 10.0)]},
+        {"name": "many_small", "ops": [("rotate", 1.0, 0.5)] * 40},
+    ]
+
+
+def main():
+    L = load()
+    tf = []
+    for c in tf_cases():
+        lut = ref_tf(L, c["pts"], c.get("alpha"))
+        tf.append({"name": c["name"], "pts": {str(k): list(v) for k, v in c["pts"].items()},
+                   "alpha": {str(k): v for k, v in c.get("alpha", {}).items()},
+                   "lut": None if lut is None else lut.tolist()})
+    (HERE / "tf_luts.json").write_text(json.dumps(tf))
+    ri = []
+    for c in render_info_cases():
+        ri.append({"name": c["name"], "ops": [list(o) for o in c["ops"]], "states": ref_render_info_run(L, c["ops"])})
+    sizes = [(w, h, *ref_vr_size(L, w, h)) for (w, h) in [(512, 512), (768, 768), (1024, 1024), (1537, 1024),
+                                                            (2048, 2048), (333, 101), (1600, 900), (3000, 700)]]
+    (HERE / "render_info.json").write_text(json.dumps({"sequences": ri, "vr_sizes": sizes}))
+    print(f"wrote {len(tf)} TF cases, {len(ri)} RenderInfo sequences, {len(sizes)} VR sizes")
+
+
+if __name__ == "__main__":
+    main()
