@@ -193,6 +193,22 @@ mk_status mk_render_cpr_async(mk_context* ctx, int width, int height, const doub
                               const double* dev_dirs, int invert_z, int sampling,
                               int16_t* dev_out, void* stream);
 
+/* Marching cubes over the resident volume (no reference counterpart: the
+ * reference carries only the triangle table, core/MarchingCube.cpp:35-292).
+ * box = {x0, y0, z0, x1, y1, z1}, voxel indices, half-open; cubes have their
+ * min corner in [x0, x1-1) x [y0, y1-1) x [z0, z1-1) and are visited x
+ * fastest. A corner is inside when its raw int16 value < iso. Triangles come
+ * in cube order, then table order; vertices are in voxel units.
+ * Writes min(total, capacity) triangles to host_out (may be NULL with
+ * capacity 0 to count only) and the total to *n_triangles. Synchronous. */
+mk_status mk_marching_cubes(mk_context* ctx, const int* box, float iso, mk_triangle* host_out,
+                            uint64_t capacity, uint64_t* n_triangles);
+
+/* Same into device memory on the given stream: up to capacity triangles to
+ * dev_out and the total (uint64) to dev_count (may be NULL). */
+mk_status mk_marching_cubes_async(mk_context* ctx, const int* box, float iso, mk_triangle* dev_out,
+                                  uint64_t capacity, uint64_t* dev_count, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,7 @@ SYMBOLS = [
     "mk_context_stream", "mk_synchronize", "mk_copy_volume_data", "mk_copy_volume_data_device",
     "mk_copy_mask_data", "mk_copy_mask_data_device", "mk_clear_mask", "mk_set_transfer_func",
     "mk_render", "mk_render_async", "mk_render_footprint", "mk_render_plane", "mk_render_plane_async",
-    "mk_render_cpr", "mk_render_cpr_async",
+    "mk_render_cpr", "mk_render_cpr_async", "mk_marching_cubes", "mk_marching_cubes_async",
 ]
 
 
@@ -104,6 +104,8 @@ def load() -> C.CDLL:
         "mk_render_plane_async": (st, [vp, C.POINTER(PlaneParams), vp, vp]),
         "mk_render_cpr": (st, [vp, i, i, vp, vp, i, i, vp]),
         "mk_render_cpr_async": (st, [vp, i, i, vp, vp, i, i, vp, vp]),
+        "mk_marching_cubes": (st, [vp, vp, C.c_float, vp, C.c_uint64, C.POINTER(C.c_uint64)]),
+        "mk_marching_cubes_async": (st, [vp, vp, C.c_float, vp, C.c_uint64, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -214,3 +216,19 @@ class Context:
                          dev_out, stream: int = 0) -> None:
         self._check(self.lib.mk_render_cpr_async(self.h, width, height, _ptr(dev_points), _ptr(dev_dirs),
                                                  int(invert_z), sampling, _ptr(dev_out), stream or None))
+
+    def marching_cubes(self, box, iso: float, capacity: int | None = None) -> np.ndarray:
+        """Triangles (n, 3, 3) float32 in voxel units; box = (x0, y0, z0, x1, y1, z1)."""
+        b = np.ascontiguousarray(box, dtype=np.int32)
+        n = C.c_uint64(0)
+        self._check(self.lib.mk_marching_cubes(self.h, _ptr(b), iso, None, 0, C.byref(n)))
+        cap = n.value if capacity is None else min(capacity, n.value)
+        out = np.empty((cap, 3, 3), np.float32)
+        if cap:
+            self._check(self.lib.mk_marching_cubes(self.h, _ptr(b), iso, _ptr(out), cap, C.byref(n)))
+        return out
+
+    def marching_cubes_async(self, box, iso: float, dev_out, capacity: int, dev_count, stream: int = 0) -> None:
+        b = np.ascontiguousarray(box, dtype=np.int32)
+        self._check(self.lib.mk_marching_cubes_async(self.h, _ptr(b), iso, _ptr(dev_out), capacity, _ptr(dev_count),
+                                                     stream or None))

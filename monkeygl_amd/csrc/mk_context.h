@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <string>
@@ -23,6 +24,7 @@ struct MkBrickStats {
     int16_t* vmax = nullptr;
     uint64_t* lab_lo = nullptr;
     uint8_t* lab_hi = nullptr;
+    uint8_t* blabel = nullptr;  // the brick's only label, 0xFF when it holds several
 };
 
 // LUT layout on the device: all labels' RGBA entries packed as float4, label l
@@ -59,6 +61,8 @@ struct mk_context {
     // per-frame skip tables (cached on their inputs)
     uint8_t* d_occ = nullptr;     // VR / Surface: brick may contribute
     float* d_mipb = nullptr;      // MIP: upper bound of the windowed value
+    uint8_t* d_occ2 = nullptr;    // the same per macro cell of 4x4x4 bricks
+    float* d_mipb2 = nullptr;
     bool occ_valid = false;
     int occ_type = -1;
     uint32_t occ_lut_version = 0;
@@ -71,6 +75,12 @@ struct mk_context {
     size_t steps_cap = 0;
     float steps_l1 = -1.0f;
     int steps_n = 0;
+
+    // marching cubes: expanded triangle table, per-block counts/offsets, total
+    uint8_t* d_mc_table = nullptr;
+    void* d_mc_scratch = nullptr;
+    size_t mc_scratch_cap = 0;
+    void* d_mc_total = nullptr;
 
     // staging for synchronous host-output calls
     void* d_stage = nullptr;

@@ -58,17 +58,29 @@ __device__ inline void mk_split(float f, int n, int& i0, int& i1, float& a) {
     i1 = mk_clampi(i + 1, 0, n - 1);
 }
 
-// Trilinear sample in HU' units (32768 * normalized-float read).
+__device__ inline float mk_rawo(const MkVolView& v, uint32_t off) {
+    int s = v.vox[off];
+    return (float)(s < -32767 ? -32767 : s);
+}
+
+// Trilinear sample in HU' units (32768 * normalized-float read). The eight
+// corner offsets come from one brick-offset computation plus per-axis deltas:
+// +1/+8/+64 inside a brick, a jump to the next brick across its face, 0 where
+// the index clamps at the volume edge.
 __device__ inline float mk_sample_hu(const MkVolView& v, float fx, float fy, float fz) {
     int x0, x1, y0, y1, z0, z1;
     float ax, ay, az;
     mk_split(fx, v.nx, x0, x1, ax);
     mk_split(fy, v.ny, y0, y1, ay);
     mk_split(fz, v.nz, z0, z1, az);
-    float c000 = mk_raw(v, x0, y0, z0), c100 = mk_raw(v, x1, y0, z0);
-    float c010 = mk_raw(v, x0, y1, z0), c110 = mk_raw(v, x1, y1, z0);
-    float c001 = mk_raw(v, x0, y0, z1), c101 = mk_raw(v, x1, y0, z1);
-    float c011 = mk_raw(v, x0, y1, z1), c111 = mk_raw(v, x1, y1, z1);
+    const uint32_t o = mk_voxel_offset(v, x0, y0, z0);
+    const uint32_t dx = x1 == x0 ? 0u : ((x0 & 7) == 7 ? 512u - 7u : 1u);
+    const uint32_t dy = y1 == y0 ? 0u : ((y0 & 7) == 7 ? (uint32_t)v.nbx * 512u - 56u : 8u);
+    const uint32_t dz = z1 == z0 ? 0u : ((z0 & 7) == 7 ? (uint32_t)v.nbx * (uint32_t)v.nby * 512u - 448u : 64u);
+    float c000 = mk_rawo(v, o), c100 = mk_rawo(v, o + dx);
+    float c010 = mk_rawo(v, o + dy), c110 = mk_rawo(v, o + dy + dx);
+    float c001 = mk_rawo(v, o + dz), c101 = mk_rawo(v, o + dz + dx);
+    float c011 = mk_rawo(v, o + dz + dy), c111 = mk_rawo(v, o + dz + dy + dx);
     float c00 = c000 + ax * (c100 - c000);
     float c10 = c010 + ax * (c110 - c010);
     float c01 = c001 + ax * (c101 - c001);

@@ -1,0 +1,252 @@
+// mk_mc.hip -- marching-cubes iso-surface extraction on gfx950.
+//
+// The reference holds only the triangle table (core/MarchingCube.cpp:35-292);
+// extraction is new. The triangle list is defined as:
+//   * cubes are the unit cells with min corner (i, j, k) inside the box
+//     [x0, x1-1) x [y0, y1-1) x [z0, z1-1), visited i fastest, then j, then k;
+//   * corner values are the raw int16 voxels as float; corner c is inside
+//     when value < iso and sets bit c of the case index;
+//   * triangles of a cube follow the table order, vertices the edge order;
+//   * a vertex on an edge interpolates from the corner with the smaller
+//     coordinate along the edge: t = (iso - f_lo) / (f_hi - f_lo), coordinate
+//     = lo + t, so an edge shared by neighbouring cubes yields the same bits.
+// Three launches: per-block triangle counts, an exclusive scan of the block
+// counts, and the emit pass that recomputes each cube and writes at its
+// block's offset plus its rank in the block. This file is compiled with
+// -ffp-contract=off so the vertices are bit-exact with the host restatement.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "mk_context.h"
+#include "mk_mc_table.h"
+
+namespace {
+
+constexpr int kMcBlock = 256;
+
+struct McArgs {
+    MkVolView v;
+    int x0, y0, z0;
+    int cx, cy, cz;  // cubes per axis
+    uint64_t ncubes;
+    float iso;
+    const int8_t* tri;    // [256][16]
+    const uint8_t* cnt;   // [256]
+};
+
+__device__ inline void cube_coords(const McArgs& A, uint64_t q, int& i, int& j, int& k) {
+    i = A.x0 + (int)(q % (uint64_t)A.cx);
+    uint64_t r = q / (uint64_t)A.cx;
+    j = A.y0 + (int)(r % (uint64_t)A.cy);
+    k = A.z0 + (int)(r / (uint64_t)A.cy);
+}
+
+__device__ inline int cube_case(const McArgs& A, int i, int j, int k, float f[8]) {
+    int c = 0;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        f[n] = (float)A.v.vox[mk_voxel_offset(A.v, i + kMkMcCorner[n][0], j + kMkMcCorner[n][1], k + kMkMcCorner[n][2])];
+        c |= (f[n] < A.iso ? 1 : 0) << n;
+    }
+    return c;
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads, 4 waves).
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int w = 0; w < wave; ++w) before += s_wave[w];
+    total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    return before + x - v;
+}
+
+__global__ void __launch_bounds__(kMcBlock) k_mc_count(McArgs A, uint32_t* __restrict__ block_sum) {
+    __shared__ uint32_t s_wave[4];
+    const uint64_t q = (uint64_t)blockIdx.x * kMcBlock + threadIdx.x;
+    uint32_t n = 0;
+    if (q < A.ncubes) {
+        int i, j, k;
+        cube_coords(A, q, i, j, k);
+        float f[8];
+        n = A.cnt[cube_case(A, i, j, k, f)];
+    }
+    uint32_t total;
+    (void)block_exclusive_scan(n, s_wave, total);
+    if (threadIdx.x == 0) block_sum[blockIdx.x] = total;
+}
+
+// Exclusive scan of the block counts by one workgroup; writes the total.
+__global__ void __launch_bounds__(1024) k_mc_scan(const uint32_t* __restrict__ block_sum, uint64_t* __restrict__ block_off,
+                                                  uint32_t nblocks, unsigned long long* __restrict__ total) {
+    __shared__ unsigned long long s[1024];
+    unsigned long long carry = 0;
+    for (uint32_t base = 0; base < nblocks; base += 1024) {
+        const uint32_t idx = base + threadIdx.x;
+        unsigned long long v = idx < nblocks ? block_sum[idx] : 0ull;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            unsigned long long y = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0ull;
+            __syncthreads();
+            s[threadIdx.x] += y;
+            __syncthreads();
+        }
+        if (idx < nblocks) block_off[idx] = carry + s[threadIdx.x] - v;
+        const unsigned long long chunk = s[1023];
+        __syncthreads();
+        carry += chunk;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(kMcBlock) k_mc_emit(McArgs A, const uint64_t* __restrict__ block_off,
+                                                      float* __restrict__ out, uint64_t capacity) {
+    __shared__ uint32_t s_wave[4];
+    const uint64_t q = (uint64_t)blockIdx.x * kMcBlock + threadIdx.x;
+    float f[8];
+    int c = 0, i = 0, j = 0, k = 0;
+    uint32_t n = 0;
+    if (q < A.ncubes) {
+        cube_coords(A, q, i, j, k);
+        c = cube_case(A, i, j, k, f);
+        n = A.cnt[c];
+    }
+    uint32_t total;
+    const uint32_t rank = block_exclusive_scan(n, s_wave, total);
+    uint64_t t = block_off[blockIdx.x] + rank;
+    const int8_t* row = A.tri + 16 * c;
+    for (uint32_t m = 0; m < n; ++m, ++t) {
+        if (t >= capacity) break;
+        float* o = out + 9 * t;
+        for (int vtx = 0; vtx < 3; ++vtx) {
+            const int e = row[3 * m + vtx];
+            int a = kMkMcEdge[e][0], b = kMkMcEdge[e][1];
+            // order the corners along the edge axis
+            const bool swap = kMkMcCorner[a][0] + kMkMcCorner[a][1] + kMkMcCorner[a][2] >
+                              kMkMcCorner[b][0] + kMkMcCorner[b][1] + kMkMcCorner[b][2];
+            const int lo = swap ? b : a, hi = swap ? a : b;
+            const float tt = (A.iso - f[lo]) / (f[hi] - f[lo]);
+            float p[3] = {(float)(i + kMkMcCorner[lo][0]), (float)(j + kMkMcCorner[lo][1]), (float)(k + kMkMcCorner[lo][2])};
+            const int axis = kMkMcCorner[lo][0] != kMkMcCorner[hi][0] ? 0 : (kMkMcCorner[lo][1] != kMkMcCorner[hi][1] ? 1 : 2);
+            p[axis] = p[axis] + tt;
+            o[3 * vtx + 0] = p[0];
+            o[3 * vtx + 1] = p[1];
+            o[3 * vtx + 2] = p[2];
+        }
+    }
+}
+
+mk_status ensure_table(mk_context* ctx) {
+    if (ctx->d_mc_table) return MK_OK;
+    int8_t tri[256][16];
+    uint8_t cnt[256];
+    mk_mc_expand(tri, cnt);
+    std::vector<uint8_t> buf(256 * 16 + 256);
+    memcpy(buf.data(), tri, 256 * 16);
+    memcpy(buf.data() + 256 * 16, cnt, 256);
+    MK_HIP(ctx, hipMalloc((void**)&ctx->d_mc_table, buf.size()));
+    MK_HIP(ctx, hipMemcpyAsync(ctx->d_mc_table, buf.data(), buf.size(), hipMemcpyHostToDevice, ctx->stream));
+    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MK_OK;
+}
+
+mk_status prepare(mk_context* ctx, const int* box, float iso, McArgs& A, uint32_t& nblocks) {
+    if (!box) return mk_fail(ctx, MK_ERR_ARG, "null box");
+    if (!ctx->d_vox) return mk_fail(ctx, MK_ERR_STATE, "marching cubes before any volume was uploaded");
+    if (!std::isfinite(iso)) return mk_fail(ctx, MK_ERR_ARG, "iso value is not finite");
+    const int x0 = box[0], y0 = box[1], z0 = box[2], x1 = box[3], y1 = box[4], z1 = box[5];
+    if (x0 < 0 || y0 < 0 || z0 < 0 || x1 > ctx->nx || y1 > ctx->ny || z1 > ctx->nz || x1 - x0 < 2 || y1 - y0 < 2 ||
+        z1 - z0 < 2)
+        return mk_fail(ctx, MK_ERR_ARG, "bad box [%d,%d,%d)-[%d,%d,%d) for volume %dx%dx%d", x0, y0, z0, x1, y1, z1,
+                       ctx->nx, ctx->ny, ctx->nz);
+    mk_status st = ensure_table(ctx);
+    if (st) return st;
+    A.v = ctx->view();
+    A.x0 = x0; A.y0 = y0; A.z0 = z0;
+    A.cx = x1 - x0 - 1; A.cy = y1 - y0 - 1; A.cz = z1 - z0 - 1;
+    A.ncubes = (uint64_t)A.cx * A.cy * A.cz;
+    A.iso = iso;
+    A.tri = (const int8_t*)ctx->d_mc_table;
+    A.cnt = ctx->d_mc_table + 256 * 16;
+    const uint64_t nb = (A.ncubes + kMcBlock - 1) / kMcBlock;
+    if (nb > 0x7FFFFFFFull) return mk_fail(ctx, MK_ERR_ARG, "box too large");
+    nblocks = (uint32_t)nb;
+    return MK_OK;
+}
+
+mk_status launch_count(mk_context* ctx, const McArgs& A, uint32_t nblocks, hipStream_t s) {
+    mk_status st = mk_ensure_buffer(ctx, &ctx->d_mc_scratch, &ctx->mc_scratch_cap,
+                                    (size_t)nblocks * (sizeof(uint32_t) + sizeof(uint64_t)) + 64);
+    if (st) return st;
+    uint32_t* sums = (uint32_t*)ctx->d_mc_scratch;
+    uint64_t* offs = (uint64_t*)(((uintptr_t)(sums + nblocks) + 63) & ~(uintptr_t)63);
+    unsigned long long* total = (unsigned long long*)ctx->d_mc_total;
+    hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(kMcBlock), 0, s, A, sums);
+    hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, s, sums, offs, nblocks, total);
+    MK_HIP(ctx, hipGetLastError());
+    return MK_OK;
+}
+
+}  // namespace
+
+extern "C" mk_status mk_marching_cubes_async(mk_context* ctx, const int* box, float iso, mk_triangle* dev_out,
+                                             uint64_t capacity, uint64_t* dev_count, void* stream) {
+    if (!ctx) return MK_ERR_ARG;
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    if (!ctx->d_mc_total) MK_HIP(ctx, hipMalloc((void**)&ctx->d_mc_total, 64));
+    McArgs A;
+    uint32_t nblocks = 0;
+    mk_status st = prepare(ctx, box, iso, A, nblocks);
+    if (st) return st;
+    if (capacity > 0 && !dev_out) return mk_fail(ctx, MK_ERR_ARG, "null triangle buffer");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    st = launch_count(ctx, A, nblocks, s);
+    if (st) return st;
+    uint32_t* sums = (uint32_t*)ctx->d_mc_scratch;
+    uint64_t* offs = (uint64_t*)(((uintptr_t)(sums + nblocks) + 63) & ~(uintptr_t)63);
+    if (capacity > 0)
+        hipLaunchKernelGGL(k_mc_emit, dim3(nblocks), dim3(kMcBlock), 0, s, A, offs, (float*)dev_out, capacity);
+    if (dev_count) MK_HIP(ctx, hipMemcpyAsync(dev_count, ctx->d_mc_total, sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    MK_HIP(ctx, hipGetLastError());
+    return MK_OK;
+}
+
+extern "C" mk_status mk_marching_cubes(mk_context* ctx, const int* box, float iso, mk_triangle* host_out,
+                                       uint64_t capacity, uint64_t* n_triangles) {
+    if (!ctx) return MK_ERR_ARG;
+    if (!n_triangles) return mk_fail(ctx, MK_ERR_ARG, "null count");
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    if (!ctx->d_mc_total) MK_HIP(ctx, hipMalloc((void**)&ctx->d_mc_total, 64));
+    McArgs A;
+    uint32_t nblocks = 0;
+    mk_status st = prepare(ctx, box, iso, A, nblocks);
+    if (st) return st;
+    st = launch_count(ctx, A, nblocks, ctx->stream);
+    if (st) return st;
+    uint64_t total = 0;
+    MK_HIP(ctx, hipMemcpyAsync(&total, ctx->d_mc_total, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
+    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *n_triangles = total;
+    if (!host_out || capacity == 0 || total == 0) return MK_OK;
+    const uint64_t n = total < capacity ? total : capacity;
+    void* d_out = nullptr;
+    MK_HIP(ctx, hipMalloc(&d_out, n * sizeof(mk_triangle)));
+    uint32_t* sums = (uint32_t*)ctx->d_mc_scratch;
+    uint64_t* offs = (uint64_t*)(((uintptr_t)(sums + nblocks) + 63) & ~(uintptr_t)63);
+    hipLaunchKernelGGL(k_mc_emit, dim3(nblocks), dim3(kMcBlock), 0, ctx->stream, A, offs, (float*)d_out, n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host_out, d_out, n * sizeof(mk_triangle), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return mk_hip_fail(ctx, e, "marching cubes emit");
+    return MK_OK;
+}

@@ -35,8 +35,12 @@ struct RenderArgs {
     MkVolView v;
     const float4* lut;
     MkLutTable tab;
-    const uint8_t* occ;
-    const float* mipb;
+    const uint8_t* occ;    // per 8^3 brick (VR, Surface)
+    const float* mipb;     // per 8^3 brick (MIP)
+    const uint8_t* occ2;   // per 32^3 macro cell (4x4x4 bricks)
+    const uint8_t* blabel; // per brick: its only label, or 0xFF when mixed
+    const float* mipb2;
+    int nmx, nmy;          // macro cell dims (x, y)
     const float* steps;
     int nsteps;
     int nlut_lds;      // entries staged in LDS (0 = read LUTs from global)
@@ -76,14 +80,21 @@ __device__ inline void ray_pos(const RenderArgs& A, float u, float v, float acc,
     if (A.invert_z) p[2] = 1.0f - p[2];
 }
 
+// Linear model of a ray, pos(a) ~ p0 + a * d, in normalized volume units.
+struct RayGeom {
+    float p0[3];
+    float d[3];
+};
+
 // Conservative [a_in, a_out] of accumulated lengths whose samples can pass
 // the VOI test; returns false when none can.
-__device__ inline bool ray_interval(const RenderArgs& A, float u, float v, float& a_in, float& a_out) {
-    float p0[3];
-    ray_pos(A, u, v, 0.0f, p0);
-    float d[3] = {A.M[1] * (float)A.scale * A.mls[0], A.M[4] * (float)A.scale * A.mls[1],
-                  A.M[7] * (float)A.scale * A.mls[2]};
-    if (A.invert_z) d[2] = -d[2];
+__device__ inline bool ray_interval(const RenderArgs& A, float u, float v, RayGeom& g, float& a_in, float& a_out) {
+    ray_pos(A, u, v, 0.0f, g.p0);
+    // pos(a) at a = 0 is taken at fy = -0.866*scale; shift to the a origin
+    g.d[0] = A.M[1] * (float)A.scale * A.mls[0];
+    g.d[1] = A.M[4] * (float)A.scale * A.mls[1];
+    g.d[2] = A.M[7] * (float)A.scale * A.mls[2];
+    if (A.invert_z) g.d[2] = -g.d[2];
     const int lo_i[3] = {A.voi.left, A.voi.posterior, A.voi.head};
     const int hi_i[3] = {A.voi.right, A.voi.anterior, A.voi.foot};
     const int n[3] = {A.v.nx, A.v.ny, A.v.nz};
@@ -93,10 +104,10 @@ __device__ inline bool ray_interval(const RenderArgs& A, float u, float v, float
         // trunc(p*N) in [lo, hi]  =>  p in (lo-1, hi+1)/N
         float lo = ((float)lo_i[i] - 1.0f) / (float)n[i] - eps;
         float hi = ((float)hi_i[i] + 1.0f) / (float)n[i] + eps;
-        if (fabsf(d[i]) < 1e-12f) {
-            if (!(p0[i] >= lo && p0[i] <= hi)) return false;
+        if (fabsf(g.d[i]) < 1e-12f) {
+            if (!(g.p0[i] >= lo && g.p0[i] <= hi)) return false;
         } else {
-            float ta = (lo - p0[i]) / d[i], tb = (hi - p0[i]) / d[i];
+            float ta = (lo - g.p0[i]) / g.d[i], tb = (hi - g.p0[i]) / g.d[i];
             t0 = fmaxf(t0, fminf(ta, tb));
             t1 = fminf(t1, fmaxf(ta, tb));
         }
@@ -107,6 +118,28 @@ __device__ inline bool ray_interval(const RenderArgs& A, float u, float v, float
     return a_in <= a_out;
 }
 
+// Accumulated length below which the ray is certain to stay in the cell of
+// 2^shift voxels per axis that holds voxel (cx, cy, cz): the earliest exit
+// over the three axes, with the cell shrunk by a margin that covers the
+// rounding of the sample positions.
+__device__ inline float cell_exit(const RenderArgs& A, const RayGeom& g, int cx, int cy, int cz, int shift) {
+    const int c[3] = {cx >> shift, cy >> shift, cz >> shift};
+    const float n[3] = {(float)A.v.nx, (float)A.v.ny, (float)A.v.nz};
+    const float size = (float)(1 << shift);
+    const float margin = 1e-5f;
+    float t = INFINITY;
+    for (int i = 0; i < 3; ++i) {
+        if (g.d[i] > 1e-12f) {
+            float face = ((float)(c[i] + 1) * size) / n[i] - margin;
+            t = fminf(t, (face - g.p0[i]) / g.d[i]);
+        } else if (g.d[i] < -1e-12f) {
+            float face = ((float)c[i] * size) / n[i] + margin;
+            t = fminf(t, (face - g.p0[i]) / g.d[i]);
+        }
+    }
+    return t;
+}
+
 __device__ inline bool in_voi(const RenderArgs& A, const float p[3], int& ix, int& iy, int& iz) {
     float fx = p[0] * (float)A.v.nx, fy = p[1] * (float)A.v.ny, fz = p[2] * (float)A.v.nz;
     fx = fminf(fmaxf(fx, -1e9f), 1e9f);
@@ -115,6 +148,10 @@ __device__ inline bool in_voi(const RenderArgs& A, const float p[3], int& ix, in
     ix = (int)fx; iy = (int)fy; iz = (int)fz;
     return !(ix < A.voi.left || ix > A.voi.right || iy < A.voi.posterior || iy > A.voi.anterior || iz < A.voi.head ||
              iz > A.voi.foot);
+}
+
+__device__ inline uint32_t macro_index(const RenderArgs& A, int cx, int cy, int cz) {
+    return ((uint32_t)(cz >> 5) * (uint32_t)A.nmy + (uint32_t)(cy >> 5)) * (uint32_t)A.nmx + (uint32_t)(cx >> 5);
 }
 
 // tex1D linear/clamp of label's LUT at normalized s.
@@ -140,15 +177,20 @@ __device__ inline float3 light_dir(const RenderArgs& A) {
 }
 
 // Normalized central-difference gradient one voxel apart (kernel.cu:60-67);
-// a zero gradient gives NaN, which fails every "> 0" test downstream.
+// a zero gradient gives NaN, which fails every "> 0" test downstream. One
+// axis per trip keeps 16 loads in flight instead of 48.
 __device__ inline float3 gradient(const RenderArgs& A, const float p[3]) {
-    float3 N;
-    N.x = mk_sample_hu(A.v, p[0] + A.sv[0], p[1], p[2]) - mk_sample_hu(A.v, p[0] - A.sv[0], p[1], p[2]);
-    N.y = mk_sample_hu(A.v, p[0], p[1] + A.sv[1], p[2]) - mk_sample_hu(A.v, p[0], p[1] - A.sv[1], p[2]);
-    N.z = mk_sample_hu(A.v, p[0], p[1], p[2] + A.sv[2]) - mk_sample_hu(A.v, p[0], p[1], p[2] - A.sv[2]);
-    if (A.invert_z) N.z = -N.z;
-    float r = rsqrtf(N.x * N.x + N.y * N.y + N.z * N.z);
-    return make_float3(N.x * r, N.y * r, N.z * r);
+    float N[3];
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+        float q0[3] = {p[0], p[1], p[2]}, q1[3] = {p[0], p[1], p[2]};
+        q0[i] += A.sv[i];
+        q1[i] -= A.sv[i];
+        N[i] = mk_sample_hu(A.v, q0[0], q0[1], q0[2]) - mk_sample_hu(A.v, q1[0], q1[1], q1[2]);
+    }
+    if (A.invert_z) N[2] = -N[2];
+    float r = rsqrtf(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+    return make_float3(N[0] * r, N[1] * r, N[2] * r);
 }
 
 // Footprint marking for the measurement variant: every voxel the reference
@@ -218,6 +260,22 @@ __device__ inline Smem stage_shared(const RenderArgs& A, const AwTable& T, bool 
     return s;
 }
 
+__device__ inline float start_acc(const RenderArgs& A, float a_in) {
+    int k0 = (int)(a_in / A.l1) - 2;
+    k0 = k0 < 0 ? 0 : (k0 >= A.nsteps ? A.nsteps - 1 : k0);
+    return A.steps[k0];
+}
+
+// Advances acc over the lattice points a_{k+1} = a_k + L1 that lie below
+// a_skip (all inside one cell that cannot change the result); it counts trips
+// against the same budget as the marching loop.
+__device__ inline void skip_lattice(float& acc, float l1, float a_skip, int& it, int max_iter) {
+    while (acc + l1 < a_skip && it < max_iter) {
+        acc += l1;
+        ++it;
+    }
+}
+
 // d_render (core/kernel.cu:133-264). TOUCH: measurement variant that skips
 // nothing and marks every voxel the reference rule reads.
 template <int TW, bool MASK, bool LDS_LUT, bool TOUCH>
@@ -232,26 +290,41 @@ __global__ void __launch_bounds__(TW * TW) k_vr(RenderArgs A, AwTable T) {
     const float v = ((float)y - (float)A.height / 2.0f - A.yT) / (float)A.height;
     float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
     float a_in, a_out;
-    if (ray_interval(A, u, v, a_in, a_out)) {
-        const float3 L = light_dir(A);
+    RayGeom g;
+    if (ray_interval(A, u, v, g, a_in, a_out)) {
         const float L1 = A.l1, L4 = L1 / 4.0f, L8 = L1 / 8.0f;
         float step = L1, aPre = 0.f, aAcc = 0.f, aobj0 = 0.f;
         if (MASK)
             for (int k = 0; k < A.nslot; ++k) s.aobj[k * BS + tid] = 0.f;
-        int k0 = (int)(a_in / L1) - 2;
-        k0 = k0 < 0 ? 0 : (k0 >= A.nsteps ? A.nsteps - 1 : k0);
-        float acc = A.steps[k0];
+        float acc = start_acc(A, a_in);
+        // skip-table entries of the brick the ray is in, reloaded when it
+        // moves to another brick
+        uint32_t cur_brick = 0xFFFFFFFFu;
+        int cur_macro_empty = 0, cur_occ = 1, cur_label = 0xFF;
         for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
             float p[3];
             ray_pos(A, u, v, acc, p);
             int ix, iy, iz;
             if (!in_voi(A, p, ix, iy, iz)) { acc += step; continue; }
             const int cx = mk_clampi(ix, 0, A.v.nx - 1), cy = mk_clampi(iy, 0, A.v.ny - 1), cz = mk_clampi(iz, 0, A.v.nz - 1);
+            // a sample in an empty cell reads alpha 0 exactly (the skip tables
+            // are conservative), so it needs no fetch
+            int empty_shift = 0;
+            if (!TOUCH) {
+                const uint32_t b = mk_brick_index(A.v, cx, cy, cz);
+                if (b != cur_brick) {
+                    cur_brick = b;
+                    cur_macro_empty = !A.occ2[macro_index(A, cx, cy, cz)];
+                    cur_occ = A.occ[b];
+                    cur_label = MASK ? A.blabel[b] : 0;
+                }
+                empty_shift = cur_macro_empty ? 5 : (cur_occ ? 0 : 3);
+            }
             float a = 0.f;
             float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
             int label = 0;
-            if (TOUCH || A.occ[mk_brick_index(A.v, cx, cy, cz)]) {
-                if (MASK) label = A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)];
+            if (!empty_shift) {
+                if (MASK) label = (TOUCH || cur_label == 0xFF) ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label;
                 if (TOUCH) {
                     if (MASK) touch_mask(A, cx, cy, cz);
                     touch_footprint(A, p[0], p[1], p[2]);
@@ -264,7 +337,7 @@ __global__ void __launch_bounds__(TW * TW) k_vr(RenderArgs A, AwTable T) {
                 a = col.w;
             }
             // getNextStep (kernel.cu:84-125)
-            if (a > 0.f && step != L1) a = 1.0f - powf(1.0f - a, step == L4 ? 0.25f : 0.125f);
+            if (a > 0.f && step != L1) a = 1.0f - __builtin_amdgcn_exp2f((step == L4 ? 0.25f : 0.125f) * __builtin_amdgcn_logf(1.0f - a));
             if (acc > 0.f) {
                 if (fmaxf(a, aPre) > 0.001f) {
                     if (step == L1) { acc -= (L1 - L4); step = L4; continue; }
@@ -282,6 +355,7 @@ __global__ void __launch_bounds__(TW * TW) k_vr(RenderArgs A, AwTable T) {
                     // tracing (kernel.cu:49-81)
                     if (TOUCH) touch_gradient(A, p);
                     const float3 N = gradient(A, p);
+                    const float3 L = light_dir(A);
                     const float d = N.x * L.x + N.y * L.y + N.z * L.z;
                     float k = 0.35f;
                     if (d > 0.f) {
@@ -298,6 +372,9 @@ __global__ void __launch_bounds__(TW * TW) k_vr(RenderArgs A, AwTable T) {
                 }
             }
             if (aAcc > A.exit_alpha) break;
+            // steady state in an empty cell: every further lattice point inside
+            // it is accepted with alpha 0 and changes nothing but acc
+            if (empty_shift && step == L1) skip_lattice(acc, L1, cell_exit(A, g, cx, cy, cz, empty_shift), it, A.max_iter);
         }
     }
     if (sum.x == 0.f && sum.y == 0.f && sum.z == 0.f && sum.w == 0.f) sum = bg4(A);
@@ -314,11 +391,13 @@ __global__ void __launch_bounds__(TW * TW) k_mip(RenderArgs A, AwTable T) {
     const float v = ((float)y - (float)A.height / 2.0f - A.yT) / (float)A.height;
     float aAcc = 0.f;
     float a_in, a_out;
-    if (ray_interval(A, u, v, a_in, a_out)) {
+    RayGeom g;
+    if (ray_interval(A, u, v, g, a_in, a_out)) {
         const float L1 = A.l1;
-        int k0 = (int)(a_in / L1) - 2;
-        k0 = k0 < 0 ? 0 : (k0 >= A.nsteps ? A.nsteps - 1 : k0);
-        float acc = A.steps[k0];
+        float acc = start_acc(A, a_in);
+        uint32_t cur_brick = 0xFFFFFFFFu;
+        float cur_bm = INFINITY, cur_bb = INFINITY;
+        int cur_label = 0xFF;
         for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
             float p[3];
             ray_pos(A, u, v, acc, p);
@@ -326,8 +405,20 @@ __global__ void __launch_bounds__(TW * TW) k_mip(RenderArgs A, AwTable T) {
             if (in_voi(A, p, ix, iy, iz)) {
                 const int cx = mk_clampi(ix, 0, A.v.nx - 1), cy = mk_clampi(iy, 0, A.v.ny - 1),
                           cz = mk_clampi(iz, 0, A.v.nz - 1);
-                if (TOUCH || A.mipb[mk_brick_index(A.v, cx, cy, cz)] > aAcc) {
-                    int label = MASK ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : 0;
+                // a cell whose bound cannot exceed the running maximum leaves it unchanged
+                int skip_shift = 0;
+                if (!TOUCH) {
+                    const uint32_t b = mk_brick_index(A.v, cx, cy, cz);
+                    if (b != cur_brick) {
+                        cur_brick = b;
+                        cur_bm = A.mipb2[macro_index(A, cx, cy, cz)];
+                        cur_bb = A.mipb[b];
+                        cur_label = MASK ? A.blabel[b] : 0;
+                    }
+                    skip_shift = cur_bm <= aAcc ? 5 : (cur_bb <= aAcc ? 3 : 0);
+                }
+                if (!skip_shift) {
+                    int label = MASK ? ((TOUCH || cur_label == 0xFF) ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label) : 0;
                     if (TOUCH && MASK) touch_mask(A, cx, cy, cz);
                     const float4 aw = s.aw[label];
                     if (aw.x > 0.f) {
@@ -337,8 +428,11 @@ __global__ void __launch_bounds__(TW * TW) k_mip(RenderArgs A, AwTable T) {
                         if (aAcc < t) aAcc = t;
                     }
                 }
+                acc += L1;
+                if (skip_shift) skip_lattice(acc, L1, cell_exit(A, g, cx, cy, cz, skip_shift), it, A.max_iter);
+            } else {
+                acc += L1;
             }
-            acc += L1;
         }
     }
     float4 c = aAcc <= 0.f ? bg4(A) : make_float4(aAcc, aAcc, aAcc, aAcc);
@@ -355,12 +449,12 @@ __global__ void __launch_bounds__(TW * TW) k_surface(RenderArgs A, AwTable T) {
     const float v = ((float)y - (float)A.height / 2.0f - A.yT) / (float)A.height;
     float aAcc = 0.f;
     float a_in, a_out;
-    if (ray_interval(A, u, v, a_in, a_out)) {
-        const float3 L = light_dir(A);
+    RayGeom g;
+    if (ray_interval(A, u, v, g, a_in, a_out)) {
         const float L1 = A.l1;
-        int k0 = (int)(a_in / L1) - 2;
-        k0 = k0 < 0 ? 0 : (k0 >= A.nsteps ? A.nsteps - 1 : k0);
-        float acc = A.steps[k0];
+        float acc = start_acc(A, a_in);
+        uint32_t cur_brick = 0xFFFFFFFFu;
+        int cur_macro_empty = 0, cur_occ = 1;
         for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
             float p[3];
             ray_pos(A, u, v, acc, p);
@@ -368,25 +462,39 @@ __global__ void __launch_bounds__(TW * TW) k_surface(RenderArgs A, AwTable T) {
             if (in_voi(A, p, ix, iy, iz)) {
                 const int cx = mk_clampi(ix, 0, A.v.nx - 1), cy = mk_clampi(iy, 0, A.v.ny - 1),
                           cz = mk_clampi(iz, 0, A.v.nz - 1);
-                if (TOUCH || A.occ[mk_brick_index(A.v, cx, cy, cz)]) {
+                int skip_shift = 0;
+                if (!TOUCH) {
+                    const uint32_t b = mk_brick_index(A.v, cx, cy, cz);
+                    if (b != cur_brick) {
+                        cur_brick = b;
+                        cur_macro_empty = !A.occ2[macro_index(A, cx, cy, cz)];
+                        cur_occ = A.occ[b];
+                    }
+                    skip_shift = cur_macro_empty ? 5 : (cur_occ ? 0 : 3);
+                }
+                if (!skip_shift) {
                     if (TOUCH) touch_footprint(A, p[0], p[1], p[2]);
                     float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
                     if (t > 400.0f) {
                         if (TOUCH) touch_gradient(A, p);
                         const float3 N = gradient(A, p);
+                        const float3 L = light_dir(A);
                         aAcc = N.x * L.x + N.y * L.y + N.z * L.z;
                         if (aAcc > 0.f) break;
                     }
                 }
+                acc += L1;
+                if (skip_shift) skip_lattice(acc, L1, cell_exit(A, g, cx, cy, cz, skip_shift), it, A.max_iter);
+            } else {
+                acc += L1;
             }
-            acc += L1;
         }
     }
     float4 c;
     if (aAcc <= 0.f) c = bg4(A);
     else {
-        float g = (float)((double)aAcc + 0.05);
-        c = make_float4(g, g, g, 1.0f);
+        float g2 = (float)((double)aAcc + 0.05);
+        c = make_float4(g2, g2, g2, 1.0f);
     }
     store_rgb(A, x, y, c);
 }
@@ -450,6 +558,28 @@ __global__ void __launch_bounds__(256) k_occupancy(const int16_t* __restrict__ v
     else occ[b] = any;
 }
 
+// Macro cells of 4x4x4 bricks: OR of the brick occupancy, max of the MIP bound.
+__global__ void __launch_bounds__(256) k_macro(const uint8_t* __restrict__ occ, const float* __restrict__ mipb,
+                                                int nbx, int nby, int nbz, int nmx, int nmy, int nmz, int type,
+                                                uint8_t* __restrict__ occ2, float* __restrict__ mipb2) {
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (uint32_t)(nmx * nmy * nmz)) return;
+    const int mx = m % nmx, my = (m / nmx) % nmy, mz = m / (nmx * nmy);
+    uint8_t any = 0;
+    float bound = -INFINITY;
+    for (int dz = 0; dz < 4; ++dz)
+        for (int dy = 0; dy < 4; ++dy)
+            for (int dx = 0; dx < 4; ++dx) {
+                const int bx = mx * 4 + dx, by = my * 4 + dy, bz = mz * 4 + dz;
+                if (bx >= nbx || by >= nby || bz >= nbz) continue;
+                const uint32_t b = ((uint32_t)bz * nby + by) * nbx + bx;
+                if (type == MK_RENDER_MIP) bound = fmaxf(bound, mipb[b]);
+                else any |= occ[b];
+            }
+    if (type == MK_RENDER_MIP) mipb2[m] = bound;
+    else occ2[m] = any;
+}
+
 mk_status ensure_steps(mk_context* ctx, float l1) {
     if (ctx->steps_l1 == l1 && ctx->d_steps) return MK_OK;
     std::vector<float> a;
@@ -478,6 +608,11 @@ mk_status ensure_occupancy(mk_context* ctx, const mk_render_params* p, const AwT
     hipLaunchKernelGGL(k_occupancy, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, ctx->stats.vmin, ctx->stats.vmax,
                        ctx->stats.lab_lo, ctx->stats.lab_hi, nb, p->render_type, T, ctx->lut_tab, ctx->d_lutnz, ctx->d_occ,
                        ctx->d_mipb);
+    MK_HIP(ctx, hipGetLastError());
+    const int nmx = (ctx->nbx + 3) / 4, nmy = (ctx->nby + 3) / 4, nmz = (ctx->nbz + 3) / 4;
+    const uint32_t nm = (uint32_t)(nmx * nmy * nmz);
+    hipLaunchKernelGGL(k_macro, dim3((nm + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_occ, ctx->d_mipb, ctx->nbx,
+                       ctx->nby, ctx->nbz, nmx, nmy, nmz, p->render_type, ctx->d_occ2, ctx->d_mipb2);
     MK_HIP(ctx, hipGetLastError());
     ctx->occ_valid = true;
     ctx->occ_type = p->render_type;
@@ -526,6 +661,11 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     A.tab = ctx->lut_tab;
     A.occ = ctx->d_occ;
     A.mipb = ctx->d_mipb;
+    A.occ2 = ctx->d_occ2;
+    A.blabel = ctx->stats.blabel;
+    A.mipb2 = ctx->d_mipb2;
+    A.nmx = (ctx->nbx + 3) / 4;
+    A.nmy = (ctx->nby + 3) / 4;
     A.steps = ctx->d_steps;
     A.nsteps = ctx->steps_n;
     A.out = dev_rgb;

@@ -100,8 +100,11 @@ static void mk_free_volume(mk_context* ctx) {
     mk_free(ctx->stats.vmax);
     mk_free(ctx->stats.lab_lo);
     mk_free(ctx->stats.lab_hi);
+    mk_free(ctx->stats.blabel);
     mk_free(ctx->d_occ);
     mk_free(ctx->d_mipb);
+    mk_free(ctx->d_occ2);
+    mk_free(ctx->d_mipb2);
     ctx->nx = ctx->ny = ctx->nz = 0;
     ctx->occ_valid = false;
 }
@@ -114,6 +117,9 @@ extern "C" void mk_context_destroy(mk_context* ctx) {
     mk_free(ctx->d_lut);
     mk_free(ctx->d_lutnz);
     mk_free(ctx->d_steps);
+    mk_free(ctx->d_mc_table);
+    if (ctx->d_mc_scratch) (void)hipFree(ctx->d_mc_scratch);
+    if (ctx->d_mc_total) (void)hipFree(ctx->d_mc_total);
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
     if (ctx->d_stage2) (void)hipFree(ctx->d_stage2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -184,7 +190,8 @@ __global__ void __launch_bounds__(256) k_brick_minmax(MkVolView v, int16_t* __re
 // accumulate the union (lo 64 bits, bit 0 of flags[2] = label 64) and
 // flags[3] != 0 flags a label above MK_MAX_OBJECTS.
 __global__ void __launch_bounds__(256) k_brick_labels(MkVolView v, uint64_t* __restrict__ lab_lo, uint8_t* __restrict__ lab_hi,
-                                                       uint32_t nbricks, unsigned long long* __restrict__ flags) {
+                                                       uint8_t* __restrict__ blabel, uint32_t nbricks,
+                                                       unsigned long long* __restrict__ flags) {
     uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
     if (b >= nbricks) return;
@@ -209,6 +216,8 @@ __global__ void __launch_bounds__(256) k_brick_labels(MkVolView v, uint64_t* __r
         uint64_t lo = ((uint64_t)lo1 << 32) | lo0;
         lab_lo[b] = lo;
         lab_hi[b] = (uint8_t)hi;
+        const int n = __popcll(lo) + (hi ? 1 : 0);
+        blabel[b] = n == 1 ? (uint8_t)(hi ? 64 : __ffsll((long long)lo) - 1) : (uint8_t)0xFF;
         if (lo) atomicOr(&flags[0], (unsigned long long)lo);
         if (hi) atomicOr(&flags[2], 1ull);
         if (bad) atomicOr(&flags[3], 1ull);
@@ -241,8 +250,12 @@ static mk_status mk_volume_from_device(mk_context* ctx, const int16_t* d_lin, in
     MK_HIP(ctx, hipMalloc((void**)&ctx->stats.vmax, nb * sizeof(int16_t)));
     MK_HIP(ctx, hipMalloc((void**)&ctx->stats.lab_lo, nb * sizeof(uint64_t)));
     MK_HIP(ctx, hipMalloc((void**)&ctx->stats.lab_hi, nb * sizeof(uint8_t)));
+    MK_HIP(ctx, hipMalloc((void**)&ctx->stats.blabel, nb * sizeof(uint8_t)));
     MK_HIP(ctx, hipMalloc((void**)&ctx->d_occ, nb * sizeof(uint8_t)));
     MK_HIP(ctx, hipMalloc((void**)&ctx->d_mipb, nb * sizeof(float)));
+    const size_t nm = (size_t)((ctx->nbx + 3) / 4) * ((ctx->nby + 3) / 4) * ((ctx->nbz + 3) / 4);
+    MK_HIP(ctx, hipMalloc((void**)&ctx->d_occ2, nm * sizeof(uint8_t)));
+    MK_HIP(ctx, hipMalloc((void**)&ctx->d_mipb2, nm * sizeof(float)));
     hipLaunchKernelGGL(k_brickify<int16_t>, dim3((uint32_t)nb), dim3(512), 0, ctx->stream, d_lin, ctx->d_vox, nx, ny, nz,
                        ctx->nbx, ctx->nby);
     MK_HIP(ctx, hipGetLastError());
@@ -254,6 +267,8 @@ static mk_status mk_volume_from_device(mk_context* ctx, const int16_t* d_lin, in
     hipLaunchKernelGGL(k_fill_u64, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.lab_lo,
                        (uint64_t)1, (uint32_t)nb);
     hipLaunchKernelGGL(k_fill_u8, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.lab_hi,
+                       (uint8_t)0, (uint32_t)nb);
+    hipLaunchKernelGGL(k_fill_u8, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.blabel,
                        (uint8_t)0, (uint32_t)nb);
     MK_HIP(ctx, hipGetLastError());
     ctx->used_lo = 1;
@@ -309,7 +324,7 @@ static mk_status mk_mask_from_device(mk_context* ctx, const uint8_t* d_lin, int 
     MkVolView v = ctx->view();
     v.mask = d_mask;
     hipLaunchKernelGGL(k_brick_labels, dim3((uint32_t)((nb + 3) / 4)), dim3(256), 0, ctx->stream, v, ctx->stats.lab_lo,
-                       ctx->stats.lab_hi, (uint32_t)nb, d_flags);
+                       ctx->stats.lab_hi, ctx->stats.blabel, (uint32_t)nb, d_flags);
     MK_HIP(ctx, hipGetLastError());
     unsigned long long flags[4];
     MK_HIP(ctx, hipMemcpyAsync(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
@@ -322,6 +337,8 @@ static mk_status mk_mask_from_device(mk_context* ctx, const uint8_t* d_lin, int 
                            (uint64_t)1, (uint32_t)nb);
         hipLaunchKernelGGL(k_fill_u8, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.lab_hi,
                            (uint8_t)0, (uint32_t)nb);
+        hipLaunchKernelGGL(k_fill_u8, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.blabel,
+                           (uint8_t)0, (uint32_t)nb);
         if (ctx->d_mask) {
             // a previous mask stays active: recompute its statistics
             MkVolView v2 = ctx->view();
@@ -329,7 +346,7 @@ static mk_status mk_mask_from_device(mk_context* ctx, const uint8_t* d_lin, int 
             MK_HIP(ctx, hipMalloc((void**)&d_f2, 4 * sizeof(unsigned long long)));
             hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, (uint64_t*)d_f2, (uint64_t)0, 4u);
             hipLaunchKernelGGL(k_brick_labels, dim3((uint32_t)((nb + 3) / 4)), dim3(256), 0, ctx->stream, v2,
-                               ctx->stats.lab_lo, ctx->stats.lab_hi, (uint32_t)nb, d_f2);
+                               ctx->stats.lab_lo, ctx->stats.lab_hi, ctx->stats.blabel, (uint32_t)nb, d_f2);
             MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
             (void)hipFree(d_f2);
         }
@@ -379,6 +396,8 @@ extern "C" mk_status mk_clear_mask(mk_context* ctx) {
     hipLaunchKernelGGL(k_fill_u64, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.lab_lo,
                        (uint64_t)1, (uint32_t)nb);
     hipLaunchKernelGGL(k_fill_u8, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.lab_hi,
+                       (uint8_t)0, (uint32_t)nb);
+    hipLaunchKernelGGL(k_fill_u8, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, ctx->stream, ctx->stats.blabel,
                        (uint8_t)0, (uint32_t)nb);
     MK_HIP(ctx, hipGetLastError());
     ctx->used_lo = 1;

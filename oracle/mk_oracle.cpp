@@ -35,6 +35,7 @@
 #include <omp.h>
 
 #include "../include/monkeygl_hip.h"
+#include "../monkeygl_amd/csrc/mk_mc_table.h"
 
 namespace {
 
@@ -420,9 +421,57 @@ static void cpr_all(int w, int h, const double* pts, const double* dirs, int inv
         for (int x = 0; x < w; ++x) out[(size_t)y * w + x] = cpr_pixel<T, Q8>(pts, dirs, invz, sampling, vol, x, y);
 }
 
+// Marching cubes with the triangle-list contract of mk_marching_cubes: cubes
+// x fastest, table order, vertices interpolated from the lower corner of
+// each edge in float. Returns the total; writes min(total, capacity).
+static uint64_t marching_cubes(const Volume& vol, const int* box, float iso, float* out, uint64_t capacity) {
+    int8_t tri[256][16];
+    uint8_t cnt[256];
+    mk_mc_expand(tri, cnt);
+    uint64_t t = 0;
+    for (int k = box[2]; k < box[5] - 1; ++k)
+        for (int j = box[1]; j < box[4] - 1; ++j)
+            for (int i = box[0]; i < box[3] - 1; ++i) {
+                float f[8];
+                int c = 0;
+                for (int n = 0; n < 8; ++n) {
+                    f[n] = (float)vol.at(i + kMkMcCorner[n][0], j + kMkMcCorner[n][1], k + kMkMcCorner[n][2]);
+                    if (f[n] < iso) c |= 1 << n;
+                }
+                for (int m = 0; m < cnt[c]; ++m, ++t) {
+                    if (t >= capacity) continue;
+                    for (int v = 0; v < 3; ++v) {
+                        int e = tri[c][3 * m + v];
+                        int a = kMkMcEdge[e][0], b = kMkMcEdge[e][1];
+                        int sa = kMkMcCorner[a][0] + kMkMcCorner[a][1] + kMkMcCorner[a][2];
+                        int sb = kMkMcCorner[b][0] + kMkMcCorner[b][1] + kMkMcCorner[b][2];
+                        int lo = sa > sb ? b : a, hi = sa > sb ? a : b;
+                        float tt = (iso - f[lo]) / (f[hi] - f[lo]);
+                        float p[3] = {(float)(i + kMkMcCorner[lo][0]), (float)(j + kMkMcCorner[lo][1]),
+                                      (float)(k + kMkMcCorner[lo][2])};
+                        int axis = kMkMcCorner[lo][0] != kMkMcCorner[hi][0] ? 0 : (kMkMcCorner[lo][1] != kMkMcCorner[hi][1] ? 1 : 2);
+                        p[axis] = p[axis] + tt;
+                        for (int d = 0; d < 3; ++d) out[9 * t + 3 * v + d] = p[d];
+                    }
+                }
+            }
+    return t;
+}
+
 }  // namespace
 
 extern "C" {
+
+uint64_t mko_marching_cubes(const int16_t* v, int nx, int ny, int nz, const int* box, float iso, float* out,
+                            uint64_t capacity) {
+    Volume vol{v, nullptr, nx, ny, nz, nullptr};
+    return marching_cubes(vol, box, iso, out, capacity);
+}
+
+// The expanded triangle table, for pinning against the reference's table.
+void mko_mc_table(int8_t* tri, uint8_t* cnt) {
+    mk_mc_expand((int8_t(*)[16])tri, cnt);
+}
 
 // precision: 0 = FP64 oracle, 1 = float CPU baseline. weights8: quantise
 // filter weights to 8 fractional bits (float only).

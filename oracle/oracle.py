@@ -33,6 +33,10 @@ def lib() -> C.CDLL:
         L.mko_render_plane.argtypes = [C.POINTER(PlaneParams), vp, i, i, i, i, i, vp, vp]
         L.mko_render_cpr.restype = i
         L.mko_render_cpr.argtypes = [i, i, vp, vp, i, i, vp, i, i, i, i, i, vp, vp]
+        L.mko_marching_cubes.restype = C.c_uint64
+        L.mko_marching_cubes.argtypes = [vp, i, i, i, vp, C.c_float, vp, C.c_uint64]
+        L.mko_mc_table.restype = None
+        L.mko_mc_table.argtypes = [vp, vp]
         L.mko_tex3_linear.restype = C.c_double
         L.mko_tex3_linear.argtypes = [vp, i, i, i, C.c_double, C.c_double, C.c_double]
         L.mko_tex1_linear.restype = i
@@ -123,3 +127,22 @@ def tex1_linear(lut: np.ndarray, s: float) -> np.ndarray:
     out = np.zeros(4, np.float64)
     lib().mko_tex1_linear(a.ctypes.data, a.shape[0], s, out.ctypes.data)
     return out
+
+
+def marching_cubes(vol_zyx, box, iso: float) -> np.ndarray:
+    """Triangles (n, 3, 3) float32, same contract as mk_marching_cubes."""
+    v = np.ascontiguousarray(vol_zyx, dtype=np.int16)
+    nz, ny, nx = v.shape
+    b = np.ascontiguousarray(box, dtype=np.int32)
+    n = lib().mko_marching_cubes(v.ctypes.data, nx, ny, nz, b.ctypes.data, iso, None, 0)
+    out = np.empty((n, 3, 3), np.float32)
+    if n:
+        lib().mko_marching_cubes(v.ctypes.data, nx, ny, nz, b.ctypes.data, iso, out.ctypes.data, n)
+    return out
+
+
+def mc_table():
+    tri = np.zeros((256, 16), np.int8)
+    cnt = np.zeros(256, np.uint8)
+    lib().mko_mc_table(tri.ctypes.data, cnt.ctypes.data)
+    return tri, cnt
