@@ -77,6 +77,11 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-footprint", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after the timed frames, copy this many MiB device-to-device once (a known-size "
+                         "kernel against which FETCH_SIZE/WRITE_SIZE counters are scaled)")
+    ap.add_argument("--traffic", type=str, default=str(ROOT / "profiles" / "traffic_vr_body1024.json"),
+                    help="measured HBM bytes per frame from rocprof counters (see the file), if present")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -128,6 +133,16 @@ def main() -> int:
     log(f"{K} frames in {elapsed * 1e3:.1f} ms -> {fps:.1f} fps; kernel mean {kernel_ms:.3f} ms "
         f"(min {min(per):.3f}, max {max(per):.3f})")
 
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {n} bytes done")
+        del a, b
+
     # algorithmic bytes per frame, on the exact traversal of every view
     bytes_per_view = []
     if not args.no_footprint:
@@ -146,6 +161,10 @@ def main() -> int:
         torch.cuda.synchronize()
     b_frame = float(np.mean([bytes_per_view[i % len(views)] for i in range(K)])) if bytes_per_view else None
     achieved = b_frame / (kernel_ms * 1e-3) / 1e9 if b_frame else None
+    traffic = None
+    tp = Path(args.traffic)
+    if tp.exists() and args.size == 1024:
+        traffic = json.loads(tp.read_text()).get("hbm_bytes_per_frame")
 
     # parity band and CPU baseline on view 1 (an oblique one)
     p = views[1 % len(views)]
@@ -167,20 +186,25 @@ def main() -> int:
         parity = {"oracle": "fp64", "rows": band, "frac_within_1_255": float((d <= 1).mean()),
                   "max_diff": int(d.max()), "oracle_s": round(t_ref, 2)}
         log(f"parity on {band} rows: {parity}")
-        # CPU baseline: float instance with 8-bit weights, as many rows as fit the budget
-        rows_done, t_cpu = 0, 0.0
-        y = 0
-        step_rows = 32
-        while t_cpu < args.cpu_seconds and rows_done < H_:
-            t = time.perf_counter()
-            O.render_rows(p, vol, luts, y, y + step_rows, mask_zyx=mask, precision="f32", weights8=True)
-            t_cpu += time.perf_counter() - t
-            rows_done += step_rows
-            y = (y + 7 * step_rows) % (H_ - step_rows)
+        # CPU baseline: float instance with 8-bit weights over whole frames of the same views
+        # (bands of 64 rows) until the time budget is spent
+        rows_done, t_cpu, nviews = 0, 0.0, 0
+        band_rows = 64
+        for q in views:
+            for y in range(0, H_, band_rows):
+                t = time.perf_counter()
+                O.render_rows(q, vol, luts, y, min(H_, y + band_rows), mask_zyx=mask, precision="f32", weights8=True)
+                t_cpu += time.perf_counter() - t
+                rows_done += min(H_, y + band_rows) - y
+                if t_cpu >= args.cpu_seconds:
+                    break
+            nviews += 1
+            if t_cpu >= args.cpu_seconds:
+                break
         cpu_fps = (rows_done / H_) / t_cpu
         cpu = {"value": round(cpu_fps, 4), "unit": "frames/s", "cores": ncores, "kind": "port",
-               "sample": f"{rows_done} of {H_} rows of one {W_}x{H_} view, float oracle with 8-bit "
-                         f"filter weights, {t_cpu:.1f}s on {ncores} threads"}
+               "sample": f"{rows_done / H_:.2f} frames ({rows_done} rows of {W_}x{H_} over {nviews} views), float "
+                         f"oracle with 8-bit filter weights, {t_cpu:.1f}s on {ncores} threads"}
         log(f"cpu baseline: {cpu}")
 
     line = {
@@ -203,7 +227,7 @@ def main() -> int:
         "kernel_ms": round(kernel_ms, 4),
         "roofline": None if achieved is None else {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "bytes_per_frame": int(b_frame), "kernel": "k_vr"},
         "parity": parity,
         "cpu_baseline": cpu,
