@@ -16,8 +16,11 @@
  * leaves a message retrievable with mk_last_error(ctx). Nothing ever exits
  * the process (the reference calls exit() from checkCudaErrors).
  *
- * Threading: a context is used by one host thread at a time. Calls on one
- * context are ordered on the context's stream. Different contexts are
+ * Threading: a context is used by one host thread at a time. Synchronous
+ * calls run on the context's own stream. Every call that takes a `stream`
+ * argument enqueues its kernels on that hipStream_t (NULL = HIP's default
+ * stream, as everywhere in HIP) after the context's pending preparation work
+ * (skip tables, uploads), and returns without waiting. Different contexts are
  * independent.
  */
 #ifndef MONKEYGL_HIP_H
@@ -166,8 +169,8 @@ mk_status mk_set_transfer_func(mk_context* ctx, int label, const float* rgba, in
  * of width*height*3 bytes (RGB24), synchronously. */
 mk_status mk_render(mk_context* ctx, const mk_render_params* p, uint8_t* host_rgb);
 
-/* Same frame into device memory (width*height*3 bytes) on the given stream
- * (NULL = the context's stream); returns after enqueueing. */
+/* Same frame into device memory (width*height*3 bytes) on the given stream;
+ * returns after enqueueing. */
 mk_status mk_render_async(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb, void* stream);
 
 /* Measurement variant of mk_render_async (no reference counterpart): renders
@@ -208,6 +211,28 @@ mk_status mk_marching_cubes(mk_context* ctx, const int* box, float iso, mk_trian
  * dev_out and the total (uint64) to dev_count (may be NULL). */
 mk_status mk_marching_cubes_async(mk_context* ctx, const int* box, float iso, mk_triangle* dev_out,
                                   uint64_t capacity, uint64_t* dev_count, void* stream);
+
+/* ---- differentiable transfer function (no reference counterpart) -------- */
+/* Fitting rule: the VR compositing of mk_render on the same float step
+ * lattice (base step 1/depth * step_scale), with a fixed step (no
+ * refinement), every sample with alpha > 0 composited, no early exit, no
+ * label caps (label 0's window from p->alpha_wwwl[0]; the mask is ignored),
+ * sample alpha = min(1, opacity_scale * LUT_alpha) and the float RGB sum as
+ * output. dev_lut holds n RGBA float entries (2 <= n <= 4096) in device
+ * memory: the LUT is a parameter, not the context's label LUT. */
+
+/* Forward: width*height*3 floats (RGB) into dev_rgb. */
+mk_status mk_tf_forward(mk_context* ctx, const mk_render_params* p, const float* dev_lut, int n,
+                        float opacity_scale, float* dev_rgb, void* stream);
+
+/* Adjoint: given dL/dRGB (width*height*3 floats), ADDS dL/dLUT (n*4 doubles)
+ * and dL/d(opacity_scale) (1 double) into the caller's zeroed or running
+ * accumulators. Each ray is re-marched (first pass: its colour; second pass:
+ * the gradients, un-compositing the rest of the ray with a (1 - alpha)
+ * division); nothing per sample is stored. */
+mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, const float* dev_lut, int n,
+                         float opacity_scale, const float* dev_grad_rgb, double* dev_grad_lut,
+                         double* dev_grad_scale, void* stream);
 
 #ifdef __cplusplus
 }

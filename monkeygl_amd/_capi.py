@@ -27,6 +27,7 @@ SYMBOLS = [
     "mk_copy_mask_data", "mk_copy_mask_data_device", "mk_clear_mask", "mk_set_transfer_func",
     "mk_render", "mk_render_async", "mk_render_footprint", "mk_render_plane", "mk_render_plane_async",
     "mk_render_cpr", "mk_render_cpr_async", "mk_marching_cubes", "mk_marching_cubes_async",
+    "mk_tf_forward", "mk_tf_backward",
 ]
 
 
@@ -106,6 +107,8 @@ def load() -> C.CDLL:
         "mk_render_cpr_async": (st, [vp, i, i, vp, vp, i, i, vp, vp]),
         "mk_marching_cubes": (st, [vp, vp, C.c_float, vp, C.c_uint64, C.POINTER(C.c_uint64)]),
         "mk_marching_cubes_async": (st, [vp, vp, C.c_float, vp, C.c_uint64, vp, vp]),
+        "mk_tf_forward": (st, [vp, C.POINTER(RenderParams), vp, i, C.c_float, vp, vp]),
+        "mk_tf_backward": (st, [vp, C.POINTER(RenderParams), vp, i, C.c_float, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -189,11 +192,11 @@ class Context:
         return out
 
     def render_async(self, p: RenderParams, dev_out, stream: int = 0) -> None:
-        self._check(self.lib.mk_render_async(self.h, C.byref(p), _ptr(dev_out), stream or None))
+        self._check(self.lib.mk_render_async(self.h, C.byref(p), _ptr(dev_out), stream))
 
     def render_footprint(self, p: RenderParams, dev_out, dev_touch_vol, dev_touch_mask=None, stream: int = 0) -> None:
         self._check(self.lib.mk_render_footprint(self.h, C.byref(p), _ptr(dev_out), _ptr(dev_touch_vol),
-                                                 _ptr(dev_touch_mask), stream or None))
+                                                 _ptr(dev_touch_mask), stream))
 
     def render_plane(self, p: PlaneParams) -> np.ndarray:
         out = np.empty((p.height, p.width), np.int16)
@@ -201,7 +204,7 @@ class Context:
         return out
 
     def render_plane_async(self, p: PlaneParams, dev_out, stream: int = 0) -> None:
-        self._check(self.lib.mk_render_plane_async(self.h, C.byref(p), _ptr(dev_out), stream or None))
+        self._check(self.lib.mk_render_plane_async(self.h, C.byref(p), _ptr(dev_out), stream))
 
     def render_cpr(self, width: int, points: np.ndarray, dirs: np.ndarray, invert_z: bool,
                    sampling: int = MK_SAMPLE_LINEAR) -> np.ndarray:
@@ -215,7 +218,7 @@ class Context:
     def render_cpr_async(self, width: int, height: int, dev_points, dev_dirs, invert_z: bool, sampling: int,
                          dev_out, stream: int = 0) -> None:
         self._check(self.lib.mk_render_cpr_async(self.h, width, height, _ptr(dev_points), _ptr(dev_dirs),
-                                                 int(invert_z), sampling, _ptr(dev_out), stream or None))
+                                                 int(invert_z), sampling, _ptr(dev_out), stream))
 
     def marching_cubes(self, box, iso: float, capacity: int | None = None) -> np.ndarray:
         """Triangles (n, 3, 3) float32 in voxel units; box = (x0, y0, z0, x1, y1, z1)."""
@@ -231,4 +234,12 @@ class Context:
     def marching_cubes_async(self, box, iso: float, dev_out, capacity: int, dev_count, stream: int = 0) -> None:
         b = np.ascontiguousarray(box, dtype=np.int32)
         self._check(self.lib.mk_marching_cubes_async(self.h, _ptr(b), iso, _ptr(dev_out), capacity, _ptr(dev_count),
-                                                     stream or None))
+                                                     stream))
+
+    def tf_forward(self, p: RenderParams, dev_lut, n: int, scale: float, dev_rgb, stream: int = 0) -> None:
+        self._check(self.lib.mk_tf_forward(self.h, C.byref(p), _ptr(dev_lut), n, scale, _ptr(dev_rgb), stream))
+
+    def tf_backward(self, p: RenderParams, dev_lut, n: int, scale: float, dev_grad_rgb, dev_grad_lut, dev_grad_scale,
+                    stream: int = 0) -> None:
+        self._check(self.lib.mk_tf_backward(self.h, C.byref(p), _ptr(dev_lut), n, scale, _ptr(dev_grad_rgb),
+                                            _ptr(dev_grad_lut), _ptr(dev_grad_scale), stream))

@@ -104,6 +104,11 @@ mk_status mk_fail(mk_context* ctx, mk_status st, const char* fmt, ...);
 mk_status mk_hip_fail(mk_context* ctx, hipError_t e, const char* what);
 mk_status mk_ensure_buffer(mk_context* ctx, void** buf, size_t* cap, size_t bytes);
 mk_status mk_upload_luts(mk_context* ctx);
+// Makes stream s wait for the work queued so far on the context's own stream
+// (skip tables, lattice and table uploads); no-op when s is that stream.
+mk_status mk_order_after_context(mk_context* ctx, hipStream_t s);
+// uploads the float lattice a_{k+1} = a_k + l1 (a_0 = 0) up to past 1.732
+mk_status mk_ensure_steps(mk_context* ctx, float l1);
 
 #define MK_HIP(ctx, call)                                          \
     do {                                                           \

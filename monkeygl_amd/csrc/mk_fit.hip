@@ -1,0 +1,352 @@
+// mk_fit.hip -- differentiable transfer-function rendering and its adjoint.
+//
+// Fitting rule: the reference's front-to-back compositing (core/kernel.cu:49-81,
+// 195-252) on its float step lattice, with the choices that make the image a
+// smooth function of the transfer function:
+//   * a fixed step (no getNextStep refinement: its lattice jumps with alpha),
+//   * every sample with alpha > 0 is composited (no 0.0005 threshold),
+//   * no early exit and no per-label caps (label 0 of the palette only),
+//   * sample alpha = min(1, s * LUT_alpha) with a learnable opacity scale s,
+//   * the same 0.35 / 0.8 / 0.16 headlight shading from the volume gradient,
+//   * output = the float RGB sum (no saturation, no 8-bit rounding).
+// For a ray with samples i (colour c_i, alpha a_i, shading k_i, transmittance
+// T_i = prod_{j<i}(1 - a_j)): C = sum_i T_i a_i k_i c_i.rgb and
+//   dC/dc_i.rgb = T_i a_i k_i,
+//   dC/da_i     = T_i k_i c_i.rgb - (C - C_{<=i}) / (1 - a_i),
+// where C_{<=i} is the colour composited up to and including sample i. The
+// adjoint re-marches each ray: a first pass gives C, a second pass
+// un-composites the remainder with that (1 - a) division and scatters the
+// LUT and scale gradients into per-workgroup LDS (double), then into HBM with
+// double atomics. No sample is stored.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "mk_context.h"
+
+namespace {
+
+struct FitArgs {
+    MkVolView v;
+    const float4* lut;  // n entries, device
+    int n;
+    float scale;        // opacity scale s
+    const float* steps;
+    int nsteps;
+    int width, height;
+    int invert_z;
+    float mls[3];
+    float sv[3];
+    float xT, yT;
+    double dscale;      // camera zoom
+    float M[9];
+    mk_voi voi;
+    float wl, ww;       // label 0 window
+    float l1;
+    int max_iter;
+};
+
+__device__ inline void fit_pos(const FitArgs& A, float u, float v, float acc, float p[3]) {
+    float fy = (float)(((double)acc - 0.866) * A.dscale);
+    float x = u * A.M[0] + fy * A.M[1] + v * A.M[2];
+    float y = u * A.M[3] + fy * A.M[4] + v * A.M[5];
+    float z = u * A.M[6] + fy * A.M[7] + v * A.M[8];
+    p[0] = x * A.mls[0] + 0.5f;
+    p[1] = y * A.mls[1] + 0.5f;
+    p[2] = z * A.mls[2] + 0.5f;
+    if (A.invert_z) p[2] = 1.0f - p[2];
+}
+
+__device__ inline bool fit_voi(const FitArgs& A, const float p[3]) {
+    float fx = fminf(fmaxf(p[0] * (float)A.v.nx, -1e9f), 1e9f);
+    float fy = fminf(fmaxf(p[1] * (float)A.v.ny, -1e9f), 1e9f);
+    float fz = fminf(fmaxf(p[2] * (float)A.v.nz, -1e9f), 1e9f);
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    return !(ix < A.voi.left || ix > A.voi.right || iy < A.voi.posterior || iy > A.voi.anterior || iz < A.voi.head ||
+             iz > A.voi.foot);
+}
+
+// Shading factor 0.35 + (d > 0 ? 0.8 d + 0.16 d^8 : 0) of the normalized
+// one-voxel central-difference gradient against the headlight.
+__device__ inline float fit_shade(const FitArgs& A, const float p[3], float3 L) {
+    float N[3];
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+        float q0[3] = {p[0], p[1], p[2]}, q1[3] = {p[0], p[1], p[2]};
+        q0[i] += A.sv[i];
+        q1[i] -= A.sv[i];
+        N[i] = mk_sample_hu(A.v, q0[0], q0[1], q0[2]) - mk_sample_hu(A.v, q1[0], q1[1], q1[2]);
+    }
+    if (A.invert_z) N[2] = -N[2];
+    float r = rsqrtf(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+    float d = (N[0] * L.x + N[1] * L.y + N[2] * L.z) * r;
+    float k = 0.35f;
+    if (d > 0.f) {
+        float d2 = d * d, d4 = d2 * d2;
+        k += d * 0.8f + 0.16f * (d4 * d4);
+    }
+    return k;
+}
+
+struct Sample {
+    int i0, i1;
+    float f;      // weight of entry i1
+    float4 c;     // LUT colour
+    float a;      // alpha after scale and clamp
+    bool grad_ok; // the clamp is inactive: 0 <= s*alpha < 1 (one-sided at 0)
+};
+
+__device__ inline Sample fit_sample(const FitArgs& A, const float4* s_lut, const float p[3]) {
+    Sample S;
+    float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
+    t = (t - A.wl) / A.ww + 0.5f;
+    t = t > 1.0f ? 1.0f : (t >= -1.0f ? t : -1.0f);
+    mk_split(t, A.n, S.i0, S.i1, S.f);
+    const float4 c0 = s_lut[S.i0], c1 = s_lut[S.i1];
+    S.c = make_float4(c0.x + S.f * (c1.x - c0.x), c0.y + S.f * (c1.y - c0.y), c0.z + S.f * (c1.z - c0.z),
+                      c0.w + S.f * (c1.w - c0.w));
+    const float a = A.scale * S.c.w;
+    S.grad_ok = a >= 0.0f && a < 1.0f;
+    S.a = fminf(fmaxf(a, 0.0f), 1.0f);
+    return S;
+}
+
+__device__ inline bool fit_ray(const FitArgs& A, int x, int y, float& u, float& v, float& acc, float& a_out) {
+    u = ((float)x - (float)A.width / 2.0f - A.xT) / (float)A.height;
+    v = ((float)y - (float)A.height / 2.0f - A.yT) / (float)A.height;
+    float p0[3];
+    fit_pos(A, u, v, 0.0f, p0);
+    float d[3] = {A.M[1] * (float)A.dscale * A.mls[0], A.M[4] * (float)A.dscale * A.mls[1],
+                  A.M[7] * (float)A.dscale * A.mls[2]};
+    if (A.invert_z) d[2] = -d[2];
+    const int lo_i[3] = {A.voi.left, A.voi.posterior, A.voi.head};
+    const int hi_i[3] = {A.voi.right, A.voi.anterior, A.voi.foot};
+    const int n[3] = {A.v.nx, A.v.ny, A.v.nz};
+    float t0 = 0.0f, t1 = 1.7321f;
+    for (int i = 0; i < 3; ++i) {
+        float lo = ((float)lo_i[i] - 1.0f) / (float)n[i] - 1e-3f;
+        float hi = ((float)hi_i[i] + 1.0f) / (float)n[i] + 1e-3f;
+        if (fabsf(d[i]) < 1e-12f) {
+            if (!(p0[i] >= lo && p0[i] <= hi)) return false;
+        } else {
+            float ta = (lo - p0[i]) / d[i], tb = (hi - p0[i]) / d[i];
+            t0 = fmaxf(t0, fminf(ta, tb));
+            t1 = fminf(t1, fmaxf(ta, tb));
+        }
+    }
+    t0 = fmaxf(t0 - 4.0f * A.l1, 0.0f);
+    a_out = t1 + 4.0f * A.l1;
+    if (t0 > a_out) return false;
+    int k0 = (int)(t0 / A.l1) - 2;
+    k0 = k0 < 0 ? 0 : (k0 >= A.nsteps ? A.nsteps - 1 : k0);
+    acc = A.steps[k0];
+    return true;
+}
+
+__device__ inline float3 fit_light(const FitArgs& A) {
+    float3 l = make_float3(A.M[1], A.M[4], A.M[7]);
+    float r = rsqrtf(l.x * l.x + l.y * l.y + l.z * l.z);
+    return make_float3(l.x * r, l.y * r, l.z * r);
+}
+
+__device__ inline void fit_tile(const FitArgs& A, int& x, int& y) {
+    const uint32_t tiles_x = (A.width + 15) / 16;
+    const uint32_t tiles = tiles_x * ((A.height + 15) / 16);
+    uint32_t t = mk_xcd_tile(blockIdx.x, tiles);
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    x = (int)(t % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    y = (int)(t / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+}
+
+// Forward: float RGB per pixel (width*height*3).
+__global__ void __launch_bounds__(256) k_fit_forward(FitArgs A, float* __restrict__ out) {
+    extern __shared__ float4 s_lut[];
+    for (int i = threadIdx.x; i < A.n; i += blockDim.x) s_lut[i] = A.lut[i];
+    __syncthreads();
+    int x, y;
+    fit_tile(A, x, y);
+    if (x >= A.width || y >= A.height) return;
+    double C[3] = {0.0, 0.0, 0.0}, T = 1.0;
+    float u, v, acc, a_out;
+    if (fit_ray(A, x, y, u, v, acc, a_out)) {
+        const float3 L = fit_light(A);
+        for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
+            float p[3];
+            fit_pos(A, u, v, acc, p);
+            acc += A.l1;
+            if (!fit_voi(A, p)) continue;
+            const Sample S = fit_sample(A, s_lut, p);
+            if (!(S.a > 0.0f)) continue;
+            const float k = fit_shade(A, p, L);
+            const double w = T * (double)S.a * (double)k;
+            C[0] += w * S.c.x;
+            C[1] += w * S.c.y;
+            C[2] += w * S.c.z;
+            T *= 1.0 - (double)S.a;
+        }
+    }
+    float* o = out + 3 * ((size_t)y * A.width + x);
+    o[0] = (float)C[0];
+    o[1] = (float)C[1];
+    o[2] = (float)C[2];
+}
+
+// Adjoint: given g = dL/dC per pixel, accumulate dL/dLUT (n x 4) and dL/ds.
+__global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __restrict__ grad_img,
+                                                      double* __restrict__ grad_lut, double* __restrict__ grad_scale) {
+    extern __shared__ float4 smem[];
+    float4* s_lut = smem;
+    double* s_g = (double*)(smem + A.n);  // 4*n + 1
+    for (int i = threadIdx.x; i < A.n; i += blockDim.x) s_lut[i] = A.lut[i];
+    for (int i = threadIdx.x; i < 4 * A.n + 1; i += blockDim.x) s_g[i] = 0.0;
+    __syncthreads();
+    int x, y;
+    fit_tile(A, x, y);
+    float u, v, acc0, a_out;
+    if (x < A.width && y < A.height && fit_ray(A, x, y, u, v, acc0, a_out)) {
+        const float* gp = grad_img + 3 * ((size_t)y * A.width + x);
+        const double g[3] = {gp[0], gp[1], gp[2]};
+        const float3 L = fit_light(A);
+        if (g[0] != 0.0 || g[1] != 0.0 || g[2] != 0.0) {
+            // pass 1: the final colour
+            double C[3] = {0.0, 0.0, 0.0}, T = 1.0;
+            float acc = acc0;
+            for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
+                float p[3];
+                fit_pos(A, u, v, acc, p);
+                acc += A.l1;
+                if (!fit_voi(A, p)) continue;
+                const Sample S = fit_sample(A, s_lut, p);
+                if (!(S.a > 0.0f)) continue;
+                const double w = T * (double)S.a * (double)fit_shade(A, p, L);
+                C[0] += w * S.c.x;
+                C[1] += w * S.c.y;
+                C[2] += w * S.c.z;
+                T *= 1.0 - (double)S.a;
+            }
+            // pass 2: gradients, un-compositing the remainder of the ray
+            double Cp[3] = {0.0, 0.0, 0.0};
+            T = 1.0;
+            acc = acc0;
+            double gs = 0.0;
+            for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
+                float p[3];
+                fit_pos(A, u, v, acc, p);
+                acc += A.l1;
+                if (!fit_voi(A, p)) continue;
+                const Sample S = fit_sample(A, s_lut, p);
+                // samples with zero alpha still move the image when their LUT
+                // entries grow: da > 0 from 0 is a smooth onset
+                const double k = (double)fit_shade(A, p, L);
+                const double a = S.a;
+                const double w = T * a * k;
+                const double c[3] = {S.c.x, S.c.y, S.c.z};
+                for (int ch = 0; ch < 3; ++ch) Cp[ch] += w * c[ch];
+                // dL/dc.rgb
+                const double drgb[3] = {g[0] * w, g[1] * w, g[2] * w};
+                // dL/da
+                // an opaque sample (alpha clamped at 1) hides the rest and passes no
+                // gradient to its alpha
+                double da = 0.0;
+                if (S.grad_ok)
+                    for (int ch = 0; ch < 3; ++ch) da += g[ch] * (T * k * c[ch] - (C[ch] - Cp[ch]) / (1.0 - a));
+                T *= 1.0 - a;
+                const double dw = da;
+                // alpha = s * c.w (unclamped region): dL/dc.w = s dL/da, dL/ds = c.w dL/da
+                const double dcw = dw * (double)A.scale;
+                gs += dw * (double)S.c.w;
+                const double f = S.f;
+                const double d4[4] = {drgb[0], drgb[1], drgb[2], dcw};
+                for (int ch = 0; ch < 4; ++ch) {
+                    if (d4[ch] == 0.0) continue;
+                    atomicAdd(&s_g[4 * S.i0 + ch], (1.0 - f) * d4[ch]);
+                    atomicAdd(&s_g[4 * S.i1 + ch], f * d4[ch]);
+                }
+                if (T == 0.0) break;
+            }
+            atomicAdd(&s_g[4 * A.n], gs);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 4 * A.n; i += blockDim.x)
+        if (s_g[i] != 0.0) atomicAdd(&grad_lut[i], s_g[i]);
+    if (threadIdx.x == 0 && s_g[4 * A.n] != 0.0) atomicAdd(grad_scale, s_g[4 * A.n]);
+}
+
+mk_status fit_prepare(mk_context* ctx, const mk_render_params* p, const float* dev_lut, int n, float scale, FitArgs& A) {
+    if (!p) return mk_fail(ctx, MK_ERR_ARG, "render params are null");
+    if (!ctx->d_vox) return mk_fail(ctx, MK_ERR_STATE, "fit before any volume was uploaded");
+    if (!dev_lut || n < 2 || n > 4096) return mk_fail(ctx, MK_ERR_ARG, "fit LUT must have 2..4096 entries (got %d)", n);
+    if (p->width <= 0 || p->height <= 0 || p->width > 16384 || p->height > 16384)
+        return mk_fail(ctx, MK_ERR_ARG, "bad image size %dx%d", p->width, p->height);
+    if (!(p->step_scale > 0.0f) || !std::isfinite(p->step_scale) || ctx->nz / p->step_scale > 1e5f)
+        return mk_fail(ctx, MK_ERR_ARG, "bad step_scale %g", (double)p->step_scale);
+    if (!std::isfinite(scale) || scale < 0.0f) return mk_fail(ctx, MK_ERR_ARG, "bad opacity scale %g", (double)scale);
+    if (!(p->alpha_wwwl[0].ww != 0.0f)) return mk_fail(ctx, MK_ERR_ARG, "label 0 window width is zero");
+    const float l1 = 1.0f / (float)ctx->nz * p->step_scale;
+    mk_status st = mk_ensure_steps(ctx, l1);
+    if (st) return st;
+    A.v = ctx->view();
+    A.lut = (const float4*)dev_lut;
+    A.n = n;
+    A.scale = scale;
+    A.steps = ctx->d_steps;
+    A.nsteps = ctx->steps_n;
+    A.width = p->width;
+    A.height = p->height;
+    A.invert_z = p->invert_z != 0;
+    for (int i = 0; i < 3; ++i) {
+        A.mls[i] = p->max_len_spacing[i];
+        A.sv[i] = p->spacing_voxel[i];
+    }
+    A.xT = p->x_translate;
+    A.yT = p->y_translate;
+    A.dscale = (double)p->scale;
+    memcpy(A.M, p->transform, sizeof(A.M));
+    A.voi = p->voi;
+    A.wl = p->alpha_wwwl[0].wl;
+    A.ww = p->alpha_wwwl[0].ww;
+    A.l1 = l1;
+    A.max_iter = ctx->steps_n + 64;
+    return MK_OK;
+}
+
+}  // namespace
+
+extern "C" mk_status mk_tf_forward(mk_context* ctx, const mk_render_params* p, const float* dev_lut, int n,
+                                   float opacity_scale, float* dev_rgb, void* stream) {
+    if (!ctx) return MK_ERR_ARG;
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    FitArgs A;
+    mk_status st = fit_prepare(ctx, p, dev_lut, n, opacity_scale, A);
+    if (st) return st;
+    if (!dev_rgb) return mk_fail(ctx, MK_ERR_ARG, "null output");
+    hipStream_t s = (hipStream_t)stream;
+    st = mk_order_after_context(ctx, s);
+    if (st) return st;
+    const uint32_t tiles = (uint32_t)(((p->width + 15) / 16) * ((p->height + 15) / 16));
+    hipLaunchKernelGGL(k_fit_forward, dim3(tiles), dim3(256), (size_t)n * sizeof(float4), s, A, dev_rgb);
+    MK_HIP(ctx, hipGetLastError());
+    return MK_OK;
+}
+
+extern "C" mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, const float* dev_lut, int n,
+                                    float opacity_scale, const float* dev_grad_rgb, double* dev_grad_lut,
+                                    double* dev_grad_scale, void* stream) {
+    if (!ctx) return MK_ERR_ARG;
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    FitArgs A;
+    mk_status st = fit_prepare(ctx, p, dev_lut, n, opacity_scale, A);
+    if (st) return st;
+    if (!dev_grad_rgb || !dev_grad_lut || !dev_grad_scale) return mk_fail(ctx, MK_ERR_ARG, "null gradient buffer");
+    hipStream_t s = (hipStream_t)stream;
+    st = mk_order_after_context(ctx, s);
+    if (st) return st;
+    const uint32_t tiles = (uint32_t)(((p->width + 15) / 16) * ((p->height + 15) / 16));
+    const size_t lds = (size_t)n * sizeof(float4) + (4 * (size_t)n + 1) * sizeof(double);
+    if (lds > 64 * 1024) return mk_fail(ctx, MK_ERR_ARG, "LUT of %d entries exceeds the LDS budget of the adjoint", n);
+    hipLaunchKernelGGL(k_fit_backward, dim3(tiles), dim3(256), lds, s, A, dev_grad_rgb, dev_grad_lut, dev_grad_scale);
+    MK_HIP(ctx, hipGetLastError());
+    return MK_OK;
+}

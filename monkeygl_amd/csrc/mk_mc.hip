@@ -208,7 +208,9 @@ extern "C" mk_status mk_marching_cubes_async(mk_context* ctx, const int* box, fl
     mk_status st = prepare(ctx, box, iso, A, nblocks);
     if (st) return st;
     if (capacity > 0 && !dev_out) return mk_fail(ctx, MK_ERR_ARG, "null triangle buffer");
-    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
+    st = mk_order_after_context(ctx, s);
+    if (st) return st;
     st = launch_count(ctx, A, nblocks, s);
     if (st) return st;
     uint32_t* sums = (uint32_t*)ctx->d_mc_scratch;

@@ -123,7 +123,9 @@ extern "C" mk_status mk_render_plane_async(mk_context* ctx, const mk_plane_param
     }
     A.ps = p->pixel_spacing;
     A.half = p->half_num;
-    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
+    st = mk_order_after_context(ctx, s);
+    if (st) return st;
     dim3 grid((uint32_t)(((p->width + 15) / 16) * ((p->height + 15) / 16))), block(256);
     const bool nn = p->sampling == MK_SAMPLE_NEAREST;
     switch (p->mpr_type) {
@@ -153,7 +155,7 @@ extern "C" mk_status mk_render_plane(mk_context* ctx, const mk_plane_params* p, 
     size_t bytes = (size_t)p->width * p->height * sizeof(int16_t);
     st = mk_ensure_buffer(ctx, &ctx->d_stage, &ctx->stage_cap, bytes);
     if (st) return st;
-    st = mk_render_plane_async(ctx, p, (int16_t*)ctx->d_stage, nullptr);
+    st = mk_render_plane_async(ctx, p, (int16_t*)ctx->d_stage, ctx->stream);
     if (st) return st;
     MK_HIP(ctx, hipMemcpyAsync(host_out, ctx->d_stage, bytes, hipMemcpyDeviceToHost, ctx->stream));
     MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -171,7 +173,9 @@ extern "C" mk_status mk_render_cpr_async(mk_context* ctx, int width, int height,
     if (sampling != MK_SAMPLE_LINEAR && sampling != MK_SAMPLE_NEAREST)
         return mk_fail(ctx, MK_ERR_ARG, "unknown sampling %d", sampling);
     MK_HIP(ctx, hipSetDevice(ctx->device));
-    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    hipStream_t s = (hipStream_t)stream;
+    mk_status st = mk_order_after_context(ctx, s);
+    if (st) return st;
     dim3 grid((uint32_t)(((width + 15) / 16) * ((height + 15) / 16))), block(256);
     if (sampling == MK_SAMPLE_NEAREST)
         hipLaunchKernelGGL(k_cpr<true>, grid, block, 0, s, ctx->view(), dev_points, dev_dirs, width, height,
@@ -201,7 +205,7 @@ extern "C" mk_status mk_render_cpr(mk_context* ctx, int width, int height, const
     double* d_dirs = d_pts + 3 * (size_t)height;
     MK_HIP(ctx, hipMemcpyAsync(d_pts, points, geo, hipMemcpyHostToDevice, ctx->stream));
     MK_HIP(ctx, hipMemcpyAsync(d_dirs, dirs, geo, hipMemcpyHostToDevice, ctx->stream));
-    st = mk_render_cpr_async(ctx, width, height, d_pts, d_dirs, invert_z, sampling, (int16_t*)ctx->d_stage, nullptr);
+    st = mk_render_cpr_async(ctx, width, height, d_pts, d_dirs, invert_z, sampling, (int16_t*)ctx->d_stage, ctx->stream);
     if (st) return st;
     MK_HIP(ctx, hipMemcpyAsync(host_out, ctx->d_stage, bytes, hipMemcpyDeviceToHost, ctx->stream));
     MK_HIP(ctx, hipStreamSynchronize(ctx->stream));

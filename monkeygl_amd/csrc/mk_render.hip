@@ -580,25 +580,6 @@ __global__ void __launch_bounds__(256) k_macro(const uint8_t* __restrict__ occ, 
     else occ2[m] = any;
 }
 
-mk_status ensure_steps(mk_context* ctx, float l1) {
-    if (ctx->steps_l1 == l1 && ctx->d_steps) return MK_OK;
-    std::vector<float> a;
-    a.reserve((size_t)(1.75 / l1) + 8);
-    float acc = 0.0f;
-    a.push_back(acc);
-    while ((double)acc < 1.74) {
-        acc += l1;
-        a.push_back(acc);
-    }
-    mk_status st = mk_ensure_buffer(ctx, (void**)&ctx->d_steps, &ctx->steps_cap, a.size() * sizeof(float));
-    if (st) return st;
-    MK_HIP(ctx, hipMemcpyAsync(ctx->d_steps, a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->steps_l1 = l1;
-    ctx->steps_n = (int)a.size();
-    return MK_OK;
-}
-
 mk_status ensure_occupancy(mk_context* ctx, const mk_render_params* p, const AwTable& T) {
     bool same = ctx->occ_valid && ctx->occ_type == p->render_type && ctx->occ_lut_version == ctx->lut_version &&
                 ctx->occ_vol_version == ctx->vol_version &&
@@ -651,7 +632,7 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     st = ensure_occupancy(ctx, p, T);
     if (st) return st;
     const float l1 = 1.0f / (float)ctx->nz * p->step_scale;
-    st = ensure_steps(ctx, l1);
+    st = mk_ensure_steps(ctx, l1);
     if (st) return st;
 
     RenderArgs A;
@@ -712,15 +693,10 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
         if (lds > kMaxLdsBytes) return mk_fail(ctx, MK_ERR_ARG, "too many labels for the LDS budget");
     }
     const uint32_t tiles = (uint32_t)(((p->width + tw - 1) / tw) * ((p->height + tw - 1) / tw));
-    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    if (s != ctx->stream) {
-        // the skip tables were (re)built on the context stream
-        hipEvent_t ev;
-        MK_HIP(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        MK_HIP(ctx, hipEventRecord(ev, ctx->stream));
-        MK_HIP(ctx, hipStreamWaitEvent(s, ev, 0));
-        MK_HIP(ctx, hipEventDestroy(ev));
-    }
+    hipStream_t s = (hipStream_t)stream;
+    // the skip tables were (re)built on the context stream
+    st = mk_order_after_context(ctx, s);
+    if (st) return st;
     dim3 grid(tiles), block(tw * tw);
 #define MK_LAUNCH(K) hipLaunchKernelGGL(K, grid, block, lds, s, A, T)
 #define MK_VR(TOUCH)                                                                              \
@@ -747,6 +723,25 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
 
 }  // namespace
 
+mk_status mk_ensure_steps(mk_context* ctx, float l1) {
+    if (ctx->steps_l1 == l1 && ctx->d_steps) return MK_OK;
+    std::vector<float> a;
+    a.reserve((size_t)(1.75 / l1) + 8);
+    float acc = 0.0f;
+    a.push_back(acc);
+    while ((double)acc < 1.74) {
+        acc += l1;
+        a.push_back(acc);
+    }
+    mk_status st = mk_ensure_buffer(ctx, (void**)&ctx->d_steps, &ctx->steps_cap, a.size() * sizeof(float));
+    if (st) return st;
+    MK_HIP(ctx, hipMemcpyAsync(ctx->d_steps, a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->steps_l1 = l1;
+    ctx->steps_n = (int)a.size();
+    return MK_OK;
+}
+
 extern "C" mk_status mk_render_async(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb, void* stream) {
     if (!ctx) return MK_ERR_ARG;
     return launch_render(ctx, p, dev_rgb, stream, nullptr, nullptr);
@@ -768,7 +763,7 @@ extern "C" mk_status mk_render(mk_context* ctx, const mk_render_params* p, uint8
     size_t bytes = (size_t)p->width * p->height * 3;
     st = mk_ensure_buffer(ctx, &ctx->d_stage, &ctx->stage_cap, bytes);
     if (st) return st;
-    st = mk_render_async(ctx, p, (uint8_t*)ctx->d_stage, nullptr);
+    st = mk_render_async(ctx, p, (uint8_t*)ctx->d_stage, ctx->stream);
     if (st) return st;
     MK_HIP(ctx, hipMemcpyAsync(host_rgb, ctx->d_stage, bytes, hipMemcpyDeviceToHost, ctx->stream));
     MK_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -134,6 +134,17 @@ extern "C" const char* mk_last_error(const mk_context* ctx) {
 
 extern "C" void* mk_context_stream(mk_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+mk_status mk_order_after_context(mk_context* ctx, hipStream_t s) {
+    if (s == ctx->stream) return MK_OK;
+    hipEvent_t ev;
+    MK_HIP(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e = hipEventRecord(ev, ctx->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, ev, 0);
+    (void)hipEventDestroy(ev);
+    if (e != hipSuccess) return mk_hip_fail(ctx, e, "ordering after the context stream");
+    return MK_OK;
+}
+
 extern "C" mk_status mk_synchronize(mk_context* ctx) {
     if (!ctx) return MK_ERR_ARG;
     MK_HIP(ctx, hipSetDevice(ctx->device));

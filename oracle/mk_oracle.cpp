@@ -421,6 +421,49 @@ static void cpr_all(int w, int h, const double* pts, const double* dirs, int inv
         for (int x = 0; x < w; ++x) out[(size_t)y * w + x] = cpr_pixel<T, Q8>(pts, dirs, invz, sampling, vol, x, y);
 }
 
+// Fitting rule of mk_tf_forward in double (kernel.cu compositing on the
+// float lattice at a fixed step, every sample with alpha > 0 composited, no
+// early exit, alpha = min(1, s * LUT_alpha) with LUT entries in double, label
+// 0 window, float RGB sum without saturation).
+static void fit_pixel(const mk_render_params& P, const Volume& vol, const double* lut, int n, double s, int x, int y,
+                      double out[3]) {
+    Ray<double> ray = make_ray<double>(P, x, y);
+    double dirL[3];
+    light_dir<double>(P, dirL);
+    double sv[3] = {P.spacing_voxel[0], P.spacing_voxel[1], P.spacing_voxel[2]};
+    const float L1 = 1.0f / (float)vol.nz * P.step_scale;
+    const double wl = P.alpha_wwwl[0].wl, ww = P.alpha_wwwl[0].ww;
+    double C[3] = {0, 0, 0}, T = 1.0;
+    float acc = 0.0f;
+    while ((double)acc < 1.732) {
+        double p[3];
+        ray.pos(acc, (double)P.scale, p);
+        acc += L1;
+        int ix, iy, iz;
+        if (!in_voi<double>(P.voi, vol, p, ix, iy, iz)) continue;
+        double t = 32768.0 * tex3_linear<double, false>(vol, p[0], p[1], p[2]);
+        t = (t - wl) / ww + 0.5;
+        if (t > 1.0) t = 1.0;
+        if (!(t >= -1.0)) t = -1.0;
+        int i0, i1;
+        double f;
+        split<double, false>(t, n, i0, i1, f);
+        double c[4];
+        for (int k = 0; k < 4; ++k) c[k] = (1.0 - f) * lut[4 * i0 + k] + f * lut[4 * i1 + k];
+        double a = s * c[3];
+        a = a < 0.0 ? 0.0 : (a > 1.0 ? 1.0 : a);
+        if (!(a > 0.0)) continue;
+        double N[3];
+        gradient<double, false>(vol, p, sv, P.invert_z, N);
+        double d = N[0] * dirL[0] + N[1] * dirL[1] + N[2] * dirL[2];
+        double k = 0.35;
+        if (d > 0.0) k += d * 0.8 + 0.16 * std::pow(d, 8.0);
+        for (int ch = 0; ch < 3; ++ch) C[ch] += T * a * k * c[ch];
+        T *= 1.0 - a;
+    }
+    out[0] = C[0]; out[1] = C[1]; out[2] = C[2];
+}
+
 // Marching cubes with the triangle-list contract of mk_marching_cubes: cubes
 // x fastest, table order, vertices interpolated from the lower corner of
 // each edge in float. Returns the total; writes min(total, capacity).
@@ -466,6 +509,17 @@ uint64_t mko_marching_cubes(const int16_t* v, int nx, int ny, int nz, const int*
                             uint64_t capacity) {
     Volume vol{v, nullptr, nx, ny, nz, nullptr};
     return marching_cubes(vol, box, iso, out, capacity);
+}
+
+// Fitting-rule image in double: out is height * width * 3.
+int mko_fit_render(const mk_render_params* P, const int16_t* v, int nx, int ny, int nz, const double* lut, int n,
+                   double s, double* out) {
+    if (!P || !v || !lut || !out || n < 2) return 1;
+    Volume vol{v, nullptr, nx, ny, nz, nullptr};
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int y = 0; y < P->height; ++y)
+        for (int x = 0; x < P->width; ++x) fit_pixel(*P, vol, lut, n, s, x, y, out + 3 * ((size_t)y * P->width + x));
+    return 0;
 }
 
 // The expanded triangle table, for pinning against the reference's table.

@@ -35,6 +35,8 @@ def lib() -> C.CDLL:
         L.mko_render_cpr.argtypes = [i, i, vp, vp, i, i, vp, i, i, i, i, i, vp, vp]
         L.mko_marching_cubes.restype = C.c_uint64
         L.mko_marching_cubes.argtypes = [vp, i, i, i, vp, C.c_float, vp, C.c_uint64]
+        L.mko_fit_render.restype = i
+        L.mko_fit_render.argtypes = [C.POINTER(RenderParams), vp, i, i, i, vp, i, C.c_double, vp]
         L.mko_mc_table.restype = None
         L.mko_mc_table.argtypes = [vp, vp]
         L.mko_tex3_linear.restype = C.c_double
@@ -146,3 +148,15 @@ def mc_table():
     cnt = np.zeros(256, np.uint8)
     lib().mko_mc_table(tri.ctypes.data, cnt.ctypes.data)
     return tri, cnt
+
+
+def fit_render(p: RenderParams, vol_zyx, lut, scale: float) -> np.ndarray:
+    """Fitting-rule image (height, width, 3) in float64 (FP64 restatement of mk_tf_forward)."""
+    v = np.ascontiguousarray(vol_zyx, dtype=np.int16)
+    nz, ny, nx = v.shape
+    l = np.ascontiguousarray(lut, dtype=np.float64).reshape(-1, 4)
+    out = np.zeros((p.height, p.width, 3), np.float64)
+    rc = lib().mko_fit_render(C.byref(p), v.ctypes.data, nx, ny, nz, l.ctypes.data, l.shape[0], float(scale),
+                              out.ctypes.data)
+    assert rc == 0
+    return out
