@@ -234,6 +234,54 @@ mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, const float
                          float opacity_scale, const float* dev_grad_rgb, double* dev_grad_lut,
                          double* dev_grad_scale, void* stream);
 
+/* ---- batched serving launches (no reference counterpart) ----------------- */
+/* The render server batches requests over several resident volumes into one
+ * launch. Outputs go to a caller-owned device arena at each item's offset:
+ * RGB24 for RENDER items, int16 for PLANE and CPR items (even offsets). CPR
+ * items read their height*3 points and height*3 directions (doubles, 8-byte
+ * aligned offsets) from the same arena. */
+typedef enum { MK_ITEM_RENDER = 0, MK_ITEM_PLANE = 1, MK_ITEM_CPR = 2 } mk_item_kind;
+
+typedef struct {
+    int kind;                    /* mk_item_kind */
+    int context;                 /* index into the batcher's contexts */
+    uint64_t out_offset;         /* byte offset of the output in the arena */
+    mk_render_params render;     /* RENDER: as mk_render_async */
+    mk_plane_params plane;       /* PLANE: as mk_render_plane_async */
+    int cpr_width, cpr_height, cpr_invert_z, cpr_sampling;  /* CPR: as mk_render_cpr_async */
+    uint64_t cpr_points_offset;
+    uint64_t cpr_dirs_offset;
+} mk_batch_item;
+
+typedef struct mk_batcher mk_batcher;
+
+/* A batcher over nctx contexts of one device with nslots launch slots. Each
+ * slot holds a hipGraph built once: descriptor upload, then one kernel per
+ * kernel variant (VR with/without mask and LDS LUT, MIP with/without mask,
+ * Surface, MPR/CPR), each over its variant's 16x16 tiles. Batches may hold up
+ * to max_items items and max_tiles tiles in all. */
+mk_status mk_batcher_create(mk_context* const* ctxs, int nctx, int max_items, int max_tiles, int nslots,
+                            mk_batcher** out);
+void mk_batcher_destroy(mk_batcher* b);
+/* Last error of b, or of the last failing mk_batcher_create when b is NULL. */
+const char* mk_batcher_last_error(const mk_batcher* b);
+
+/* Launches a batch in a free slot on the given stream (NULL: the HIP default
+ * stream) and returns. A render item whose LUTs, skip tables or step lattice
+ * must be rebuilt first waits for every batch in flight. Within one batch,
+ * the render items of one context share one step_scale, and those of one
+ * context and render type one alpha_wwwl table (MK_ERR_ARG otherwise). A
+ * context used by a batcher must not be rendered through the one-request
+ * entry points while its batches run. */
+mk_status mk_batch_submit(mk_batcher* b, int slot, const mk_batch_item* items, int n, uint8_t* dev_arena,
+                          uint64_t arena_bytes, void* stream);
+
+/* 1 when the slot is free (its last batch finished), 0 while it runs, -1 on error. */
+int mk_batch_query(mk_batcher* b, int slot);
+
+/* Blocks until the slot's last batch has finished. */
+mk_status mk_batch_wait(mk_batcher* b, int slot);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,7 @@ MK_OK, MK_ERR_ARG, MK_ERR_STATE, MK_ERR_HIP, MK_ERR_NOMEM, MK_ERR_RANGE = range(
 MK_RENDER_VR, MK_RENDER_MIP, MK_RENDER_SURFACE = range(3)
 MK_MPR_AVERAGE, MK_MPR_MIP, MK_MPR_MINIP = range(3)
 MK_SAMPLE_LINEAR, MK_SAMPLE_NEAREST = range(2)
+MK_ITEM_RENDER, MK_ITEM_PLANE, MK_ITEM_CPR = range(3)
 
 # exported symbols, in header order
 SYMBOLS = [
@@ -27,7 +28,8 @@ SYMBOLS = [
     "mk_copy_mask_data", "mk_copy_mask_data_device", "mk_clear_mask", "mk_set_transfer_func",
     "mk_render", "mk_render_async", "mk_render_footprint", "mk_render_plane", "mk_render_plane_async",
     "mk_render_cpr", "mk_render_cpr_async", "mk_marching_cubes", "mk_marching_cubes_async",
-    "mk_tf_forward", "mk_tf_backward",
+    "mk_tf_forward", "mk_tf_backward", "mk_batcher_create", "mk_batcher_destroy", "mk_batcher_last_error",
+    "mk_batch_submit", "mk_batch_query", "mk_batch_wait",
 ]
 
 
@@ -55,6 +57,14 @@ class PlaneParams(C.Structure):
         ("invert_z", C.c_int), ("spacing", C.c_float * 3), ("dir_h", C.c_float * 3), ("dir_v", C.c_float * 3),
         ("dir_n", C.c_float * 3), ("pt_left_top", C.c_float * 3), ("pixel_spacing", C.c_float),
         ("half_num", C.c_float),
+    ]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("context", C.c_int), ("out_offset", C.c_uint64), ("render", RenderParams),
+        ("plane", PlaneParams), ("cpr_width", C.c_int), ("cpr_height", C.c_int), ("cpr_invert_z", C.c_int),
+        ("cpr_sampling", C.c_int), ("cpr_points_offset", C.c_uint64), ("cpr_dirs_offset", C.c_uint64),
     ]
 
 
@@ -109,6 +119,12 @@ def load() -> C.CDLL:
         "mk_marching_cubes_async": (st, [vp, vp, C.c_float, vp, C.c_uint64, vp, vp]),
         "mk_tf_forward": (st, [vp, C.POINTER(RenderParams), vp, i, C.c_float, vp, vp]),
         "mk_tf_backward": (st, [vp, C.POINTER(RenderParams), vp, i, C.c_float, vp, vp, vp, vp]),
+        "mk_batcher_create": (st, [C.POINTER(vp), i, i, i, i, C.POINTER(vp)]),
+        "mk_batcher_destroy": (None, [vp]),
+        "mk_batcher_last_error": (C.c_char_p, [vp]),
+        "mk_batch_submit": (st, [vp, i, C.POINTER(BatchItem), i, vp, C.c_uint64, vp]),
+        "mk_batch_query": (i, [vp, i]),
+        "mk_batch_wait": (st, [vp, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -243,3 +259,48 @@ class Context:
                     stream: int = 0) -> None:
         self._check(self.lib.mk_tf_backward(self.h, C.byref(p), _ptr(dev_lut), n, scale, _ptr(dev_grad_rgb),
                                             _ptr(dev_grad_lut), _ptr(dev_grad_scale), stream))
+
+
+class Batcher:
+    """Batched launches over several contexts of one device (``mk_batcher``)."""
+
+    def __init__(self, contexts: list, max_items: int, max_tiles: int, nslots: int = 2):
+        self.lib = load()
+        self.contexts = list(contexts)   # keeps the contexts alive
+        arr = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = C.c_void_p()
+        st = self.lib.mk_batcher_create(arr, len(self.contexts), max_items, max_tiles, nslots, C.byref(h))
+        if st != MK_OK:
+            raise MkError(st, self.lib.mk_batcher_last_error(None).decode())
+        self.h = h
+        self.nslots = nslots
+        self.max_items = max_items
+        self.max_tiles = max_tiles
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.mk_batcher_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int) -> None:
+        if st != MK_OK:
+            raise MkError(st, self.lib.mk_batcher_last_error(self.h).decode())
+
+    def submit(self, slot: int, items, dev_arena, arena_bytes: int, stream: int = 0) -> None:
+        arr = (BatchItem * len(items))(*items)
+        self._check(self.lib.mk_batch_submit(self.h, slot, arr, len(items), _ptr(dev_arena), arena_bytes, stream))
+
+    def query(self, slot: int) -> bool:
+        r = self.lib.mk_batch_query(self.h, slot)
+        if r < 0:
+            raise MkError(MK_ERR_HIP, self.lib.mk_batcher_last_error(self.h).decode())
+        return r == 1
+
+    def wait(self, slot: int) -> None:
+        self._check(self.lib.mk_batch_wait(self.h, slot))

@@ -36,7 +36,7 @@ HIP_FLAGS = [
 # files whose arithmetic must match the host restatement bit for bit
 NO_CONTRACT = {"mk_plane.hip", "mk_mc.hip"}
 
-HIP_SOURCES = ["mk_volume.hip", "mk_render.hip", "mk_plane.hip", "mk_mc.hip", "mk_fit.hip"]
+HIP_SOURCES = ["mk_volume.hip", "mk_render.hip", "mk_plane.hip", "mk_mc.hip", "mk_fit.hip", "mk_batch.hip"]
 
 
 def _run(cmd: list[str]) -> None:

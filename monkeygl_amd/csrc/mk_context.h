@@ -35,6 +35,21 @@ struct MkLutTable {
     int len[MK_NUM_LABELS];
 };
 
+// Skip tables of one render type. occ/occ2: brick / macro cell (4x4x4 bricks)
+// may contribute (VR, Surface); mipb/mipb2: upper bound of the windowed value
+// (MIP). Valid for the (window/alpha palette, LUT version, volume version)
+// they were built from.
+struct MkSkipTables {
+    uint8_t* occ = nullptr;
+    float* mipb = nullptr;
+    uint8_t* occ2 = nullptr;
+    float* mipb2 = nullptr;
+    bool valid = false;
+    uint32_t lut_version = 0;
+    uint32_t vol_version = 0;
+    mk_alpha_wwwl aw[MK_NUM_LABELS];
+};
+
 struct mk_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -58,16 +73,8 @@ struct mk_context {
     MkLutTable lut_tab{};
     uint32_t lut_version = 1;
 
-    // per-frame skip tables (cached on their inputs)
-    uint8_t* d_occ = nullptr;     // VR / Surface: brick may contribute
-    float* d_mipb = nullptr;      // MIP: upper bound of the windowed value
-    uint8_t* d_occ2 = nullptr;    // the same per macro cell of 4x4x4 bricks
-    float* d_mipb2 = nullptr;
-    bool occ_valid = false;
-    int occ_type = -1;
-    uint32_t occ_lut_version = 0;
-    uint32_t occ_vol_version = 0;
-    mk_alpha_wwwl occ_aw[MK_NUM_LABELS];
+    // per-frame skip tables, one set per render type, cached on their inputs
+    MkSkipTables skip[3];
     uint32_t vol_version = 0;
 
     // exact float step lattice a_{k+1} = a_k + L1 from a_0 = 0
@@ -107,6 +114,9 @@ mk_status mk_upload_luts(mk_context* ctx);
 // Makes stream s wait for the work queued so far on the context's own stream
 // (skip tables, lattice and table uploads); no-op when s is that stream.
 mk_status mk_order_after_context(mk_context* ctx, hipStream_t s);
+// builds (on the context stream) the skip tables of render_type for the given
+// palette unless the cached ones match; the LUTs must be uploaded
+mk_status mk_ensure_skip_tables(mk_context* ctx, int render_type, const mk_alpha_wwwl* aw);
 // uploads the float lattice a_{k+1} = a_k + l1 (a_0 = 0) up to past 1.732
 mk_status mk_ensure_steps(mk_context* ctx, float l1);
 

@@ -12,20 +12,11 @@
 #include <cstring>
 
 #include "mk_context.h"
+#include "mk_plane_dev.h"
 
 namespace {
 
-struct PlaneArgs {
-    MkVolView v;
-    int16_t* out;
-    int width, height;
-    int invert_z;
-    float sp[3];
-    float dh[3], dv[3], dn[3];
-    float lt[3];
-    float ps;
-    float half;
-};
+using namespace mkp;
 
 template <int MODE, bool NEAREST>
 __global__ void __launch_bounds__(256) k_plane(PlaneArgs A) {
@@ -35,53 +26,18 @@ __global__ void __launch_bounds__(256) k_plane(PlaneArgs A) {
     const int x = (int)(t % tiles_x) * 16 + (threadIdx.x & 15);
     const int y = (int)(t / tiles_x) * 16 + (threadIdx.x >> 4);
     if (x >= A.width || y >= A.height) return;
-    const float W = (float)A.v.nx, H = (float)A.v.ny, D = (float)A.v.nz;
-    short best = MODE == MK_MPR_MINIP ? (short)32767 : (short)-32768;
-    double fsum = 0.0;
-    for (float tt = -A.half; tt <= A.half; tt += 1.0f) {
-        float fl = tt * A.ps;
-        float px = A.lt[0] + fl * A.dn[0];
-        float py = A.lt[1] + fl * A.dn[1];
-        float pz = A.lt[2] + fl * A.dn[2];
-        float fx = (px + (float)x * A.ps * A.dh[0] + (float)y * A.ps * A.dv[0]) / (A.sp[0] * W);
-        float fy = (py + (float)x * A.ps * A.dh[1] + (float)y * A.ps * A.dv[1]) / (A.sp[1] * H);
-        float fz = (pz + (float)x * A.ps * A.dh[2] + (float)y * A.ps * A.dv[2]) / (A.sp[2] * D);
-        if (!A.invert_z) fz = (float)(1.0 - (double)fz);
-        const bool in = fx >= 0.f && fx <= 1.f && fy >= 0.f && fy <= 1.f && fz >= 0.f && fz <= 1.f;
-        float s = 0.f;
-        if (in) s = NEAREST ? mk_sample_nearest_hu(A.v, fx, fy, fz) : mk_sample_hu(A.v, fx, fy, fz);
-        if (MODE == MK_MPR_AVERAGE) {
-            fsum += in ? (double)s : -32768.0;
-        } else {
-            short val = in ? (short)s : (short)-32768;
-            if (MODE == MK_MPR_MIP) best = best > val ? best : val;
-            else best = best < val ? best : val;
-        }
-    }
-    short r = best;
-    if (MODE == MK_MPR_AVERAGE) r = (short)(fsum / (double)(2.0f * A.half + 1.0f));
-    A.out[(size_t)y * A.width + x] = r;
+    plane_pixel<MODE, NEAREST>(A, x, y);
 }
 
 template <bool NEAREST>
-__global__ void __launch_bounds__(256) k_cpr(MkVolView v, const double* __restrict__ pts, const double* __restrict__ dirs,
-                                             int width, int height, int invert_z, int16_t* __restrict__ out) {
-    const uint32_t tiles_x = (width + 15) / 16;
-    const uint32_t tiles = tiles_x * ((height + 15) / 16);
+__global__ void __launch_bounds__(256) k_cpr(CprArgs A) {
+    const uint32_t tiles_x = (A.width + 15) / 16;
+    const uint32_t tiles = tiles_x * ((A.height + 15) / 16);
     const uint32_t t = mk_xcd_tile(blockIdx.x, tiles);
     const int x = (int)(t % tiles_x) * 16 + (threadIdx.x & 15);
     const int y = (int)(t / tiles_x) * 16 + (threadIdx.x >> 4);
-    if (x >= width || y >= height) return;
-    const double* pt = pts + 3 * (size_t)y;
-    const double* dr = dirs + 3 * (size_t)y;
-    float fx = (float)((pt[0] + x * dr[0]) / (double)v.nx);
-    float fy = (float)((pt[1] + x * dr[1]) / (double)v.ny);
-    float fz = (float)((pt[2] + x * dr[2]) / (double)v.nz);
-    if (!invert_z) fz = (float)(1.0 - (double)fz);
-    short r;
-    if (fx < 0.f || fx > 1.f || fy < 0.f || fy > 1.f || fz < 0.f || fz > 1.f) r = -2048;
-    else r = (short)(NEAREST ? mk_sample_nearest_hu(v, fx, fy, fz) : mk_sample_hu(v, fx, fy, fz));
-    out[(size_t)y * width + x] = r;
+    if (x >= A.width || y >= A.height) return;
+    cpr_pixel<NEAREST>(A, x, y);
 }
 
 mk_status validate_plane(mk_context* ctx, const mk_plane_params* p) {
@@ -177,12 +133,18 @@ extern "C" mk_status mk_render_cpr_async(mk_context* ctx, int width, int height,
     mk_status st = mk_order_after_context(ctx, s);
     if (st) return st;
     dim3 grid((uint32_t)(((width + 15) / 16) * ((height + 15) / 16))), block(256);
+    CprArgs C;
+    C.v = ctx->view();
+    C.pts = dev_points;
+    C.dirs = dev_dirs;
+    C.out = dev_out;
+    C.width = width;
+    C.height = height;
+    C.invert_z = invert_z != 0;
     if (sampling == MK_SAMPLE_NEAREST)
-        hipLaunchKernelGGL(k_cpr<true>, grid, block, 0, s, ctx->view(), dev_points, dev_dirs, width, height,
-                           invert_z != 0, dev_out);
+        hipLaunchKernelGGL(k_cpr<true>, grid, block, 0, s, C);
     else
-        hipLaunchKernelGGL(k_cpr<false>, grid, block, 0, s, ctx->view(), dev_points, dev_dirs, width, height,
-                           invert_z != 0, dev_out);
+        hipLaunchKernelGGL(k_cpr<false>, grid, block, 0, s, C);
     MK_HIP(ctx, hipGetLastError());
     return MK_OK;
 }

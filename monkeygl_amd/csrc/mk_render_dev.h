@@ -1,0 +1,471 @@
+// mk_render_dev.h -- device code of VR / MIP / Surface ray casting, shared
+// by the per-frame kernels (mk_render.hip) and the batched serving kernel
+// (mk_batch.hip). See mk_render.hip for the design notes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mk_context.h"
+
+namespace mkr {
+
+struct RenderArgs {
+    MkVolView v;
+    const float4* lut;
+    MkLutTable tab;
+    const uint8_t* occ;    // per 8^3 brick (VR, Surface)
+    const float* mipb;     // per 8^3 brick (MIP)
+    const uint8_t* occ2;   // per 32^3 macro cell (4x4x4 bricks)
+    const uint8_t* blabel; // per brick: its only label, or 0xFF when mixed
+    const float* mipb2;
+    int nmx, nmy;          // macro cell dims (x, y)
+    const float* steps;
+    int nsteps;
+    int nlut_lds;      // entries staged in LDS (0 = read LUTs from global)
+    int nslot;         // per-label accumulator slots (masked VR)
+    uint8_t slot_of[MK_NUM_LABELS + 3];
+    uint8_t* out;
+    uint8_t* touch_v;  // footprint maps (measurement variant only)
+    uint8_t* touch_m;
+    int width, height;
+    int invert_z;
+    float mls[3];
+    float sv[3];
+    float xT, yT;
+    double scale;
+    float M[9];
+    mk_voi voi;
+    float bg[4];
+    float l1;
+    float exit_alpha;
+    int max_iter;      // hard bound on loop trips per ray
+};
+
+struct AwTable {
+    float alpha[MK_NUM_LABELS];
+    float ww[MK_NUM_LABELS];
+    float wl[MK_NUM_LABELS];
+};
+
+__device__ inline void ray_pos(const RenderArgs& A, float u, float v, float acc, float p[3]) {
+    float fy = (float)(((double)acc - 0.866) * A.scale);
+    float x = u * A.M[0] + fy * A.M[1] + v * A.M[2];
+    float y = u * A.M[3] + fy * A.M[4] + v * A.M[5];
+    float z = u * A.M[6] + fy * A.M[7] + v * A.M[8];
+    p[0] = x * A.mls[0] + 0.5f;
+    p[1] = y * A.mls[1] + 0.5f;
+    p[2] = z * A.mls[2] + 0.5f;
+    if (A.invert_z) p[2] = 1.0f - p[2];
+}
+
+// Linear model of a ray, pos(a) ~ p0 + a * d, in normalized volume units.
+struct RayGeom {
+    float p0[3];
+    float d[3];
+};
+
+// Conservative [a_in, a_out] of accumulated lengths whose samples can pass
+// the VOI test; returns false when none can.
+__device__ inline bool ray_interval(const RenderArgs& A, float u, float v, RayGeom& g, float& a_in, float& a_out) {
+    ray_pos(A, u, v, 0.0f, g.p0);
+    // pos(a) at a = 0 is taken at fy = -0.866*scale; shift to the a origin
+    g.d[0] = A.M[1] * (float)A.scale * A.mls[0];
+    g.d[1] = A.M[4] * (float)A.scale * A.mls[1];
+    g.d[2] = A.M[7] * (float)A.scale * A.mls[2];
+    if (A.invert_z) g.d[2] = -g.d[2];
+    const int lo_i[3] = {A.voi.left, A.voi.posterior, A.voi.head};
+    const int hi_i[3] = {A.voi.right, A.voi.anterior, A.voi.foot};
+    const int n[3] = {A.v.nx, A.v.ny, A.v.nz};
+    float t0 = 0.0f, t1 = 1.7321f;
+    const float eps = 1e-3f;
+    for (int i = 0; i < 3; ++i) {
+        // trunc(p*N) in [lo, hi]  =>  p in (lo-1, hi+1)/N
+        float lo = ((float)lo_i[i] - 1.0f) / (float)n[i] - eps;
+        float hi = ((float)hi_i[i] + 1.0f) / (float)n[i] + eps;
+        if (fabsf(g.d[i]) < 1e-12f) {
+            if (!(g.p0[i] >= lo && g.p0[i] <= hi)) return false;
+        } else {
+            float ta = (lo - g.p0[i]) / g.d[i], tb = (hi - g.p0[i]) / g.d[i];
+            t0 = fmaxf(t0, fminf(ta, tb));
+            t1 = fminf(t1, fmaxf(ta, tb));
+        }
+    }
+    const float m = 4.0f * A.l1;
+    a_in = fmaxf(t0 - m, 0.0f);
+    a_out = t1 + m;
+    return a_in <= a_out;
+}
+
+// Accumulated length below which the ray is certain to stay in the cell of
+// 2^shift voxels per axis that holds voxel (cx, cy, cz): the earliest exit
+// over the three axes, with the cell shrunk by a margin that covers the
+// rounding of the sample positions.
+__device__ inline float cell_exit(const RenderArgs& A, const RayGeom& g, int cx, int cy, int cz, int shift) {
+    const int c[3] = {cx >> shift, cy >> shift, cz >> shift};
+    const float n[3] = {(float)A.v.nx, (float)A.v.ny, (float)A.v.nz};
+    const float size = (float)(1 << shift);
+    const float margin = 1e-5f;
+    float t = INFINITY;
+    for (int i = 0; i < 3; ++i) {
+        if (g.d[i] > 1e-12f) {
+            float face = ((float)(c[i] + 1) * size) / n[i] - margin;
+            t = fminf(t, (face - g.p0[i]) / g.d[i]);
+        } else if (g.d[i] < -1e-12f) {
+            float face = ((float)c[i] * size) / n[i] + margin;
+            t = fminf(t, (face - g.p0[i]) / g.d[i]);
+        }
+    }
+    return t;
+}
+
+__device__ inline bool in_voi(const RenderArgs& A, const float p[3], int& ix, int& iy, int& iz) {
+    float fx = p[0] * (float)A.v.nx, fy = p[1] * (float)A.v.ny, fz = p[2] * (float)A.v.nz;
+    fx = fminf(fmaxf(fx, -1e9f), 1e9f);
+    fy = fminf(fmaxf(fy, -1e9f), 1e9f);
+    fz = fminf(fmaxf(fz, -1e9f), 1e9f);
+    ix = (int)fx; iy = (int)fy; iz = (int)fz;
+    return !(ix < A.voi.left || ix > A.voi.right || iy < A.voi.posterior || iy > A.voi.anterior || iz < A.voi.head ||
+             iz > A.voi.foot);
+}
+
+__device__ inline uint32_t macro_index(const RenderArgs& A, int cx, int cy, int cz) {
+    return ((uint32_t)(cz >> 5) * (uint32_t)A.nmy + (uint32_t)(cy >> 5)) * (uint32_t)A.nmx + (uint32_t)(cx >> 5);
+}
+
+// tex1D linear/clamp of label's LUT at normalized s.
+template <bool LDS>
+__device__ inline float4 lut_sample(const RenderArgs& A, const float4* s_lut, int label, float s) {
+    int n = A.tab.len[label];
+    if (n <= 0) return make_float4(0.f, 0.f, 0.f, 0.f);
+    int off = A.tab.off[label];
+    s = s >= -1.0f ? s : -1.0f;
+    int i0, i1;
+    float a;
+    mk_split(s, n, i0, i1, a);
+    float4 c0 = LDS ? s_lut[off + i0] : A.lut[off + i0];
+    float4 c1 = LDS ? s_lut[off + i1] : A.lut[off + i1];
+    return make_float4(c0.x + a * (c1.x - c0.x), c0.y + a * (c1.y - c0.y), c0.z + a * (c1.z - c0.z),
+                       c0.w + a * (c1.w - c0.w));
+}
+
+__device__ inline float3 light_dir(const RenderArgs& A) {
+    float3 l = make_float3(A.M[1], A.M[4], A.M[7]);
+    float r = rsqrtf(l.x * l.x + l.y * l.y + l.z * l.z);
+    return make_float3(l.x * r, l.y * r, l.z * r);
+}
+
+// Normalized central-difference gradient one voxel apart (kernel.cu:60-67);
+// a zero gradient gives NaN, which fails every "> 0" test downstream. One
+// axis per trip keeps 16 loads in flight instead of 48.
+__device__ inline float3 gradient(const RenderArgs& A, const float p[3]) {
+    float N[3];
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+        float q0[3] = {p[0], p[1], p[2]}, q1[3] = {p[0], p[1], p[2]};
+        q0[i] += A.sv[i];
+        q1[i] -= A.sv[i];
+        N[i] = mk_sample_hu(A.v, q0[0], q0[1], q0[2]) - mk_sample_hu(A.v, q1[0], q1[1], q1[2]);
+    }
+    if (A.invert_z) N[2] = -N[2];
+    float r = rsqrtf(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+    return make_float3(N[0] * r, N[1] * r, N[2] * r);
+}
+
+// Footprint marking for the measurement variant: every voxel the reference
+// rule reads is set to 1 in a dense x-fastest byte map.
+__device__ inline void touch_footprint(const RenderArgs& A, float fx, float fy, float fz) {
+    int x0, x1, y0, y1, z0, z1;
+    float a;
+    mk_split(fx, A.v.nx, x0, x1, a);
+    mk_split(fy, A.v.ny, y0, y1, a);
+    mk_split(fz, A.v.nz, z0, z1, a);
+    const size_t nx = A.v.nx, nxy = (size_t)A.v.nx * A.v.ny;
+    const int xs[2] = {x0, x1}, ys[2] = {y0, y1}, zs[2] = {z0, z1};
+    for (int k = 0; k < 8; ++k) A.touch_v[zs[k >> 2] * nxy + ys[(k >> 1) & 1] * nx + xs[k & 1]] = 1;
+}
+
+__device__ inline void touch_gradient(const RenderArgs& A, const float p[3]) {
+    touch_footprint(A, p[0] + A.sv[0], p[1], p[2]);
+    touch_footprint(A, p[0] - A.sv[0], p[1], p[2]);
+    touch_footprint(A, p[0], p[1] + A.sv[1], p[2]);
+    touch_footprint(A, p[0], p[1] - A.sv[1], p[2]);
+    touch_footprint(A, p[0], p[1], p[2] + A.sv[2]);
+    touch_footprint(A, p[0], p[1], p[2] - A.sv[2]);
+}
+
+__device__ inline void touch_mask(const RenderArgs& A, int cx, int cy, int cz) {
+    if (A.touch_m) A.touch_m[((size_t)cz * A.v.ny + cy) * A.v.nx + cx] = 1;
+}
+
+__device__ inline void store_rgb(const RenderArgs& A, int x, int y, float4 c) {
+    uint8_t* o = A.out + 3 * ((size_t)y * A.width + x);
+    o[0] = (uint8_t)(unsigned)(mk_sat(c.x) * 255.0f);
+    o[1] = (uint8_t)(unsigned)(mk_sat(c.y) * 255.0f);
+    o[2] = (uint8_t)(unsigned)(mk_sat(c.z) * 255.0f);
+}
+
+__device__ inline float4 bg4(const RenderArgs& A) { return make_float4(A.bg[0], A.bg[1], A.bg[2], A.bg[3]); }
+
+template <int TW>
+__device__ inline bool tile_pixel(const RenderArgs& A, int& x, int& y) {
+    const uint32_t tiles_x = (A.width + TW - 1) / TW;
+    const uint32_t tiles = tiles_x * ((A.height + TW - 1) / TW);
+    uint32_t t = mk_xcd_tile(blockIdx.x, tiles);
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int WPR = TW / 8;  // waves per tile row
+    x = (int)(t % tiles_x) * TW + (wave % WPR) * 8 + (lane & 7);
+    y = (int)(t / tiles_x) * TW + (wave / WPR) * 8 + (lane >> 3);
+    return x < A.width && y < A.height;
+}
+
+// Shared-memory carve-up common to the kernels.
+struct Smem {
+    float4* aw;     // MK_NUM_LABELS x (alpha, ww, wl, -)
+    float4* lut;    // nlut_lds entries
+    float* aobj;    // nslot x blockDim
+};
+
+__device__ inline Smem stage_shared(const RenderArgs& A, const AwTable& T, bool stage_lut) {
+    extern __shared__ float4 smem[];
+    Smem s;
+    s.aw = smem;
+    s.lut = smem + 68;
+    s.aobj = (float*)(s.lut + A.nlut_lds);
+    for (int i = threadIdx.x; i < MK_NUM_LABELS; i += blockDim.x) s.aw[i] = make_float4(T.alpha[i], T.ww[i], T.wl[i], 0.f);
+    if (stage_lut)
+        for (int i = threadIdx.x; i < A.nlut_lds; i += blockDim.x) s.lut[i] = A.lut[i];
+    __syncthreads();
+    return s;
+}
+
+__device__ inline float start_acc(const RenderArgs& A, float a_in) {
+    int k0 = (int)(a_in / A.l1) - 2;
+    k0 = k0 < 0 ? 0 : (k0 >= A.nsteps ? A.nsteps - 1 : k0);
+    return A.steps[k0];
+}
+
+// Advances acc over the lattice points a_{k+1} = a_k + L1 that lie below
+// a_skip (all inside one cell that cannot change the result); it counts trips
+// against the same budget as the marching loop.
+__device__ inline void skip_lattice(float& acc, float l1, float a_skip, int& it, int max_iter) {
+    while (acc + l1 < a_skip && it < max_iter) {
+        acc += l1;
+        ++it;
+    }
+}
+
+// d_render (core/kernel.cu:133-264) for pixel (x, y); tid indexes the
+// per-label accumulators of a block of BS threads. TOUCH: measurement variant
+// that skips nothing and marks every voxel the reference rule reads.
+template <int BS, bool MASK, bool LDS_LUT, bool TOUCH>
+__device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y, int tid) {
+
+    const float u = ((float)x - (float)A.width / 2.0f - A.xT) / (float)A.height;
+    const float v = ((float)y - (float)A.height / 2.0f - A.yT) / (float)A.height;
+    float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+    float a_in, a_out;
+    RayGeom g;
+    if (ray_interval(A, u, v, g, a_in, a_out)) {
+        const float L1 = A.l1, L4 = L1 / 4.0f, L8 = L1 / 8.0f;
+        float step = L1, aPre = 0.f, aAcc = 0.f, aobj0 = 0.f;
+        if (MASK)
+            for (int k = 0; k < A.nslot; ++k) s.aobj[k * BS + tid] = 0.f;
+        float acc = start_acc(A, a_in);
+        // skip-table entries of the brick the ray is in, reloaded when it
+        // moves to another brick
+        uint32_t cur_brick = 0xFFFFFFFFu;
+        int cur_macro_empty = 0, cur_occ = 1, cur_label = 0xFF;
+        for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
+            float p[3];
+            ray_pos(A, u, v, acc, p);
+            int ix, iy, iz;
+            if (!in_voi(A, p, ix, iy, iz)) { acc += step; continue; }
+            const int cx = mk_clampi(ix, 0, A.v.nx - 1), cy = mk_clampi(iy, 0, A.v.ny - 1), cz = mk_clampi(iz, 0, A.v.nz - 1);
+            // a sample in an empty cell reads alpha 0 exactly (the skip tables
+            // are conservative), so it needs no fetch
+            int empty_shift = 0;
+            if (!TOUCH) {
+                const uint32_t b = mk_brick_index(A.v, cx, cy, cz);
+                if (b != cur_brick) {
+                    cur_brick = b;
+                    cur_macro_empty = !A.occ2[macro_index(A, cx, cy, cz)];
+                    cur_occ = A.occ[b];
+                    cur_label = MASK ? A.blabel[b] : 0;
+                }
+                empty_shift = cur_macro_empty ? 5 : (cur_occ ? 0 : 3);
+            }
+            float a = 0.f;
+            float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+            int label = 0;
+            if (!empty_shift) {
+                if (MASK) label = (TOUCH || cur_label == 0xFF) ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label;
+                if (TOUCH) {
+                    if (MASK) touch_mask(A, cx, cy, cz);
+                    touch_footprint(A, p[0], p[1], p[2]);
+                }
+                const float4 aw = s.aw[label];
+                float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
+                t = (t - aw.z) / aw.y + 0.5f;
+                t = t > 1.0f ? 1.0f : t;
+                col = lut_sample<LDS_LUT>(A, s.lut, label, t);
+                a = col.w;
+            }
+            // getNextStep (kernel.cu:84-125)
+            if (a > 0.f && step != L1) a = 1.0f - __builtin_amdgcn_exp2f((step == L4 ? 0.25f : 0.125f) * __builtin_amdgcn_logf(1.0f - a));
+            if (acc > 0.f) {
+                if (fmaxf(a, aPre) > 0.001f) {
+                    if (step == L1) { acc -= (L1 - L4); step = L4; continue; }
+                    if (step == L4) { acc -= (L4 - L8); step = L8; continue; }
+                } else {
+                    step = (step == L8) ? L4 : L1;
+                }
+            }
+            aPre = a;
+            acc += step;
+            if (a > 0.0005f) {
+                const float cap = s.aw[label].x;
+                float* slot = MASK ? &s.aobj[A.slot_of[label] * BS + tid] : &aobj0;
+                if (*slot < cap) {
+                    // tracing (kernel.cu:49-81)
+                    if (TOUCH) touch_gradient(A, p);
+                    const float3 N = gradient(A, p);
+                    const float3 L = light_dir(A);
+                    const float d = N.x * L.x + N.y * L.y + N.z * L.z;
+                    float k = 0.35f;
+                    if (d > 0.f) {
+                        float d2 = d * d, d4 = d2 * d2;
+                        k += d * 0.8f + 0.16f * (d4 * d4);
+                    }
+                    const float w = (1.0f - aAcc) * a;
+                    sum.x += w * (col.x * k);
+                    sum.y += w * (col.y * k);
+                    sum.z += w * (col.z * k);
+                    sum.w += w * (a * k);
+                    *slot += w;
+                    aAcc += w;
+                }
+            }
+            if (aAcc > A.exit_alpha) break;
+            // steady state in an empty cell: every further lattice point inside
+            // it is accepted with alpha 0 and changes nothing but acc
+            if (empty_shift && step == L1) skip_lattice(acc, L1, cell_exit(A, g, cx, cy, cz, empty_shift), it, A.max_iter);
+        }
+    }
+    if (sum.x == 0.f && sum.y == 0.f && sum.z == 0.f && sum.w == 0.f) sum = bg4(A);
+    store_rgb(A, x, y, sum);
+}
+
+// d_renderMIP (core/kernel.cu:266-364) for pixel (x, y).
+template <bool MASK, bool TOUCH>
+__device__ inline void mip_pixel(const RenderArgs& A, const Smem& s, int x, int y) {
+    const float u = ((float)x - (float)A.width / 2.0f - A.xT) / (float)A.height;
+    const float v = ((float)y - (float)A.height / 2.0f - A.yT) / (float)A.height;
+    float aAcc = 0.f;
+    float a_in, a_out;
+    RayGeom g;
+    if (ray_interval(A, u, v, g, a_in, a_out)) {
+        const float L1 = A.l1;
+        float acc = start_acc(A, a_in);
+        uint32_t cur_brick = 0xFFFFFFFFu;
+        float cur_bm = INFINITY, cur_bb = INFINITY;
+        int cur_label = 0xFF;
+        for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
+            float p[3];
+            ray_pos(A, u, v, acc, p);
+            int ix, iy, iz;
+            if (in_voi(A, p, ix, iy, iz)) {
+                const int cx = mk_clampi(ix, 0, A.v.nx - 1), cy = mk_clampi(iy, 0, A.v.ny - 1),
+                          cz = mk_clampi(iz, 0, A.v.nz - 1);
+                // a cell whose bound cannot exceed the running maximum leaves it unchanged
+                int skip_shift = 0;
+                if (!TOUCH) {
+                    const uint32_t b = mk_brick_index(A.v, cx, cy, cz);
+                    if (b != cur_brick) {
+                        cur_brick = b;
+                        cur_bm = A.mipb2[macro_index(A, cx, cy, cz)];
+                        cur_bb = A.mipb[b];
+                        cur_label = MASK ? A.blabel[b] : 0;
+                    }
+                    skip_shift = cur_bm <= aAcc ? 5 : (cur_bb <= aAcc ? 3 : 0);
+                }
+                if (!skip_shift) {
+                    int label = MASK ? ((TOUCH || cur_label == 0xFF) ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label) : 0;
+                    if (TOUCH && MASK) touch_mask(A, cx, cy, cz);
+                    const float4 aw = s.aw[label];
+                    if (aw.x > 0.f) {
+                        if (TOUCH) touch_footprint(A, p[0], p[1], p[2]);
+                        float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
+                        t = (t - aw.z) / aw.y + 0.5f;
+                        if (aAcc < t) aAcc = t;
+                    }
+                }
+                acc += L1;
+                if (skip_shift) skip_lattice(acc, L1, cell_exit(A, g, cx, cy, cz, skip_shift), it, A.max_iter);
+            } else {
+                acc += L1;
+            }
+        }
+    }
+    float4 c = aAcc <= 0.f ? bg4(A) : make_float4(aAcc, aAcc, aAcc, aAcc);
+    store_rgb(A, x, y, c);
+}
+
+// d_renderSurface (core/kernel.cu:366-469) for pixel (x, y).
+template <bool TOUCH>
+__device__ inline void surface_pixel(const RenderArgs& A, int x, int y) {
+    const float u = ((float)x - (float)A.width / 2.0f - A.xT) / (float)A.height;
+    const float v = ((float)y - (float)A.height / 2.0f - A.yT) / (float)A.height;
+    float aAcc = 0.f;
+    float a_in, a_out;
+    RayGeom g;
+    if (ray_interval(A, u, v, g, a_in, a_out)) {
+        const float L1 = A.l1;
+        float acc = start_acc(A, a_in);
+        uint32_t cur_brick = 0xFFFFFFFFu;
+        int cur_macro_empty = 0, cur_occ = 1;
+        for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
+            float p[3];
+            ray_pos(A, u, v, acc, p);
+            int ix, iy, iz;
+            if (in_voi(A, p, ix, iy, iz)) {
+                const int cx = mk_clampi(ix, 0, A.v.nx - 1), cy = mk_clampi(iy, 0, A.v.ny - 1),
+                          cz = mk_clampi(iz, 0, A.v.nz - 1);
+                int skip_shift = 0;
+                if (!TOUCH) {
+                    const uint32_t b = mk_brick_index(A.v, cx, cy, cz);
+                    if (b != cur_brick) {
+                        cur_brick = b;
+                        cur_macro_empty = !A.occ2[macro_index(A, cx, cy, cz)];
+                        cur_occ = A.occ[b];
+                    }
+                    skip_shift = cur_macro_empty ? 5 : (cur_occ ? 0 : 3);
+                }
+                if (!skip_shift) {
+                    if (TOUCH) touch_footprint(A, p[0], p[1], p[2]);
+                    float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
+                    if (t > 400.0f) {
+                        if (TOUCH) touch_gradient(A, p);
+                        const float3 N = gradient(A, p);
+                        const float3 L = light_dir(A);
+                        aAcc = N.x * L.x + N.y * L.y + N.z * L.z;
+                        if (aAcc > 0.f) break;
+                    }
+                }
+                acc += L1;
+                if (skip_shift) skip_lattice(acc, L1, cell_exit(A, g, cx, cy, cz, skip_shift), it, A.max_iter);
+            } else {
+                acc += L1;
+            }
+        }
+    }
+    float4 c;
+    if (aAcc <= 0.f) c = bg4(A);
+    else {
+        float g2 = (float)((double)aAcc + 0.05);
+        c = make_float4(g2, g2, g2, 1.0f);
+    }
+    store_rgb(A, x, y, c);
+}
+
+
+}  // namespace mkr

@@ -88,7 +88,6 @@ extern "C" mk_status mk_context_create(int device, mk_context** out) {
         delete ctx;
         return mk_fail(nullptr, MK_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
-    for (int l = 0; l < MK_NUM_LABELS; ++l) ctx->occ_aw[l] = mk_alpha_wwwl{0, 0, 0};
     *out = ctx;
     return MK_OK;
 }
@@ -101,12 +100,14 @@ static void mk_free_volume(mk_context* ctx) {
     mk_free(ctx->stats.lab_lo);
     mk_free(ctx->stats.lab_hi);
     mk_free(ctx->stats.blabel);
-    mk_free(ctx->d_occ);
-    mk_free(ctx->d_mipb);
-    mk_free(ctx->d_occ2);
-    mk_free(ctx->d_mipb2);
+    for (auto& t : ctx->skip) {
+        mk_free(t.occ);
+        mk_free(t.mipb);
+        mk_free(t.occ2);
+        mk_free(t.mipb2);
+        t.valid = false;
+    }
     ctx->nx = ctx->ny = ctx->nz = 0;
-    ctx->occ_valid = false;
 }
 
 extern "C" void mk_context_destroy(mk_context* ctx) {
@@ -262,11 +263,13 @@ static mk_status mk_volume_from_device(mk_context* ctx, const int16_t* d_lin, in
     MK_HIP(ctx, hipMalloc((void**)&ctx->stats.lab_lo, nb * sizeof(uint64_t)));
     MK_HIP(ctx, hipMalloc((void**)&ctx->stats.lab_hi, nb * sizeof(uint8_t)));
     MK_HIP(ctx, hipMalloc((void**)&ctx->stats.blabel, nb * sizeof(uint8_t)));
-    MK_HIP(ctx, hipMalloc((void**)&ctx->d_occ, nb * sizeof(uint8_t)));
-    MK_HIP(ctx, hipMalloc((void**)&ctx->d_mipb, nb * sizeof(float)));
     const size_t nm = (size_t)((ctx->nbx + 3) / 4) * ((ctx->nby + 3) / 4) * ((ctx->nbz + 3) / 4);
-    MK_HIP(ctx, hipMalloc((void**)&ctx->d_occ2, nm * sizeof(uint8_t)));
-    MK_HIP(ctx, hipMalloc((void**)&ctx->d_mipb2, nm * sizeof(float)));
+    for (auto& t : ctx->skip) {
+        MK_HIP(ctx, hipMalloc((void**)&t.occ, nb * sizeof(uint8_t)));
+        MK_HIP(ctx, hipMalloc((void**)&t.mipb, nb * sizeof(float)));
+        MK_HIP(ctx, hipMalloc((void**)&t.occ2, nm * sizeof(uint8_t)));
+        MK_HIP(ctx, hipMalloc((void**)&t.mipb2, nm * sizeof(float)));
+    }
     hipLaunchKernelGGL(k_brickify<int16_t>, dim3((uint32_t)nb), dim3(512), 0, ctx->stream, d_lin, ctx->d_vox, nx, ny, nz,
                        ctx->nbx, ctx->nby);
     MK_HIP(ctx, hipGetLastError());
@@ -285,7 +288,7 @@ static mk_status mk_volume_from_device(mk_context* ctx, const int16_t* d_lin, in
     ctx->used_lo = 1;
     ctx->used_hi = false;
     ctx->vol_version++;
-    ctx->occ_valid = false;
+    for (auto& t : ctx->skip) t.valid = false;
     return MK_OK;
 }
 
@@ -369,7 +372,7 @@ static mk_status mk_mask_from_device(mk_context* ctx, const uint8_t* d_lin, int 
     ctx->used_lo = flags[0];
     ctx->used_hi = flags[2] != 0;
     ctx->vol_version++;
-    ctx->occ_valid = false;
+    for (auto& t : ctx->skip) t.valid = false;
     return MK_OK;
 }
 
@@ -414,7 +417,7 @@ extern "C" mk_status mk_clear_mask(mk_context* ctx) {
     ctx->used_lo = 1;
     ctx->used_hi = false;
     ctx->vol_version++;
-    ctx->occ_valid = false;
+    for (auto& t : ctx->skip) t.valid = false;
     return MK_OK;
 }
 

@@ -156,3 +156,100 @@ def centerline(dims, n: int = 200, seed: int = 2) -> np.ndarray:
     ctrl = np.stack([x, y, z], axis=1)
     pts = _spline(ctrl, n - 1)
     return pts[:n]
+
+
+# ---------------------------------------------------------------------------
+# Render serving: three resident volumes and a Poisson request stream
+
+SERVE_MIX = (("vr", 0.60), ("mip", 0.20), ("mpr", 0.15), ("cpr", 0.05))
+SERVE_SIZES = (256, 512, 768, 1024)
+
+
+@dataclass
+class ServedVolume:
+    spec: VolumeSpec
+    vol: np.ndarray
+    mask: np.ndarray | None
+    luts: dict            # label -> (n, 4) float32
+    labels: dict          # label -> params.LabelState
+    line: np.ndarray      # CPR centreline (voxel units)
+
+
+def serving_scene(specs=None, seed: int = 0) -> list:
+    """The served volumes: the 128^3 blob set, the cardiac-sized set and the
+    body-sized set with a 4-label mask, each with its own TFs and a seeded
+    centreline."""
+    from . import params as P
+
+    specs = specs if specs is not None else [(SMALL, 0), (CARDIAC, 0), (BODY, 4)]
+    out = []
+    for k, (spec, n_labels) in enumerate(specs):
+        vol = make_volume(spec)
+        mask = ellipsoid_mask(spec.dims, n_labels, seed=1 + k) if n_labels else None
+        luts, labels = {}, {}
+        for l in range(n_labels + 1):
+            luts[l] = P.transfer_function(tf_points(100 + 10 * k + l))
+            labels[l] = P.LabelState(alpha=(0.4 if l == 0 and n_labels else 1.0), ww=1200.0 + 200.0 * l,
+                                     wl=-200.0 + 60.0 * l)
+        out.append(ServedVolume(spec, vol, mask, luts, labels, centerline(spec.dims, 200, seed=2 + k)))
+    return out
+
+
+def serving_requests(scene: list, n: int, qps: float, seed: int = 0, sizes=SERVE_SIZES, mix=SERVE_MIX):
+    """n requests with Poisson arrivals at ``qps`` (t_arrival in seconds from
+    0), volumes uniform over the scene, kinds by ``mix``, square viewports
+    uniform over ``sizes``, random camera rotations, MPR planes of random
+    type / slab mode / sampling around a random crosshair, and stretched or
+    straightened CPRs at a random rotation."""
+    from . import cpr as G
+    from . import params as P
+    from ._capi import MK_MPR_AVERAGE, MK_MPR_MINIP, MK_MPR_MIP, MK_RENDER_MIP, MK_RENDER_VR, MK_SAMPLE_LINEAR, \
+        MK_SAMPLE_NEAREST
+    from .serve import KIND_CPR, KIND_PLANE, KIND_RENDER, Request
+
+    rng = np.random.default_rng(seed)
+    kinds = [k for k, _ in mix]
+    probs = np.array([p for _, p in mix], float)
+    probs /= probs.sum()
+    t = 0.0
+    reqs = []
+    cprs = {}
+    for i in range(n):
+        t += rng.exponential(1.0 / qps) if qps > 0 else 0.0
+        vi = int(rng.integers(len(scene)))
+        sv = scene[vi]
+        dims, sp = sv.spec.dims, sv.spec.spacing
+        kind = kinds[int(rng.choice(len(kinds), p=probs))]
+        size = int(sizes[int(rng.integers(len(sizes)))])
+        if kind in ("vr", "mip"):
+            info = P.RenderInfo()
+            info.set_vr_size(size, size)
+            info.rotate(float(rng.uniform(0.0, 360.0)), float(rng.uniform(-60.0, 60.0)))
+            rt = MK_RENDER_VR if kind == "vr" else MK_RENDER_MIP
+            p = P.render_params(dims, sp, info, sv.labels, render_type=rt, width=size, height=size)
+            reqs.append(Request(KIND_RENDER, vi, render=p, t_arrival=t, tag=(i, kind)))
+        elif kind == "mpr":
+            plane = int(rng.integers(3))
+            mode = (MK_MPR_AVERAGE, MK_MPR_MIP, MK_MPR_MINIP)[int(rng.integers(3))]
+            samp = MK_SAMPLE_NEAREST if rng.random() < 0.5 else MK_SAMPLE_LINEAR
+            ps = min(sp)
+            thick = ps if rng.random() < 0.5 else float(rng.uniform(2.0, 10.0))
+            cross = [float(rng.uniform(0.3, 0.7)) * dims[a] * sp[a] for a in range(3)]
+            p = P.plane_params(dims, sp, plane, mpr_type=mode, sampling=samp, thickness=thick, crosshair=cross,
+                               size=(size, size))
+            reqs.append(Request(KIND_PLANE, vi, plane=p, t_arrival=t, tag=(i, kind)))
+        else:
+            which = G.PLANE_STRETCHED_CPR if rng.random() < 0.5 else G.PLANE_STRAIGHTENED_CPR
+            key = (vi, which)
+            if key not in cprs:
+                info = G.CPRInfo()
+                info.set_spacing(sp)
+                info.set_cpr_line_voxel(sv.line)
+                cprs[key] = info
+            info = cprs[key]
+            info.rotate_cpr(float(rng.uniform(0.0, 360.0)), which)
+            pts, dirs, w, h = info.get_cpr_info(which)
+            samp = MK_SAMPLE_NEAREST if rng.random() < 0.5 else MK_SAMPLE_LINEAR
+            reqs.append(Request(KIND_CPR, vi, cpr_width=w, cpr_points=pts, cpr_dirs=dirs, cpr_sampling=samp,
+                                t_arrival=t, tag=(i, "cpr")))
+    return reqs

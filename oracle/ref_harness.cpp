@@ -5,6 +5,10 @@
 //   TransferFunction::GetTransferFunction (core/TransferFunction.cpp:38-140)
 //   RenderInfo + Methods::ComputeTransformMatrix (core/RenderInfo.cpp,
 //   core/Methods.cpp:52-147)
+//   CPR geometry helpers: Methods::RotatePoint3D, Projection_Point2Plane,
+//   Distance_Point2Line (core/Methods.cpp:563-633) over Direction3d /
+//   Point3d (core/Direction.h, core/Point.h). Directions are built the way
+//   CPRInfo builds them, from two points.
 // Used only to write golden vectors (tests/golden/make_golden.py) and to
 // cross-check the restatements where the reference is present.
 #include <cstring>
@@ -70,6 +74,42 @@ void ref_render_info_get(void* h, float* m, float* m_t, float* rot, float* rot_t
     xyz[0] = r->GetTotalXTranslate();
     xyz[1] = r->GetTotalYTranslate();
     xyz[2] = r->GetTotalScale();
+}
+
+static Point3d P3(const double* p) { return Point3d(p[0], p[1], p[2]); }
+static void out3(const Point3d& p, double* o) {
+    Point3d q = p;
+    o[0] = q.x();
+    o[1] = q.y();
+    o[2] = q.z();
+}
+
+// direction = Direction3d(a, b)
+void ref_direction(const double* a, const double* b, double* o) {
+    Direction3d d(P3(a), P3(b));
+    o[0] = d.x();
+    o[1] = d.y();
+    o[2] = d.z();
+}
+// Direction3d(a0, a1).cross(Direction3d(b0, b1))
+void ref_cross(const double* a0, const double* a1, const double* b0, const double* b1, double* o) {
+    Direction3d d = Direction3d(P3(a0), P3(a1)).cross(Direction3d(P3(b0), P3(b1)));
+    o[0] = d.x();
+    o[1] = d.y();
+    o[2] = d.z();
+}
+void ref_rotate_point(const double* pt, const double* ax0, const double* ax1, const double* center, float theta,
+                      double* o) {
+    out3(Methods::RotatePoint3D(P3(pt), Direction3d(P3(ax0), P3(ax1)), P3(center), theta), o);
+}
+void ref_projection_point_to_plane(const double* pt, const double* n0, const double* n1, const double* pp, double* o) {
+    out3(Methods::Projection_Point2Plane(P3(pt), Direction3d(P3(n0), P3(n1)), P3(pp)), o);
+}
+double ref_distance_point_to_line(const double* pt, const double* d0, const double* d1, const double* pl) {
+    return Methods::Distance_Point2Line(P3(pt), Direction3d(P3(d0), P3(d1)), P3(pl));
+}
+double ref_patient_distance(const double* a, const double* b, const double* sp) {
+    return P3(a).PatientDistanceTo(P3(b), P3(sp));
 }
 
 }  // extern "C"

@@ -6,6 +6,9 @@ fixed set of inputs and stores inputs and outputs as JSON next to this script:
 
   tf_luts.json        control points (+ optional alpha points) -> 100-entry LUT
   render_info.json    RenderInfo operation sequences -> matrices after each op
+  cpr_geometry.json   CPR geometry helpers (direction, cross, RotatePoint3D,
+                      Projection_Point2Plane, Distance_Point2Line,
+                      PatientDistanceTo) on seeded inputs
 
 Usage: sh oracle/build_ref.sh && python tests/golden/make_golden.py
 """
@@ -35,7 +38,53 @@ def load():
     L.ref_render_info_preset.argtypes = [vp, i]
     L.ref_render_info_set_vr_size.argtypes = [vp, i, i, vp, vp]
     L.ref_render_info_get.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.ref_direction.argtypes = [vp, vp, vp]
+    L.ref_cross.argtypes = [vp, vp, vp, vp, vp]
+    L.ref_rotate_point.argtypes = [vp, vp, vp, vp, f, vp]
+    L.ref_projection_point_to_plane.argtypes = [vp, vp, vp, vp, vp]
+    L.ref_distance_point_to_line.restype = C.c_double
+    L.ref_distance_point_to_line.argtypes = [vp, vp, vp, vp]
+    L.ref_patient_distance.restype = C.c_double
+    L.ref_patient_distance.argtypes = [vp, vp, vp]
     return L
+
+
+def cpr_geometry_cases(L):
+    """Seeded points in voxel-like ranges; each helper's output (doubles, as
+    hex strings so the JSON round trip is exact)."""
+    rng = np.random.default_rng(77)
+
+    def v():
+        return rng.uniform(-50.0, 550.0, 3)
+
+    def hx(a):
+        return [float(x).hex() for x in np.atleast_1d(a)]
+
+    out = {"direction": [], "cross": [], "rotate": [], "project": [], "dist_line": [], "patient_dist": []}
+    o = np.zeros(3)
+    for _ in range(64):
+        a, b = v(), v()
+        L.ref_direction(a.ctypes.data, b.ctypes.data, o.ctypes.data)
+        out["direction"].append({"a": hx(a), "b": hx(b), "out": hx(o)})
+        a0, a1, b0, b1 = v(), v(), v(), v()
+        L.ref_cross(a0.ctypes.data, a1.ctypes.data, b0.ctypes.data, b1.ctypes.data, o.ctypes.data)
+        out["cross"].append({"a0": hx(a0), "a1": hx(a1), "b0": hx(b0), "b1": hx(b1), "out": hx(o)})
+        pt, x0, x1, c = v(), v(), v(), v()
+        theta = float(rng.uniform(-7.0, 7.0))
+        L.ref_rotate_point(pt.ctypes.data, x0.ctypes.data, x1.ctypes.data, c.ctypes.data, theta, o.ctypes.data)
+        out["rotate"].append({"pt": hx(pt), "ax0": hx(x0), "ax1": hx(x1), "center": hx(c),
+                              "theta": float(np.float32(theta)).hex(), "out": hx(o)})
+        pt, n0, n1, pp = v(), v(), v(), v()
+        L.ref_projection_point_to_plane(pt.ctypes.data, n0.ctypes.data, n1.ctypes.data, pp.ctypes.data,
+                                        o.ctypes.data)
+        out["project"].append({"pt": hx(pt), "n0": hx(n0), "n1": hx(n1), "pp": hx(pp), "out": hx(o)})
+        pt, d0, d1, pl = v(), v(), v(), v()
+        r = L.ref_distance_point_to_line(pt.ctypes.data, d0.ctypes.data, d1.ctypes.data, pl.ctypes.data)
+        out["dist_line"].append({"pt": hx(pt), "d0": hx(d0), "d1": hx(d1), "pl": hx(pl), "out": hx(r)})
+        a, b, sp = v(), v(), rng.uniform(0.3, 1.5, 3)
+        r = L.ref_patient_distance(a.ctypes.data, b.ctypes.data, sp.ctypes.data)
+        out["patient_dist"].append({"a": hx(a), "b": hx(b), "sp": hx(sp), "out": hx(r)})
+    return out
 
 
 def ref_tf(L, pts: dict, alpha: dict | None = None):
@@ -147,7 +196,10 @@ def main():
     sizes = [(w, h, *ref_vr_size(L, w, h)) for (w, h) in [(512, 512), (768, 768), (1024, 1024), (1537, 1024),
                                                             (2048, 2048), (333, 101), (1600, 900), (3000, 700)]]
     (HERE / "render_info.json").write_text(json.dumps({"sequences": ri, "vr_sizes": sizes}))
-    print(f"wrote {len(tf)} TF cases, {len(ri)} RenderInfo sequences, {len(sizes)} VR sizes")
+    geo = cpr_geometry_cases(L)
+    (HERE / "cpr_geometry.json").write_text(json.dumps(geo))
+    print(f"wrote {len(tf)} TF cases, {len(ri)} RenderInfo sequences, {len(sizes)} VR sizes, "
+          f"{sum(len(v) for v in geo.values())} CPR geometry vectors")
 
 
 if __name__ == "__main__":

@@ -46,6 +46,9 @@ def test_struct_layout_matches_c():
 int main(void) {
     printf("%zu %zu %zu %zu %zu\n", sizeof(mk_render_params), offsetof(mk_render_params, alpha_wwwl),
            offsetof(mk_render_params, step_scale), sizeof(mk_plane_params), offsetof(mk_plane_params, half_num));
+    printf("%zu %zu %zu %zu %zu\n", sizeof(mk_batch_item), offsetof(mk_batch_item, render),
+           offsetof(mk_batch_item, plane), offsetof(mk_batch_item, cpr_sampling),
+           offsetof(mk_batch_item, cpr_dirs_offset));
     return 0;
 }
 '''
@@ -55,8 +58,9 @@ int main(void) {
         exe = Path(d) / "layout"
         subprocess.run(["gcc", "-I", str(ROOT / "include"), str(c), "-o", str(exe)], check=True)
         got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    R, Pp = _capi.RenderParams, _capi.PlaneParams
-    assert got == [C.sizeof(R), R.alpha_wwwl.offset, R.step_scale.offset, C.sizeof(Pp), Pp.half_num.offset]
+    R, Pp, B = _capi.RenderParams, _capi.PlaneParams, _capi.BatchItem
+    assert got == [C.sizeof(R), R.alpha_wwwl.offset, R.step_scale.offset, C.sizeof(Pp), Pp.half_num.offset,
+                   C.sizeof(B), B.render.offset, B.plane.offset, B.cpr_sampling.offset, B.cpr_dirs_offset.offset]
 
 
 def test_no_gpu_calls_fail_cleanly_without_device():

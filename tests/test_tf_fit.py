@@ -75,7 +75,7 @@ def test_tf_forward_matches_fp64(gpu_ctx):
 @pytest.mark.gpu
 def test_tf_gradient_matches_fp64_finite_differences(gpu_ctx):
     """dLoss/dLUT and dLoss/dscale from the adjoint vs FP64 central differences
-    (h = 1e-4) on 64 LUT entries: relative error <= 1e-3."""
+    (h = 1e-4) on 64 random live LUT entries: relative error <= 1e-3."""
     import torch
 
     from monkeygl_amd.tf_fit import render_tf
@@ -104,9 +104,11 @@ def test_tf_gradient_matches_fp64_finite_differences(gpu_ctx):
     base = lut0.astype(np.float64)
     h = 1e-4
     rng = np.random.default_rng(7)
-    # 64 random entries among the 128 the loss depends on most (most of the
-    # LUT is read by few samples and carries next to no gradient)
-    live = np.argsort(-np.abs(g_an.reshape(-1)))[:128]
+    # 64 random entries among those the loss depends on (most of the LUT is
+    # read by no sample and has an exactly zero gradient)
+    g_flat = np.abs(g_an.reshape(-1))
+    live = np.flatnonzero(g_flat > 1e-5 * g_flat.max())
+    assert live.size >= 64, live.size
     idx = rng.choice(live, size=64, replace=False)
     g_fd = np.zeros(64)
     for j, flat in enumerate(idx):
@@ -117,12 +119,15 @@ def test_tf_gradient_matches_fp64_finite_differences(gpu_ctx):
     gs_fd = (fp64_loss(base, s0 + h) - fp64_loss(base, s0 - h)) / (2 * h)
     an = g_an.reshape(-1)[idx]
     scale_ref = np.abs(g_fd).max()
-    big = np.abs(g_fd) > 1e-3 * scale_ref
+    # relative error wherever the difference quotient stands clear of its own
+    # rounding noise (loss ~1e-2, h = 1e-4: noise ~1e-13)
+    big = np.abs(g_fd) > 1e-9 * max(scale_ref, 1e-12) + 1e-11
     rel = np.abs(an - g_fd)[big] / np.abs(g_fd)[big]
-    print(f"checked {big.sum()} entries, max rel err {rel.max():.2e}, scale rel err {abs(gs_an - gs_fd) / abs(gs_fd):.2e}")
-    assert big.sum() >= 48
+    print(f"checked {big.sum()} entries (|g| from {np.abs(g_fd)[big].min():.2e} to {scale_ref:.2e}), "
+          f"max rel err {rel.max():.2e}, scale rel err {abs(gs_an - gs_fd) / abs(gs_fd):.2e}")
+    assert big.sum() >= 56
     assert rel.max() <= 1e-3, rel.max()
-    assert np.all(np.abs(an - g_fd)[~big] <= 1e-6 * scale_ref)
+    assert np.all(np.abs(an - g_fd)[~big] <= 1e-9 * scale_ref + 1e-11)
     assert abs(gs_an - gs_fd) <= 1e-3 * abs(gs_fd)
 
 
