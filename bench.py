@@ -1,5 +1,5 @@
 #!/usr/bin/env python
-"""Plain VR throughput of the render path on one MI355X.
+"""Plain VR throughput and served latency of the render path on one MI355X.
 
 Workload: the body-sized volume, 512x512x1559 int16 (spacing 0.7422 x 0.7422
 x 1.0 mm, blob chambers + 40 spline vessels), with a 4-label uint8 ellipsoid
@@ -16,7 +16,15 @@ The JSON line carries:
     on the launch stream, against the 8 TB/s HBM peak;
   * parity: a band of rows of one frame against the FP64 oracle;
   * cpu_baseline: the oracle's float instance (8-bit filter weights) on a
-    band of rows of the same frame, converted to frames per second.
+    band of rows of the same frame, converted to frames per second;
+  * served: the render-serving mix over three resident volumes (the
+    128^3 blob set, the 512x512x361 cardiac-sized set and the body set above
+    with its mask): Poisson arrivals at --serve-qps, 60% VR / 20% MIP /
+    15% MPR / 5% CPR, viewports uniform over 256^2..1024^2, random cameras;
+    p50/p99 latency (arrival to result in host memory) and frames/s of the
+    continuous-batching server next to one-at-a-time plain rendering of the
+    same request stream, the same two at saturation (every request queued at
+    once), and the count of served images bit-identical to the plain ones.
 """
 from __future__ import annotations
 
@@ -67,6 +75,180 @@ def view_params(spec, labels, size: int, n_views: int):
     return ps
 
 
+def fit_main(args) -> int:
+    """Transfer-function fitting: Adam steps per second through the adjoint,
+    loss reduction, and the adjoint against FP64 central differences."""
+    import torch
+
+    from monkeygl_amd import params as P
+    from monkeygl_amd import workloads as W
+    from monkeygl_amd._capi import Context
+    from monkeygl_amd.tf_fit import fit, render_tf
+    from oracle import oracle as O
+
+    spec = W.TF_FIT
+    vol = W.make_volume(spec)
+    ctx = Context(0)
+    ctx.set_volume(vol)
+    ctx.synchronize()
+    n_lut, size = 256, 512
+
+    def views(sz):
+        # generic angles: at multiples of 90 degrees whole pixel columns run
+        # exactly on a volume face, where float positions (the reference's
+        # and the kernel's) and the FP64 check place samples on opposite
+        # sides of the face
+        out = []
+        for k in range(8):
+            info = P.RenderInfo()
+            info.set_vr_size(sz, sz)
+            info.rotate(45.0 * k + 15.0, 20.0 * (k % 3) - 17.0)
+            out.append(P.render_params(spec.dims, spec.spacing, info, {0: P.LabelState(1.0, 2200.0, 0.0)},
+                                       width=sz, height=sz))
+        return out
+
+    def lut(seed):
+        rng = np.random.default_rng(seed)
+        a = np.empty((n_lut, 4), np.float32)
+        a[:, :3] = rng.uniform(0.1, 1.0, (n_lut, 3))
+        a[:, 3] = rng.uniform(0.0, 0.08, n_lut)
+        return a
+
+    lut_gt, lut0, s_gt, s0 = lut(1), lut(2), 1.2, 1.5
+    vs = views(size)
+    with torch.no_grad():
+        target = render_tf(ctx, vs, torch.tensor(lut_gt, device="cuda"), torch.tensor(s_gt))
+    fit(ctx, vs, target, torch.tensor(lut0, device="cuda"), s0, steps=2)   # warm
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    lut_f, s_f, losses = fit(ctx, vs, target, torch.tensor(lut0, device="cuda"), s0, steps=args.fit_steps, lr=0.01)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    log(f"{args.fit_steps} Adam steps in {el:.2f}s; loss {losses[0]:.4g} -> {losses[-1]:.4g}; scale {s_f:.4f}")
+
+    # adjoint vs FP64 central differences (h = 1e-4) on 64 live entries, 64^2 views
+    small = views(64)
+    tgt = np.stack([O.fit_render(p, vol, lut_gt.astype(np.float64), s_gt) for p in small])
+    lt = torch.tensor(lut0, device="cuda", requires_grad=True)
+    sc = torch.tensor(s0, dtype=torch.float32, requires_grad=True)
+    img = render_tf(ctx, small, lt, sc)
+    loss = torch.mean((img - torch.tensor(tgt, dtype=torch.float32, device="cuda")) ** 2)
+    loss.backward()
+    g_an = lt.grad.detach().cpu().numpy().astype(np.float64).reshape(-1)
+    ncores = min(len(os.sched_getaffinity(0)), 16)
+    O.set_threads(ncores)
+
+    def l64(l, s):
+        return np.mean((np.stack([O.fit_render(p, vol, l, s) for p in small]) - tgt) ** 2)
+
+    base, h = lut0.astype(np.float64), 1e-4
+    rng = np.random.default_rng(7)
+    live = np.flatnonzero(np.abs(g_an) > 1e-5 * np.abs(g_an).max())
+    idx = rng.choice(live, size=min(64, live.size), replace=False)
+    rel = []
+    for flat in idx:
+        lp, lm = base.copy(), base.copy()
+        lp.flat[flat] += h
+        lm.flat[flat] -= h
+        fd = (l64(lp, s0) - l64(lm, s0)) / (2 * h)
+        rel.append(abs(g_an[flat] - fd) / max(abs(fd), 1e-30))
+    gs_fd = (l64(base, s0 + h) - l64(base, s0 - h)) / (2 * h)
+    rel_s = abs(float(sc.grad) - gs_fd) / abs(gs_fd)
+    log(f"FD check: {len(idx)} entries, max rel err {max(rel):.3e}; scale rel err {rel_s:.3e}")
+    line = {
+        "metric": "TF fitting Adam steps/s (adjoint), 256^3 i16, 8 views 512^2, 256-entry LUT + opacity scale",
+        "value": round(args.fit_steps / el, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.fit_steps,
+        "warmup": 2, "ms_per_step": round(el * 1e3 / args.fit_steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 render, fp64 gradient accumulation",
+        "data": "synthetic",
+        "config": {"workload": "TF fitting: 256^3 16-blob volume, 8 views at 512^2, MSE to a seeded ground-truth TF",
+                   "volume": list(spec.dims), "views": 8, "image": [size, size], "lut_entries": n_lut},
+        "loss_first": losses[0], "loss_last": losses[-1], "loss_ratio": losses[-1] / losses[0],
+        "fd_check": {"entries": int(len(idx)), "h": h, "image": [64, 64], "max_rel_err": float(max(rel)),
+                     "scale_rel_err": float(rel_s), "bound": 1e-3},
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+def serve_section(args, body_ctx, spec, vol, mask, luts, labels):
+    """Served vs sequential plain rendering of one seeded request stream."""
+    import dataclasses
+
+    from monkeygl_amd import workloads as W
+    from monkeygl_amd._capi import Context
+    from monkeygl_amd.serve import RenderServer, latency_stats, run_sequential
+
+    t = time.time()
+    scene = W.serving_scene([(W.SMALL, 0), (W.CARDIAC, 0)])
+    scene.append(W.ServedVolume(spec, vol, mask, luts, labels, W.centerline(spec.dims, 200, seed=4)))
+    ctxs = []
+    for sv in scene[:2]:
+        c = Context(0)
+        c.set_volume(sv.vol)
+        for l, lut in sv.luts.items():
+            c.set_lut(l, lut)
+        c.synchronize()
+        ctxs.append(c)
+    ctxs.append(body_ctx)
+    log(f"serving scene ready in {time.time() - t:.1f}s")
+
+    def fresh(rs, keep_time=True):
+        return [dataclasses.replace(r, t_arrival=(r.t_arrival if keep_time else 0.0), result=None, error=None,
+                                    t_admit=0.0, t_done=0.0, solo=False) for r in rs]
+
+    reqs = W.serving_requests(scene, args.serve_requests, args.serve_qps, seed=0)
+    srv = RenderServer(ctxs, hbm_budget=512 << 20, nslots=args.serve_slots, max_items=64, max_tiles=65536)
+    try:
+        # warm both paths: tables, lattices, graphs, pinned buffers
+        warm = fresh(reqs[:48], keep_time=False)
+        srv.run(warm)
+        run_sequential(ctxs, fresh(reqs[:48], keep_time=False))
+        b0 = srv.batches
+        served = fresh(reqs)
+        srv.run(served)
+        n_batches = srv.batches - b0
+        sizes = srv.batch_sizes[b0:]
+        seq = fresh(reqs)
+        run_sequential(ctxs, seq)
+        same, mism = 0, []
+        for a, b in zip(served, seq):
+            if a.error is None and np.array_equal(a.result, b.result):
+                same += 1
+            elif len(mism) < 8:
+                d = (np.abs(a.result.astype(np.int32) - b.result.astype(np.int32)) if a.error is None
+                     else None)
+                mism.append({"kind": a.tag[1], "volume": a.volume, "size": list(a.size), "error": a.error,
+                             "max_diff": None if d is None else int(d.max()),
+                             "values_differing": None if d is None else int((d > 0).sum())})
+        st_srv, st_seq = latency_stats(served), latency_stats(seq)
+        log(f"served @ {args.serve_qps} qps: {st_srv}; sequential: {st_seq}; identical {same}/{len(reqs)}")
+        # saturation: every request queued at t = 0
+        sat_srv = fresh(reqs, keep_time=False)
+        srv.run(sat_srv)
+        sat_seq = fresh(reqs, keep_time=False)
+        run_sequential(ctxs, sat_seq)
+        sat = {"served_frames_per_s": latency_stats(sat_srv)["frames_per_s"],
+               "sequential_frames_per_s": latency_stats(sat_seq)["frames_per_s"]}
+        log(f"saturation: {sat}")
+    finally:
+        srv.close()
+        for c in ctxs[:2]:
+            c.close()
+    rnd = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
+    return {
+        "qps": args.serve_qps, "requests": len(reqs), "mix": dict(W.SERVE_MIX),
+        "viewports": list(W.SERVE_SIZES), "volumes": [sv.spec.name for sv in scene],
+        "slots": args.serve_slots, "hbm_budget_bytes": 512 << 20,
+        "p50_ms": round(st_srv["p50_ms"], 3), "p99_ms": round(st_srv["p99_ms"], 3),
+        "frames_per_s": round(st_srv["frames_per_s"], 2),
+        "sequential_plain": rnd(st_seq), "served_detail": rnd(st_srv),
+        "batches": n_batches, "mean_batch": round(float(np.mean(sizes)), 2) if sizes else 0,
+        "saturation": rnd(sat), "identical_to_plain": f"{same}/{len(reqs)}", "mismatches": mism,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +262,14 @@ def main() -> int:
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed frames, copy this many MiB device-to-device once (a known-size "
                          "kernel against which FETCH_SIZE/WRITE_SIZE counters are scaled)")
+    ap.add_argument("--fit", action="store_true",
+                    help="measure transfer-function fitting instead (256^3 blob volume, 8 views at 512^2, "
+                         "256-entry LUT + opacity scale, 200 Adam steps, FD check of the adjoint at 64^2)")
+    ap.add_argument("--fit-steps", type=int, default=200)
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-qps", type=float, default=500.0)
+    ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--traffic", type=str, default=str(ROOT / "profiles" / "traffic_vr_body1024.json"),
                     help="measured HBM bytes per frame from rocprof counters (see the file), if present")
     args = ap.parse_args()
@@ -88,6 +278,9 @@ def main() -> int:
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1 (no multi-GPU path)"}))
         return 2
+
+    if args.fit:
+        return fit_main(args)
 
     import torch
 
@@ -207,8 +400,10 @@ def main() -> int:
                          f"oracle with 8-bit filter weights, {t_cpu:.1f}s on {ncores} threads"}
         log(f"cpu baseline: {cpu}")
 
+    served = None if args.no_serve else serve_section(args, ctx, spec, vol, mask, luts, labels)
+
     line = {
-        "metric": "VR fps 512^2x1559 i16 -> 1024^2 (plain), 4-label mask",
+        "metric": "VR fps 512\u00b2\u00d71559 i16\u21921024\u00b2 plain vs served p50/p99 @\u22641/255 err; HBM GB/s",
         "value": round(fps, 3),
         "unit": "frames/s",
         "n_gpus": 1,
@@ -231,6 +426,7 @@ def main() -> int:
             "bytes_per_frame": int(b_frame), "kernel": "k_vr"},
         "parity": parity,
         "cpu_baseline": cpu,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

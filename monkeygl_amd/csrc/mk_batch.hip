@@ -29,68 +29,15 @@
 #include <string>
 #include <vector>
 
+#include "mk_batch_dev.h"
 #include "mk_context.h"
-#include "mk_plane_dev.h"
-#include "mk_render_dev.h"
 
 namespace {
 
-constexpr int kTile = 16;
-constexpr int kBatchLabelSlots = 16;   // per-label accumulators per thread in LDS
-constexpr int kBatchLutLds = 1024;     // LUT entries staged in LDS
+using namespace mkb;
 
-enum : int { F_NEAREST = 1 };
-
-// Kernel variants; an item's variant fixes the template arguments of its
-// per-pixel code.
-enum Variant : int {
-    V_VR_MASK_LDS,
-    V_VR_MASK_GLB,
-    V_VR_LDS,
-    V_VR_GLB,
-    V_MIP_MASK,
-    V_MIP,
-    V_SURFACE,
-    V_PLANE_CPR,
-    V_COUNT
-};
-
-constexpr size_t lds_bytes(int v) {
-    return v == V_VR_MASK_LDS || v == V_VR_MASK_GLB
-               ? 68 * sizeof(float4) + kBatchLutLds * sizeof(float4) + kBatchLabelSlots * 256 * sizeof(float)
-           : v == V_VR_LDS || v == V_VR_GLB ? 68 * sizeof(float4) + kBatchLutLds * sizeof(float4)
-           : v == V_MIP_MASK || v == V_MIP ? 68 * sizeof(float4)
-                                            : 0;
-}
-
-struct BatchDesc {
-    int kind;      // MK_ITEM_*
-    int sub;       // MPR type for planes
-    int flags;
-    int tiles_x;
-    int width, height;
-    float4 aw[68];
-    union {
-        mkr::RenderArgs ra;
-        mkp::PlaneArgs pa;
-        mkp::CprArgs ca;
-    };
-};
-
-struct VariantRange {
-    int first, count;   // items [first, first + count) of the sorted descriptor array
-    uint32_t tiles;
-    uint32_t pad;
-};
-
-struct BatchHeader {
-    int n_items;
-    int pad;
-    VariantRange var[V_COUNT];
-};
-
-// Device block: header, first-tile table (per item, relative to its
-// variant's first item), descriptors sorted by variant.
+// Layout of a slot's descriptor block: header, first-tile table (per item,
+// relative to its variant's first item), descriptors sorted by variant.
 struct BatchLayout {
     size_t tile_base_off, desc_off;
     size_t bytes(int n) const { return desc_off + sizeof(BatchDesc) * (size_t)n; }
@@ -106,47 +53,12 @@ BatchLayout layout(int max_items) {
 template <int V>
 __global__ void __launch_bounds__(256) k_batch(const uint8_t* __restrict__ block, uint32_t tile_base_off,
                                                uint32_t desc_off) {
-    const BatchHeader& H = *(const BatchHeader*)block;
-    const VariantRange R = H.var[V];
-    const uint32_t g = blockIdx.x;
-    if (g >= R.tiles) return;
-    const uint32_t* base = (const uint32_t*)(block + tile_base_off);
-    const BatchDesc* items = (const BatchDesc*)(block + desc_off);
-    int lo = R.first, hi = R.first + R.count - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (base[mid] <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    // lo is the same for the whole workgroup; saying so lets the descriptor
-    // fields load through the scalar unit
-    lo = __builtin_amdgcn_readfirstlane(lo);
-    const BatchDesc& d = items[lo];
-    const uint32_t t = g - base[lo];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int x = (int)(t % (uint32_t)d.tiles_x) * kTile + (wave & 1) * 8 + (lane & 7);
-    const int y = (int)(t / (uint32_t)d.tiles_x) * kTile + (wave >> 1) * 8 + (lane >> 3);
-    if constexpr (V == V_PLANE_CPR) {
-        if (x >= d.width || y >= d.height) return;
-        const bool nn = d.flags & F_NEAREST;
-        if (d.kind == MK_ITEM_PLANE) {
-            if (d.sub == MK_MPR_AVERAGE) {
-                if (nn) mkp::plane_pixel<MK_MPR_AVERAGE, true>(d.pa, x, y);
-                else mkp::plane_pixel<MK_MPR_AVERAGE, false>(d.pa, x, y);
-            } else if (d.sub == MK_MPR_MIP) {
-                if (nn) mkp::plane_pixel<MK_MPR_MIP, true>(d.pa, x, y);
-                else mkp::plane_pixel<MK_MPR_MIP, false>(d.pa, x, y);
-            } else {
-                if (nn) mkp::plane_pixel<MK_MPR_MINIP, true>(d.pa, x, y);
-                else mkp::plane_pixel<MK_MPR_MINIP, false>(d.pa, x, y);
-            }
-        } else {
-            if (nn) mkp::cpr_pixel<true>(d.ca, x, y);
-            else mkp::cpr_pixel<false>(d.ca, x, y);
-        }
-    } else if constexpr (V == V_SURFACE) {
-        if (x >= d.width || y >= d.height) return;
-        mkr::surface_pixel<false>(d.ra, x, y);
+    const BatchDesc* d;
+    int x, y;
+    if (!batch_item<V>(block, tile_base_off, desc_off, d, x, y)) return;
+    if constexpr (V == V_SURFACE) {
+        if (x >= d->width || y >= d->height) return;
+        mkr::surface_pixel<false>(d->ra, x, y);
     } else {
         constexpr bool kMask = V == V_VR_MASK_LDS || V == V_VR_MASK_GLB || V == V_MIP_MASK;
         constexpr bool kLds = V == V_VR_MASK_LDS || V == V_VR_LDS;
@@ -155,13 +67,13 @@ __global__ void __launch_bounds__(256) k_batch(const uint8_t* __restrict__ block
         s.aw = smem;
         s.lut = smem + 68;
         s.aobj = (float*)(s.lut + kBatchLutLds);
-        for (int i = threadIdx.x; i < 68; i += blockDim.x) s.aw[i] = d.aw[i];
+        for (int i = threadIdx.x; i < 68; i += blockDim.x) s.aw[i] = d->aw[i];
         if constexpr (kLds)
-            for (int i = threadIdx.x; i < d.ra.nlut_lds; i += blockDim.x) s.lut[i] = d.ra.lut[i];
+            for (int i = threadIdx.x; i < d->ra.nlut_lds; i += blockDim.x) s.lut[i] = d->ra.lut[i];
         __syncthreads();
-        if (x >= d.width || y >= d.height) return;
-        if constexpr (V == V_MIP_MASK || V == V_MIP) mkr::mip_pixel<kMask, false>(d.ra, s, x, y);
-        else mkr::vr_pixel<256, kMask, kLds, false>(d.ra, s, x, y, threadIdx.x);
+        if (x >= d->width || y >= d->height) return;
+        if constexpr (V == V_MIP_MASK || V == V_MIP) mkr::mip_pixel<kMask, false>(d->ra, s, x, y);
+        else mkr::vr_pixel<256, kMask, kLds, false>(d->ra, s, x, y, threadIdx.x);
     }
 }
 
@@ -174,7 +86,7 @@ const void* variant_kernel(int v) {
     case V_MIP_MASK: return (const void*)k_batch<V_MIP_MASK>;
     case V_MIP: return (const void*)k_batch<V_MIP>;
     case V_SURFACE: return (const void*)k_batch<V_SURFACE>;
-    default: return (const void*)k_batch<V_PLANE_CPR>;
+    default: return plane_cpr_kernel();
     }
 }
 
