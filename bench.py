@@ -326,6 +326,15 @@ def main() -> int:
     log(f"{K} frames in {elapsed * 1e3:.1f} ms -> {fps:.1f} fps; kernel mean {kernel_ms:.3f} ms "
         f"(min {min(per):.3f}, max {max(per):.3f})")
 
+    # PCIe-inclusive: the synchronous entry point writing caller-owned host
+    # memory, as cu_render does (never reported as value)
+    host_frames = max(args.views, 12)
+    t = time.perf_counter()
+    for i in range(host_frames):
+        ctx.render(views[i % len(views)])
+    host_fps = host_frames / (time.perf_counter() - t)
+    log(f"synchronous host-output rate: {host_fps:.1f} fps")
+
     if args.calibrate_copy_mib:
         n = args.calibrate_copy_mib << 20
         a = torch.empty(n, dtype=torch.uint8, device="cuda")
@@ -420,6 +429,7 @@ def main() -> int:
                    "volume": list(spec.dims), "spacing": list(spec.spacing), "image": [W_, H_],
                    "views": args.views, "parallelism": "none (1 GPU)"},
         "kernel_ms": round(kernel_ms, 4),
+        "plain_host_output_fps": round(host_fps, 2),
         "roofline": None if achieved is None else {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
