@@ -9,6 +9,10 @@
 //   Distance_Point2Line (core/Methods.cpp:563-633) over Direction3d /
 //   Point3d (core/Direction.h, core/Point.h). Directions are built the way
 //   CPRInfo builds them, from two points.
+//   MPR plane helpers: the (dirH, dirV) overloads of Distance_Point2Plane and
+//   Projection_Point2Plane (core/Methods.cpp:582-598), GetTransferPoint
+//   (core/Methods.cpp:667-679), PlaneInfo::GetNormDirection / GetLeftTopPoint
+//   (core/PlaneInfo.h:37-45).
 // Used only to write golden vectors (tests/golden/make_golden.py) and to
 // cross-check the restatements where the reference is present.
 #include <cstring>
@@ -21,6 +25,7 @@
 #include "RenderInfo.h"
 #undef private
 #include "Methods.h"
+#include "PlaneInfo.h"
 #include "TransferFunction.h"
 
 using namespace MonkeyGL;
@@ -110,6 +115,36 @@ double ref_distance_point_to_line(const double* pt, const double* d0, const doub
 }
 double ref_patient_distance(const double* a, const double* b, const double* sp) {
     return P3(a).PatientDistanceTo(P3(b), P3(sp));
+}
+
+double ref_distance_point_to_plane_hv(const double* pt, const double* h0, const double* h1, const double* v0,
+                                      const double* v1, const double* pp) {
+    return Methods::Distance_Point2Plane(P3(pt), Direction3d(P3(h0), P3(h1)), Direction3d(P3(v0), P3(v1)), P3(pp));
+}
+void ref_projection_point_to_plane_hv(const double* pt, const double* h0, const double* h1, const double* v0,
+                                      const double* v1, const double* pp, double* o) {
+    out3(Methods::Projection_Point2Plane(P3(pt), Direction3d(P3(h0), P3(h1)), Direction3d(P3(v0), P3(v1)), P3(pp)), o);
+}
+void ref_transfer_point(const double* m9, const double* pt, double* o) {
+    double m[3][3];
+    for (int i = 0; i < 9; ++i) m[i / 3][i % 3] = m9[i];
+    out3(Methods::GetTransferPoint(m, P3(pt)), o);
+}
+// PlaneInfo with dirH = Direction3d(h0, h1), dirV = Direction3d(v0, v1): its
+// normal and the left-top corner for a centre point
+void ref_plane_info(const double* h0, const double* h1, const double* v0, const double* v1, int w, int h, double ps,
+                    const double* center, double* norm, double* left_top) {
+    PlaneInfo info;
+    info.m_dirH = Direction3d(P3(h0), P3(h1));
+    info.m_dirV = Direction3d(P3(v0), P3(v1));
+    info.m_nWidth = w;
+    info.m_nHeight = h;
+    info.m_fPixelSpacing = ps;
+    Direction3d n = info.GetNormDirection();
+    norm[0] = n.x();
+    norm[1] = n.y();
+    norm[2] = n.z();
+    out3(info.GetLeftTopPoint(P3(center)), left_top);
 }
 
 }  // extern "C"

@@ -9,6 +9,9 @@ fixed set of inputs and stores inputs and outputs as JSON next to this script:
   cpr_geometry.json   CPR geometry helpers (direction, cross, RotatePoint3D,
                       Projection_Point2Plane, Distance_Point2Line,
                       PatientDistanceTo) on seeded inputs
+  mpr_geometry.json   MPR plane helpers ((dirH, dirV) overloads of
+                      Distance_Point2Plane / Projection_Point2Plane,
+                      GetTransferPoint, PlaneInfo normal and left-top corner)
 
 Usage: sh oracle/build_ref.sh && python tests/golden/make_golden.py
 """
@@ -46,7 +49,46 @@ def load():
     L.ref_distance_point_to_line.argtypes = [vp, vp, vp, vp]
     L.ref_patient_distance.restype = C.c_double
     L.ref_patient_distance.argtypes = [vp, vp, vp]
+    L.ref_distance_point_to_plane_hv.restype = C.c_double
+    L.ref_distance_point_to_plane_hv.argtypes = [vp] * 6
+    L.ref_projection_point_to_plane_hv.argtypes = [vp] * 7
+    L.ref_transfer_point.argtypes = [vp, vp, vp]
+    L.ref_plane_info.argtypes = [vp, vp, vp, vp, i, i, C.c_double, vp, vp, vp]
     return L
+
+
+def mpr_geometry_cases(L):
+    rng = np.random.default_rng(78)
+
+    def v():
+        return rng.uniform(-50.0, 550.0, 3)
+
+    def hx(a):
+        return [float(x).hex() for x in np.atleast_1d(a)]
+
+    out = {"dist_plane_hv": [], "project_hv": [], "transfer": [], "plane_info": []}
+    o, o2 = np.zeros(3), np.zeros(3)
+    for _ in range(48):
+        pt, h0, h1, v0, v1, pp = (v() for _ in range(6))
+        r = L.ref_distance_point_to_plane_hv(pt.ctypes.data, h0.ctypes.data, h1.ctypes.data, v0.ctypes.data,
+                                             v1.ctypes.data, pp.ctypes.data)
+        out["dist_plane_hv"].append({"pt": hx(pt), "h0": hx(h0), "h1": hx(h1), "v0": hx(v0), "v1": hx(v1),
+                                     "pp": hx(pp), "out": hx(r)})
+        L.ref_projection_point_to_plane_hv(pt.ctypes.data, h0.ctypes.data, h1.ctypes.data, v0.ctypes.data,
+                                           v1.ctypes.data, pp.ctypes.data, o.ctypes.data)
+        out["project_hv"].append({"pt": hx(pt), "h0": hx(h0), "h1": hx(h1), "v0": hx(v0), "v1": hx(v1),
+                                  "pp": hx(pp), "out": hx(o)})
+        m = rng.uniform(-1.0, 1.0, 9)
+        L.ref_transfer_point(m.ctypes.data, pt.ctypes.data, o.ctypes.data)
+        out["transfer"].append({"m": hx(m), "pt": hx(pt), "out": hx(o)})
+        w, h = int(rng.integers(1, 1200)), int(rng.integers(1, 1200))
+        ps = float(rng.uniform(0.2, 1.5))
+        c = v()
+        L.ref_plane_info(h0.ctypes.data, h1.ctypes.data, v0.ctypes.data, v1.ctypes.data, w, h, ps, c.ctypes.data,
+                         o.ctypes.data, o2.ctypes.data)
+        out["plane_info"].append({"h0": hx(h0), "h1": hx(h1), "v0": hx(v0), "v1": hx(v1), "w": w, "h": h,
+                                  "ps": float(ps).hex(), "center": hx(c), "norm": hx(o), "left_top": hx(o2)})
+    return out
 
 
 def cpr_geometry_cases(L):
@@ -198,8 +240,10 @@ def main():
     (HERE / "render_info.json").write_text(json.dumps({"sequences": ri, "vr_sizes": sizes}))
     geo = cpr_geometry_cases(L)
     (HERE / "cpr_geometry.json").write_text(json.dumps(geo))
+    mpr = mpr_geometry_cases(L)
+    (HERE / "mpr_geometry.json").write_text(json.dumps(mpr))
     print(f"wrote {len(tf)} TF cases, {len(ri)} RenderInfo sequences, {len(sizes)} VR sizes, "
-          f"{sum(len(v) for v in geo.values())} CPR geometry vectors")
+          f"{sum(len(v) for v in geo.values())} CPR and {sum(len(v) for v in mpr.values())} MPR geometry vectors")
 
 
 if __name__ == "__main__":

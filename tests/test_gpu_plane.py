@@ -90,3 +90,47 @@ def test_cpr(gpu_ctx, vol, invert_z):
             ok, dmax = _hu_ok(got, ref)
             assert ok, dmax
         assert (got == -2048).any() or True
+
+
+def test_mpr_from_mpr_state(gpu_ctx, vol):
+    """Planes produced by the MPRInfo restatement after browsing, rotating the
+    crosshair and a thick slab, through mk_render_plane."""
+    from monkeygl_amd import mpr as M
+
+    gpu_ctx.set_volume(vol)
+    m = M.MPRInfo(DIMS, SPACING)
+    m.update_thickness(3.5)
+    m.browse(7.0, M.PLANE_AXIAL_OBLIQUE)
+    m.rotate_crosshair(27.0, M.PLANE_AXIAL_OBLIQUE)
+    m.rotate_crosshair(-14.0, M.PLANE_SAGITTAL_OBLIQUE)
+    for mode in (MK_MPR_AVERAGE, MK_MPR_MIP, MK_MPR_MINIP):
+        m.set_mpr_type(mode)
+        for pt in (M.PLANE_AXIAL_OBLIQUE, M.PLANE_SAGITTAL_OBLIQUE, M.PLANE_CORONAL_OBLIQUE):
+            for sampling in (MK_SAMPLE_NEAREST, MK_SAMPLE_LINEAR):
+                p = m.plane_params(pt, sampling=sampling)
+                got = gpu_ctx.render_plane(p)
+                if sampling == MK_SAMPLE_NEAREST:
+                    assert np.array_equal(got, O.render_plane(p, vol, precision="f32")), (mode, pt)
+                else:
+                    ok, dmax = _hu_ok(got, O.render_plane(p, vol))
+                    assert ok, (mode, pt, dmax)
+
+
+def test_cpr_from_cpr_state(gpu_ctx, vol):
+    """Stretched and straightened CPR geometry from the CPRInfo restatement."""
+    from monkeygl_amd import cpr as G
+
+    gpu_ctx.set_volume(vol)
+    info = G.CPRInfo()
+    info.set_spacing(SPACING)
+    info.set_cpr_line_voxel(W.centerline(DIMS, 120, seed=6))
+    for which, ang in ((G.PLANE_STRETCHED_CPR, 40.0), (G.PLANE_STRAIGHTENED_CPR, -25.0)):
+        info.rotate_cpr(ang, which)
+        pts, dirs, w, h = info.get_cpr_info(which)
+        for sampling in (MK_SAMPLE_NEAREST, MK_SAMPLE_LINEAR):
+            got = gpu_ctx.render_cpr(w, pts, dirs, False, sampling)
+            if sampling == MK_SAMPLE_NEAREST:
+                assert np.array_equal(got, O.render_cpr(w, pts, dirs, False, vol, sampling, precision="f32")), which
+            else:
+                ok, dmax = _hu_ok(got, O.render_cpr(w, pts, dirs, False, vol, sampling, precision="f64"))
+                assert ok, (which, dmax)
