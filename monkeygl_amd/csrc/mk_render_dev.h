@@ -155,16 +155,94 @@ __device__ inline float3 light_dir(const RenderArgs& A) {
 }
 
 // Normalized central-difference gradient one voxel apart (kernel.cu:60-67);
-// a zero gradient gives NaN, which fails every "> 0" test downstream. One
-// axis per trip keeps 16 loads in flight instead of 48.
+// a zero gradient gives NaN, which fails every "> 0" test downstream.
+// The six samples at p +- sv (sv = 1/N per axis) sit one texel from p, so they
+// share p's fractional weights: each difference is the trilinear
+// interpolation, with p's weights, of voxel differences two apart. That reads
+// the 32 distinct voxels of the 4x2x2, 2x4x2 and 2x2x4 neighbourhoods once
+// instead of 48 voxels through six separate samples. Brick offsets are
+// separable per axis (brick index * 512 plus the in-brick position), so the
+// 32 addresses are sums of 12 per-axis terms; clamping is per index, as the
+// texture's clamp addressing is. The common 32768/32767 scale cancels in the
+// normalization and is not applied.
+__device__ inline void grad_axis_base(float f, int n, int& i, float& a) {
+    float xt = f * (float)n - 0.5f;
+    float fl = floorf(xt);
+    a = xt - fl;
+    fl = fminf(fmaxf(fl, -3.0f), (float)(n + 2));
+    i = (int)fl;
+}
+
 __device__ inline float3 gradient(const RenderArgs& A, const float p[3]) {
+    const MkVolView& v = A.v;
+    int ix, iy, iz;
+    float ax, ay, az;
+    grad_axis_base(p[0], v.nx, ix, ax);
+    grad_axis_base(p[1], v.ny, iy, ay);
+    grad_axis_base(p[2], v.nz, iz, az);
+    const uint32_t sy = (uint32_t)v.nbx * 512u, sz = (uint32_t)v.nbx * (uint32_t)v.nby * 512u;
+    uint32_t OX[4], OY[4], OZ[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int x = mk_clampi(ix - 1 + k, 0, v.nx - 1);
+        const int y = mk_clampi(iy - 1 + k, 0, v.ny - 1);
+        const int z = mk_clampi(iz - 1 + k, 0, v.nz - 1);
+        OX[k] = (uint32_t)(x >> 3) * 512u + (uint32_t)(x & 7);
+        OY[k] = (uint32_t)(y >> 3) * sy + (uint32_t)(y & 7) * 8u;
+        OZ[k] = (uint32_t)(z >> 3) * sz + (uint32_t)(z & 7) * 64u;
+    }
+#define MK_V(a, b, c) mk_rawo(v, OX[a] + OY[b] + OZ[c])
+    // center 2x2x2 block (indices 1..2 on every axis)
+    float c[2][2][2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) c[k][j][i] = MK_V(1 + i, 1 + j, 1 + k);
+    float ex[2][2][2], ey[2][2][2], ez[2][2][2];   // the outer layers at index 0 and 3
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            ex[k][j][0] = MK_V(0, 1 + j, 1 + k);
+            ex[k][j][1] = MK_V(3, 1 + j, 1 + k);
+            ey[k][j][0] = MK_V(1 + j, 0, 1 + k);
+            ey[k][j][1] = MK_V(1 + j, 3, 1 + k);
+            ez[k][j][0] = MK_V(1 + j, 1 + k, 0);
+            ez[k][j][1] = MK_V(1 + j, 1 + k, 3);
+        }
+#undef MK_V
+    // x: lerp(V2 - V0, V3 - V1, ax) per (y, z), then bilinear in (ay, az)
+    float dx[2][2], dy[2][2], dz[2][2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const float lo = c[k][j][1] - ex[k][j][0], hi = ex[k][j][1] - c[k][j][0];
+            dx[k][j] = lo + ax * (hi - lo);
+            // y: index j of ey runs over x (1 + j), k over z (1 + k)
+            const float loy = c[k][1][j] - ey[k][j][0], hiy = ey[k][j][1] - c[k][0][j];
+            dy[k][j] = loy + ay * (hiy - loy);
+            // z: ez[k][j] is at x = 1 + j, y = 1 + k
+            const float loz = c[1][k][j] - ez[k][j][0], hiz = ez[k][j][1] - c[0][k][j];
+            dz[k][j] = loz + az * (hiz - loz);
+        }
     float N[3];
-#pragma unroll 1
-    for (int i = 0; i < 3; ++i) {
-        float q0[3] = {p[0], p[1], p[2]}, q1[3] = {p[0], p[1], p[2]};
-        q0[i] += A.sv[i];
-        q1[i] -= A.sv[i];
-        N[i] = mk_sample_hu(A.v, q0[0], q0[1], q0[2]) - mk_sample_hu(A.v, q1[0], q1[1], q1[2]);
+    {   // dx[k][j]: j over y, k over z
+        const float a0 = dx[0][0] + ay * (dx[0][1] - dx[0][0]);
+        const float a1 = dx[1][0] + ay * (dx[1][1] - dx[1][0]);
+        N[0] = a0 + az * (a1 - a0);
+    }
+    {   // dy[k][j]: j over x, k over z
+        const float a0 = dy[0][0] + ax * (dy[0][1] - dy[0][0]);
+        const float a1 = dy[1][0] + ax * (dy[1][1] - dy[1][0]);
+        N[1] = a0 + az * (a1 - a0);
+    }
+    {   // dz[k][j]: j over x, k over y
+        const float a0 = dz[0][0] + ax * (dz[0][1] - dz[0][0]);
+        const float a1 = dz[1][0] + ax * (dz[1][1] - dz[1][0]);
+        N[2] = a0 + ay * (a1 - a0);
     }
     if (A.invert_z) N[2] = -N[2];
     float r = rsqrtf(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
