@@ -173,33 +173,60 @@ __device__ inline void grad_axis_base(float f, int n, int& i, float& a) {
     i = (int)fl;
 }
 
-__device__ inline float3 gradient(const RenderArgs& A, const float p[3]) {
-    const MkVolView& v = A.v;
+// A trilinear sample that keeps what the gradient at the same point reuses:
+// the unclamped floor index and weight per axis and the eight corner voxels
+// (the gradient's centre block). Same arithmetic as mk_sample_hu.
+struct TriSample {
     int ix, iy, iz;
     float ax, ay, az;
-    grad_axis_base(p[0], v.nx, ix, ax);
-    grad_axis_base(p[1], v.ny, iy, ay);
-    grad_axis_base(p[2], v.nz, iz, az);
+    float c[2][2][2];   // [z][y][x], raw voxels (-32768 read as -32767)
+};
+
+__device__ inline float sample_keep(const MkVolView& v, const float p[3], TriSample& T) {
+    grad_axis_base(p[0], v.nx, T.ix, T.ax);
+    grad_axis_base(p[1], v.ny, T.iy, T.ay);
+    grad_axis_base(p[2], v.nz, T.iz, T.az);
+    const int x0 = mk_clampi(T.ix, 0, v.nx - 1), x1 = mk_clampi(T.ix + 1, 0, v.nx - 1);
+    const int y0 = mk_clampi(T.iy, 0, v.ny - 1), y1 = mk_clampi(T.iy + 1, 0, v.ny - 1);
+    const int z0 = mk_clampi(T.iz, 0, v.nz - 1), z1 = mk_clampi(T.iz + 1, 0, v.nz - 1);
+    const uint32_t o = mk_voxel_offset(v, x0, y0, z0);
+    const uint32_t dx = x1 == x0 ? 0u : ((x0 & 7) == 7 ? 512u - 7u : 1u);
+    const uint32_t dy = y1 == y0 ? 0u : ((y0 & 7) == 7 ? (uint32_t)v.nbx * 512u - 56u : 8u);
+    const uint32_t dz = z1 == z0 ? 0u : ((z0 & 7) == 7 ? (uint32_t)v.nbx * (uint32_t)v.nby * 512u - 448u : 64u);
+    T.c[0][0][0] = mk_rawo(v, o);
+    T.c[0][0][1] = mk_rawo(v, o + dx);
+    T.c[0][1][0] = mk_rawo(v, o + dy);
+    T.c[0][1][1] = mk_rawo(v, o + dy + dx);
+    T.c[1][0][0] = mk_rawo(v, o + dz);
+    T.c[1][0][1] = mk_rawo(v, o + dz + dx);
+    T.c[1][1][0] = mk_rawo(v, o + dz + dy);
+    T.c[1][1][1] = mk_rawo(v, o + dz + dy + dx);
+    const float c00 = T.c[0][0][0] + T.ax * (T.c[0][0][1] - T.c[0][0][0]);
+    const float c10 = T.c[0][1][0] + T.ax * (T.c[0][1][1] - T.c[0][1][0]);
+    const float c01 = T.c[1][0][0] + T.ax * (T.c[1][0][1] - T.c[1][0][0]);
+    const float c11 = T.c[1][1][0] + T.ax * (T.c[1][1][1] - T.c[1][1][0]);
+    const float c0 = c00 + T.ay * (c10 - c00);
+    const float c1 = c01 + T.ay * (c11 - c01);
+    const float c = c0 + T.az * (c1 - c0);
+    return c * (32768.0f / 32767.0f);
+}
+
+// The gradient from a kept sample: 24 more voxels around its centre block.
+__device__ inline float3 gradient_kept(const RenderArgs& A, const TriSample& T) {
+    const MkVolView& v = A.v;
+    const float ax = T.ax, ay = T.ay, az = T.az;
     const uint32_t sy = (uint32_t)v.nbx * 512u, sz = (uint32_t)v.nbx * (uint32_t)v.nby * 512u;
     uint32_t OX[4], OY[4], OZ[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int x = mk_clampi(ix - 1 + k, 0, v.nx - 1);
-        const int y = mk_clampi(iy - 1 + k, 0, v.ny - 1);
-        const int z = mk_clampi(iz - 1 + k, 0, v.nz - 1);
+        const int x = mk_clampi(T.ix - 1 + k, 0, v.nx - 1);
+        const int y = mk_clampi(T.iy - 1 + k, 0, v.ny - 1);
+        const int z = mk_clampi(T.iz - 1 + k, 0, v.nz - 1);
         OX[k] = (uint32_t)(x >> 3) * 512u + (uint32_t)(x & 7);
         OY[k] = (uint32_t)(y >> 3) * sy + (uint32_t)(y & 7) * 8u;
         OZ[k] = (uint32_t)(z >> 3) * sz + (uint32_t)(z & 7) * 64u;
     }
 #define MK_V(a, b, c) mk_rawo(v, OX[a] + OY[b] + OZ[c])
-    // center 2x2x2 block (indices 1..2 on every axis)
-    float c[2][2][2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int i = 0; i < 2; ++i) c[k][j][i] = MK_V(1 + i, 1 + j, 1 + k);
     float ex[2][2][2], ey[2][2][2], ez[2][2][2];   // the outer layers at index 0 and 3
 #pragma unroll
     for (int k = 0; k < 2; ++k)
@@ -213,6 +240,7 @@ __device__ inline float3 gradient(const RenderArgs& A, const float p[3]) {
             ez[k][j][1] = MK_V(1 + j, 1 + k, 3);
         }
 #undef MK_V
+    const float(&c)[2][2][2] = T.c;
     // x: lerp(V2 - V0, V3 - V1, ax) per (y, z), then bilinear in (ay, az)
     float dx[2][2], dy[2][2], dz[2][2];
 #pragma unroll
@@ -247,6 +275,12 @@ __device__ inline float3 gradient(const RenderArgs& A, const float p[3]) {
     if (A.invert_z) N[2] = -N[2];
     float r = rsqrtf(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
     return make_float3(N[0] * r, N[1] * r, N[2] * r);
+}
+
+__device__ inline float3 gradient(const RenderArgs& A, const float p[3]) {
+    TriSample T;
+    (void)sample_keep(A.v, p, T);
+    return gradient_kept(A, T);
 }
 
 // Footprint marking for the measurement variant: every voxel the reference
@@ -375,6 +409,7 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
             float a = 0.f;
             float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
             int label = 0;
+            TriSample tri;
             if (!empty_shift) {
                 if (MASK) label = (TOUCH || cur_label == 0xFF) ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label;
                 if (TOUCH) {
@@ -382,7 +417,7 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
                     touch_footprint(A, p[0], p[1], p[2]);
                 }
                 const float4 aw = s.aw[label];
-                float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
+                float t = sample_keep(A.v, p, tri);
                 t = (t - aw.z) / aw.y + 0.5f;
                 t = t > 1.0f ? 1.0f : t;
                 col = lut_sample<LDS_LUT>(A, s.lut, label, t);
@@ -406,7 +441,7 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
                 if (*slot < cap) {
                     // tracing (kernel.cu:49-81)
                     if (TOUCH) touch_gradient(A, p);
-                    const float3 N = gradient(A, p);
+                    const float3 N = gradient_kept(A, tri);
                     const float3 L = light_dir(A);
                     const float d = N.x * L.x + N.y * L.y + N.z * L.z;
                     float k = 0.35f;
