@@ -234,6 +234,19 @@ def serve_section(args, body_ctx, spec, vol, mask, luts, labels):
         log(f"saturation: {sat}")
     finally:
         srv.close()
+    # the same stream with the reference's wire format: base64 PNG per result
+    srv_enc = RenderServer(ctxs, hbm_budget=1024 << 20, nslots=args.serve_slots, max_items=64, max_tiles=65536,
+                           encode=True)
+    try:
+        srv_enc.run(fresh(reqs[:48], keep_time=False))
+        enc_reqs = fresh(reqs)
+        srv_enc.run(enc_reqs)
+        st_enc = latency_stats(enc_reqs)
+        text_bytes = sum(len(r.result) for r in enc_reqs if r.error is None)
+        log(f"served with PNG+base64 @ {args.serve_qps} qps: {st_enc}; {text_bytes / max(1, st_enc['n']) / 1e3:.1f} "
+            f"kB text per result")
+    finally:
+        srv_enc.close()
         for c in ctxs[:2]:
             c.close()
     rnd = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
@@ -246,6 +259,10 @@ def serve_section(args, body_ctx, spec, vol, mask, luts, labels):
         "sequential_plain": rnd(st_seq), "served_detail": rnd(st_srv),
         "batches": n_batches, "mean_batch": round(float(np.mean(sizes)), 2) if sizes else 0,
         "saturation": rnd(sat), "identical_to_plain": f"{same}/{len(reqs)}", "mismatches": mism,
+        "encoded": {"format": "base64 PNG per result (RGB frames, int16 images as RGBA of width W/2)",
+                    "p50_ms": round(st_enc["p50_ms"], 3), "p99_ms": round(st_enc["p99_ms"], 3),
+                    "frames_per_s": round(st_enc["frames_per_s"], 2),
+                    "mean_text_bytes": int(text_bytes / max(1, st_enc["n"]))},
     }
 
 
@@ -368,6 +385,23 @@ def main() -> int:
     if tp.exists() and args.size == 1024:
         traffic = json.loads(tp.read_text()).get("hbm_bytes_per_frame")
 
+    # device PNG + base64 of the headline frame (the reference encodes on the host)
+    from monkeygl_amd.encode import Encoder
+    enc = Encoder(W_, H_, 3)
+    ctx.render_async(views[1 % len(views)], out, stream=sh)
+    for _ in range(3):
+        enc.encode_async(out, W_, H_, 3, stream=sh)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(10):
+        enc.encode_async(out, W_, H_, 3, stream=sh)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    encode = {"gpu_png_base64_ms": round(e0.elapsed_time(e1) / 10, 4), "png_bytes": int(enc.lens[0].item()),
+              "base64_bytes": int(enc.lens[1].item()), "image": [W_, H_, 3]}
+    log(f"device PNG+base64 of a {W_}x{H_} frame: {encode}")
+    del enc
+
     # parity band and CPU baseline on view 1 (an oblique one)
     p = views[1 % len(views)]
     ctx.render_async(p, out, stream=torch.cuda.current_stream().cuda_stream)
@@ -408,6 +442,22 @@ def main() -> int:
                "sample": f"{rows_done / H_:.2f} frames ({rows_done} rows of {W_}x{H_} over {nviews} views), float "
                          f"oracle with 8-bit filter weights, {t_cpu:.1f}s on {ncores} threads"}
         log(f"cpu baseline: {cpu}")
+        # the reference's own encoder (core/fpng/fpng.cpp, built into oracle/_ref) on the same frame
+        ref_lib = ROOT / "oracle" / "_ref" / "libmkref.so"
+        if ref_lib.exists():
+            import ctypes as C
+            L = C.CDLL(str(ref_lib))
+            L.ref_fpng_encode.restype = C.c_ulonglong
+            L.ref_fpng_encode.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_ulonglong]
+            frame = np.ascontiguousarray(gpu_img)
+            buf = np.zeros(frame.size * 2 + 4096, np.uint8)
+            t = time.perf_counter()
+            for _ in range(10):
+                nb = L.ref_fpng_encode(frame.ctypes.data, W_, H_, 3, buf.ctypes.data, buf.size)
+            encode["fpng_cpu_ms"] = round((time.perf_counter() - t) / 10 * 1e3, 4)
+            encode["fpng_bytes"] = int(nb)
+            encode["fpng_cores"] = 1
+            log(f"fpng on the host: {encode}")
 
     served = None if args.no_serve else serve_section(args, ctx, spec, vol, mask, luts, labels)
 
@@ -436,6 +486,7 @@ def main() -> int:
             "bytes_per_frame": int(b_frame), "kernel": "k_vr"},
         "parity": parity,
         "cpu_baseline": cpu,
+        "encode": encode,
         "served": served,
     }
     print(json.dumps(line), flush=True)

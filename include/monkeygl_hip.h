@@ -234,6 +234,34 @@ mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, const float
                          float opacity_scale, const float* dev_grad_rgb, double* dev_grad_lut,
                          double* dev_grad_scale, void* stream);
 
+/* ---- output encoding: PNG + base64 on the device ------------------------- */
+/* Replaces the host encoding of served images: fpng_encode_image_to_memory
+ * of the RGB VR frame (core/HelloMonkey.cpp:333-365) and of an int16 plane
+ * read as an RGBA image of width W/2 (core/HelloMonkey.cpp:212-259), then
+ * Base64::Encode (core/Base64.hpp). The PNG differs in its bytes from fpng's
+ * but decodes to the same pixels (8-bit RGB or RGBA, one IDAT chunk, zlib
+ * stream with one fixed-Huffman deflate block). Rows hold at most 24576 bytes
+ * (width * channels), height at most 65536. */
+
+/* Upper bound of the encoded PNG size (0 for unsupported dims). */
+uint64_t mk_png_max_bytes(int width, int height, int channels);
+/* Device workspace the encoder needs (256-byte aligned). */
+uint64_t mk_png_workspace_bytes(int width, int height, int channels);
+/* Encodes width x height x channels (3 or 4) bytes at dev_pixels into dev_png
+ * (capacity >= mk_png_max_bytes, 4-byte aligned) on the stream; the length is
+ * written to *dev_png_len (device memory) when the stream reaches it. Errors
+ * of the arguments are reported through mk_last_error(NULL). */
+mk_status mk_png_encode_async(const uint8_t* dev_pixels, int width, int height, int channels, void* dev_workspace,
+                              uint64_t workspace_bytes, uint8_t* dev_png, uint64_t png_capacity,
+                              uint64_t* dev_png_len, void* stream);
+/* 4 * ceil(n / 3). */
+uint64_t mk_base64_max_bytes(uint64_t n);
+/* Base64 (standard alphabet, '=' padding, no line breaks) of the
+ * min(*dev_in_len, in_capacity) bytes at dev_in, with the length read on the
+ * device, so it can follow mk_png_encode_async on the same stream. */
+mk_status mk_base64_encode_async(const uint8_t* dev_in, const uint64_t* dev_in_len, uint64_t in_capacity,
+                                 uint8_t* dev_out, uint64_t out_capacity, uint64_t* dev_out_len, void* stream);
+
 /* ---- batched serving launches (no reference counterpart) ----------------- */
 /* The render server batches requests over several resident volumes into one
  * launch. Outputs go to a caller-owned device arena at each item's offset:

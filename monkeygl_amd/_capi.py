@@ -28,7 +28,8 @@ SYMBOLS = [
     "mk_copy_mask_data", "mk_copy_mask_data_device", "mk_clear_mask", "mk_set_transfer_func",
     "mk_render", "mk_render_async", "mk_render_footprint", "mk_render_plane", "mk_render_plane_async",
     "mk_render_cpr", "mk_render_cpr_async", "mk_marching_cubes", "mk_marching_cubes_async",
-    "mk_tf_forward", "mk_tf_backward", "mk_batcher_create", "mk_batcher_destroy", "mk_batcher_last_error",
+    "mk_tf_forward", "mk_tf_backward", "mk_png_max_bytes", "mk_png_workspace_bytes", "mk_png_encode_async",
+    "mk_base64_max_bytes", "mk_base64_encode_async", "mk_batcher_create", "mk_batcher_destroy", "mk_batcher_last_error",
     "mk_batch_submit", "mk_batch_query", "mk_batch_wait",
 ]
 
@@ -119,6 +120,11 @@ def load() -> C.CDLL:
         "mk_marching_cubes_async": (st, [vp, vp, C.c_float, vp, C.c_uint64, vp, vp]),
         "mk_tf_forward": (st, [vp, C.POINTER(RenderParams), vp, i, C.c_float, vp, vp]),
         "mk_tf_backward": (st, [vp, C.POINTER(RenderParams), vp, i, C.c_float, vp, vp, vp, vp]),
+        "mk_png_max_bytes": (C.c_uint64, [i, i, i]),
+        "mk_png_workspace_bytes": (C.c_uint64, [i, i, i]),
+        "mk_png_encode_async": (st, [vp, i, i, i, vp, C.c_uint64, vp, C.c_uint64, vp, vp]),
+        "mk_base64_max_bytes": (C.c_uint64, [C.c_uint64]),
+        "mk_base64_encode_async": (st, [vp, vp, C.c_uint64, vp, C.c_uint64, vp, vp]),
         "mk_batcher_create": (st, [C.POINTER(vp), i, i, i, i, C.POINTER(vp)]),
         "mk_batcher_destroy": (None, [vp]),
         "mk_batcher_last_error": (C.c_char_p, [vp]),

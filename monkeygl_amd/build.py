@@ -37,7 +37,7 @@ HIP_FLAGS = [
 NO_CONTRACT = {"mk_plane.hip", "mk_mc.hip", "mk_batch_plane.hip"}
 
 HIP_SOURCES = ["mk_volume.hip", "mk_render.hip", "mk_plane.hip", "mk_mc.hip", "mk_fit.hip", "mk_batch.hip",
-               "mk_batch_plane.hip"]
+               "mk_batch_plane.hip", "mk_encode.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -1,0 +1,651 @@
+// mk_encode.hip -- PNG and base64 encoding of rendered images on the GPU.
+//
+// The reference encodes every served image on the host: VR frames as RGB PNG
+// and int16 MPR/CPR images as RGBA PNG of width W/2 (the int16 buffer read as
+// bytes), both through fpng, then base64 (core/HelloMonkey.cpp:212-259,
+// 333-413; core/Base64.hpp). Here the image stays in HBM and the PNG is built
+// there; any PNG decoder returns the same pixels, which is the contract the
+// client (a PNG decoder after atob) relies on.
+//
+// Format. One IDAT chunk holding a zlib stream with a single fixed-Huffman
+// deflate block. Each row gets the PNG filter (None, Sub, Up, Average, Paeth)
+// with the smallest sum of |signed byte| (the usual heuristic); the filtered
+// row is coded as runs of equal bytes: a run's first byte is a literal and the
+// rest are distance-1 copies in matches of up to 258 (a remainder below 3
+// stays literal). Rows never reference each other, so every row is coded
+// independently: the bit length of each row, an exclusive scan, then every
+// row writes its bits at its offset. Adler-32 is combined from per-row sums,
+// CRC-32 from per-block raw CRCs (CRC is linear: a block's contribution is
+// moved past the following bytes by multiplying with x^(8n) mod P).
+//
+// Launch sequence (one stream, no host round trip): zero the output ->
+// filter -> count -> scan (header bytes, row offsets) -> emit rows -> tail
+// (end-of-block code, Adler-32) -> block CRCs -> combine (CRC, IEND, length).
+// The encoded length is left in device memory, so base64 can follow on the
+// same stream. Rows are limited to 24576 bytes: the count and emit kernels
+// hold a filtered row and its bits in LDS.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "mk_context.h"
+
+namespace {
+
+constexpr int kMaxRowBytes = 24576;          // width * channels (row + its bits fit in LDS)
+constexpr uint32_t kCrcPoly = 0xEDB88320u;   // reflected CRC-32 polynomial
+constexpr uint32_t kAdlerMod = 65521u;
+constexpr int kCrcBlock = 4096;              // bytes per CRC workgroup
+constexpr uint64_t kHeadBytes = 8 + 25 + 8;  // signature, IHDR, IDAT length + type
+constexpr uint32_t kDeflateStartBit = (uint32_t)(kHeadBytes + 2) * 8;  // after the zlib header
+
+// ---- bit codes of the fixed Huffman table (RFC 1951 3.2.6) -----------------
+
+__host__ __device__ inline uint32_t bit_reverse(uint32_t c, int n) {
+    uint32_t r = 0;
+    for (int i = 0; i < n; ++i) r |= ((c >> i) & 1u) << (n - 1 - i);
+    return r;
+}
+
+// Literal byte v: (bits as written LSB first, length).
+__device__ inline void lit_code(uint32_t v, uint32_t& bits, int& len) {
+    if (v < 144) {
+        len = 8;
+        bits = bit_reverse(0x30u + v, 8);
+    } else {
+        len = 9;
+        bits = bit_reverse(0x190u + (v - 144u), 9);
+    }
+}
+
+__device__ inline int lit_len(uint32_t v) { return v < 144 ? 8 : 9; }
+
+// Length L in 3..258 with distance 1: symbol code, extra bits, distance code
+// 0 (five zero bits).
+__device__ inline void match_code(int L, uint32_t& bits, int& len) {
+    int sym, nextra, base;
+    if (L == 258) {
+        sym = 285; nextra = 0; base = 258;
+    } else if (L <= 10) {
+        sym = 254 + L; nextra = 0; base = L;
+    } else {
+        // groups of four symbols with 1..5 extra bits, starting at 11, 19, 35, 67, 131
+        int e = 1, start = 11, s0 = 265;
+        while (L >= start + (4 << e)) { start += 4 << e; s0 += 4; ++e; }
+        const int k = (L - start) >> e;
+        sym = s0 + k; nextra = e; base = start + (k << e);
+    }
+    uint32_t code;
+    int clen;
+    if (sym <= 279) { code = (uint32_t)(sym - 256); clen = 7; }
+    else { code = 0xC0u + (uint32_t)(sym - 280); clen = 8; }
+    bits = bit_reverse(code, clen) | ((uint32_t)(L - base) << clen);
+    len = clen + nextra + 5;   // distance code 0: 00000
+}
+
+__device__ inline int match_len(int L) {
+    uint32_t b;
+    int n;
+    match_code(L, b, n);
+    return n;
+}
+
+// Bits of a run of n equal bytes v (n >= 1): one literal, then n-1 copies.
+__device__ inline uint32_t run_bits(uint32_t v, int n) {
+    const int ll = lit_len(v);
+    uint32_t t = (uint32_t)ll;
+    int m = n - 1;
+    t += (uint32_t)(m / 258) * (uint32_t)match_len(258);
+    const int r = m % 258;
+    t += r >= 3 ? (uint32_t)match_len(r) : (uint32_t)(r * ll);
+    return t;
+}
+
+// ---- GF(2) arithmetic for CRC combination (reflected representation) -------
+
+__host__ __device__ inline uint32_t gf2_mulmod(uint32_t a, uint32_t b) {
+    // a * b mod P, bit 31 = x^0
+    uint32_t p = 0;
+    for (int i = 0; i < 32; ++i) {
+        if (a & (0x80000000u >> i)) p ^= b;
+        b = (b & 1u) ? (b >> 1) ^ kCrcPoly : b >> 1;
+    }
+    return p;
+}
+
+// x^(8 n) mod P by squaring from x^8.
+__host__ __device__ inline uint32_t x_pow_8n(uint64_t n) {
+    uint32_t result = 0x80000000u;   // 1
+    uint32_t sq = 0x00800000u;       // x^8
+    while (n) {
+        if (n & 1) result = gf2_mulmod(result, sq);
+        sq = gf2_mulmod(sq, sq);
+        n >>= 1;
+    }
+    return result;
+}
+
+__device__ inline uint32_t crc_update_raw(uint32_t c, uint8_t byte, const uint32_t* table) {
+    return table[(c ^ byte) & 0xFFu] ^ (c >> 8);
+}
+
+// ---- kernels -----------------------------------------------------------------
+
+struct PngWs {
+    uint8_t* filt;      // H * (1 + stride)
+    uint32_t* row_bits; // H
+    uint64_t* row_s;    // H: sum of filtered bytes
+    uint64_t* row_t;    // H: sum of (len - k) * byte, k = 0..len-1
+    uint64_t* row_off;  // H: bit offset of each row
+    uint64_t* totals;   // [0] deflate end bit, [1] zlib length, [2] png length, [3] adler
+    uint32_t* crc_part; // per CRC block
+};
+
+__global__ void __launch_bounds__(256) k_zero_words(uint32_t* p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+// PNG filter per row; one workgroup per row.
+__device__ inline int paeth(int a, int b, int c) {
+    const int p = a + b - c;
+    const int pa = abs(p - a), pb = abs(p - b), pc = abs(p - c);
+    return (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
+}
+
+__device__ inline uint8_t filt_byte(int f, const uint8_t* cur, const uint8_t* prev, int i, int bpp) {
+    const int x = cur[i];
+    const int a = i >= bpp ? cur[i - bpp] : 0;
+    const int b = prev ? prev[i] : 0;
+    const int c = (prev && i >= bpp) ? prev[i - bpp] : 0;
+    switch (f) {
+    case 0: return (uint8_t)x;
+    case 1: return (uint8_t)(x - a);
+    case 2: return (uint8_t)(x - b);
+    case 3: return (uint8_t)(x - ((a + b) >> 1));
+    default: return (uint8_t)(x - paeth(a, b, c));
+    }
+}
+
+// Runs of one filtered row (in LDS), swept by one wave in windows of 64
+// bytes. A run starts where the byte differs from the previous one (and at
+// the row start). In each window: the run carried from earlier windows closes
+// at the window's first start; runs starting in the window close at the next
+// start in the window; the window's last start opens the carried run. The
+// run open at the row's end closes there. Counting and emitting use this same
+// parse, so a row's bits match its count. fn(lane_run_start, value, length,
+// carried) is called for every closed run on the lane that owns it (lane 0
+// for carried runs); it returns the run's bits for the offset bookkeeping.
+template <typename Fn>
+__device__ inline uint32_t sweep_row(const uint8_t* row, int n, Fn fn) {
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    int open_start = 0, open_val = row[0];
+    for (int w0 = 0; w0 < n; w0 += 64) {
+        const int i = w0 + lane;
+        const bool in = i < n;
+        const int v = in ? (int)row[i] : -1;
+        const int pv = i == 0 ? -2 : (in ? (int)row[i - 1] : -1);
+        const bool start = in && v != pv;
+        const uint64_t smask = __ballot(start);
+        uint32_t carried = 0;
+        if (smask) {
+            const int first = w0 + __ffsll((long long)smask) - 1;
+            if (!(w0 == 0 && first == 0)) {
+                const int len = first - open_start;
+                carried = run_bits((uint32_t)open_val, len);
+                if (lane == 0) fn(base, open_val, len);
+            }
+        }
+        uint32_t my_bits = 0;
+        int my_len = 0;
+        if (start) {
+            const uint64_t later = lane == 63 ? 0ull : (smask >> (lane + 1)) << (lane + 1);
+            if (later) {
+                my_len = w0 + __ffsll((long long)later) - 1 - i;
+                my_bits = run_bits((uint32_t)v, my_len);
+            }
+        }
+        uint32_t incl = my_bits;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        const uint32_t total = __shfl(incl, 63, 64);
+        if (my_len) fn(base + carried + incl - my_bits, v, my_len);
+        base += carried + total;
+        if (smask) {
+            const int last = w0 + 63 - __clzll((long long)smask);
+            open_start = last;
+            open_val = row[last];
+        }
+    }
+    const uint32_t tail = run_bits((uint32_t)open_val, n - open_start);
+    if (lane == 0) fn(base, open_val, n - open_start);
+    return base + tail;
+}
+
+// Filter, count and Adler partials of one row; one workgroup of 256 per row,
+// the filtered row staged in LDS for the run sweep of wave 0.
+__global__ void __launch_bounds__(256) k_png_filter_count(const uint8_t* __restrict__ img, int width, int height,
+                                                          int ch, PngWs ws) {
+    extern __shared__ uint8_t s_row[];
+    const int r = blockIdx.x;
+    const int stride = width * ch;
+    const int n = stride + 1;
+    const uint8_t* cur = img + (size_t)r * stride;
+    const uint8_t* prev = r > 0 ? cur - stride : nullptr;
+    __shared__ uint32_t s_sum[5][4];
+    __shared__ uint64_t s_ad[2][4];
+    __shared__ int s_best;
+    uint32_t acc[5] = {0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < stride; i += blockDim.x)
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const uint8_t v = filt_byte(f, cur, prev, i, ch);
+            acc[f] += v < 128 ? v : 256u - v;
+        }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        uint32_t v = acc[f];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) s_sum[f][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t best = ~0ull;
+        int bf = 0;
+        for (int f = 0; f < 5; ++f) {
+            const uint64_t t = (uint64_t)s_sum[f][0] + s_sum[f][1] + s_sum[f][2] + s_sum[f][3];
+            if (t < best) { best = t; bf = f; }
+        }
+        s_best = bf;
+    }
+    __syncthreads();
+    const int f = s_best;
+    uint8_t* out = ws.filt + (size_t)r * n;
+    uint64_t sa = 0, sb = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint8_t b = i == 0 ? (uint8_t)f : filt_byte(f, cur, prev, i - 1, ch);
+        s_row[i] = b;
+        out[i] = b;
+        sa += b;
+        sb += (uint64_t)(n - i) * b;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sa += __shfl_down(sa, o, 64);
+        sb += __shfl_down(sb, o, 64);
+    }
+    if (lane == 0) {
+        s_ad[0][wave] = sa;
+        s_ad[1][wave] = sb;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t bits = sweep_row(s_row, n, [](uint32_t, int, int) {});
+        if (lane == 0) {
+            ws.row_bits[r] = bits;
+            ws.row_s[r] = s_ad[0][0] + s_ad[0][1] + s_ad[0][2] + s_ad[0][3];
+            ws.row_t[r] = s_ad[1][0] + s_ad[1][1] + s_ad[1][2] + s_ad[1][3];
+        }
+    }
+}
+
+// One workgroup: row bit offsets, the deflate end, Adler-32, and the bytes
+// before the deflate data (signature, IHDR, IDAT type, zlib header, block
+// header). The IDAT length is written by the tail kernel.
+__global__ void __launch_bounds__(1024) k_png_scan(int height, int stride, PngWs ws, uint8_t* png, uint32_t ihdr_crc,
+                                                   int width, int ch) {
+    __shared__ uint64_t s_part[1024];
+    __shared__ uint64_t s_a[1024], s_b[1024];
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int per = (height + nt - 1) / nt;
+    const int r0 = min(height, t * per), r1 = min(height, r0 + per);
+    const uint64_t n = (uint64_t)stride + 1, N = n * (uint64_t)height;
+    uint64_t sum = 0, A = 0, B = 0;
+    for (int r = r0; r < r1; ++r) {
+        sum += ws.row_bits[r];
+        // Adler-32 of the filtered rows: A = 1 + sum d, B = N + sum_i (N - i + 1) d_i
+        const uint64_t o = (uint64_t)r * n;
+        const uint64_t S = ws.row_s[r] % kAdlerMod;
+        A = (A + S) % kAdlerMod;
+        B = (B + ((N - o - n) % kAdlerMod) * S + ws.row_t[r] % kAdlerMod) % kAdlerMod;
+    }
+    s_part[t] = sum;
+    s_a[t] = A;
+    s_b[t] = B;
+    __syncthreads();
+    // inclusive scan (Hillis-Steele) of the bit counts; tree sums of A and B
+    for (int o = 1; o < nt; o <<= 1) {
+        const uint64_t y = t >= o ? s_part[t - o] : 0;
+        uint64_t ya = 0, yb = 0;
+        const bool red = (t & (2 * o - 1)) == 0 && t + o < nt;
+        if (red) { ya = s_a[t + o]; yb = s_b[t + o]; }
+        __syncthreads();
+        s_part[t] += y;
+        if (red) {
+            s_a[t] = (s_a[t] + ya) % kAdlerMod;
+            s_b[t] = (s_b[t] + yb) % kAdlerMod;
+        }
+        __syncthreads();
+    }
+    const uint64_t start = kDeflateStartBit + 3;   // after BFINAL + BTYPE
+    uint64_t off = start + (t > 0 ? s_part[t - 1] : 0);
+    for (int r = r0; r < r1; ++r) {
+        ws.row_off[r] = off;
+        off += ws.row_bits[r];
+    }
+    if (t == 0) {
+        ws.totals[0] = start + s_part[nt - 1];   // end of the row bits (the end-of-block code follows)
+        const uint64_t Af = (1 + s_a[0]) % kAdlerMod, Bf = (N % kAdlerMod + s_b[0]) % kAdlerMod;
+        ws.totals[3] = (Bf << 16) | Af;
+        // signature
+        const uint8_t sig[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
+        for (int i = 0; i < 8; ++i) png[i] = sig[i];
+        // IHDR: length 13, type, width, height, depth 8, colour type 2 (RGB) / 6 (RGBA), 0, 0, 0, CRC
+        uint8_t* h = png + 8;
+        h[0] = 0; h[1] = 0; h[2] = 0; h[3] = 13;
+        h[4] = 'I'; h[5] = 'H'; h[6] = 'D'; h[7] = 'R';
+        h[8] = (uint8_t)(width >> 24); h[9] = (uint8_t)(width >> 16); h[10] = (uint8_t)(width >> 8); h[11] = (uint8_t)width;
+        h[12] = (uint8_t)(height >> 24); h[13] = (uint8_t)(height >> 16); h[14] = (uint8_t)(height >> 8);
+        h[15] = (uint8_t)height;
+        h[16] = 8; h[17] = ch == 3 ? 2 : 6; h[18] = 0; h[19] = 0; h[20] = 0;
+        h[21] = (uint8_t)(ihdr_crc >> 24); h[22] = (uint8_t)(ihdr_crc >> 16); h[23] = (uint8_t)(ihdr_crc >> 8);
+        h[24] = (uint8_t)ihdr_crc;
+        // IDAT type; the length (bytes 33..36) is known after the tail
+        png[37] = 'I'; png[38] = 'D'; png[39] = 'A'; png[40] = 'T';
+        // zlib header: deflate, 32K window, no dictionary, fastest (0x78 0x01)
+        png[41] = 0x78; png[42] = 0x01;
+        // block header at bit 344: BFINAL = 1, BTYPE = 01 (fixed Huffman)
+        png[43] = 0x03;
+    }
+}
+
+// Emit: one wave per row stages the filtered row in LDS, builds the row's
+// bits there, then ORs them into the output at the row's bit offset (the
+// first and last words, shared with neighbours, through atomics).
+__global__ void __launch_bounds__(64) k_png_emit(int height, int stride, PngWs ws, uint8_t* png) {
+    extern __shared__ uint32_t s_mem[];
+    const int r = blockIdx.x;
+    if (r >= height) return;
+    const int n = stride + 1;
+    const uint32_t row_words = ((uint32_t)n + 3) / 4;
+    uint8_t* s_row = (uint8_t*)s_mem;
+    uint32_t* s_bits = s_mem + row_words;
+    const uint8_t* row = ws.filt + (size_t)r * n;
+    const uint32_t nbits = ws.row_bits[r];
+    const uint32_t nwords = (nbits + 31) / 32 + 1;
+    for (int i = threadIdx.x; i < n; i += 64) s_row[i] = row[i];
+    for (uint32_t i = threadIdx.x; i < nwords; i += 64) s_bits[i] = 0u;
+    __syncthreads();
+    auto put = [&](uint32_t pos, uint32_t bits, int len) {
+        // len <= 32 (a match: at most 8 + 5 + 5 bits)
+        const uint32_t w = pos >> 5, o = pos & 31;
+        atomicOr(&s_bits[w], bits << o);
+        if (o + len > 32) atomicOr(&s_bits[w + 1], bits >> (32 - o));
+    };
+    sweep_row(s_row, n, [&](uint32_t pos, int v, int len) {
+        uint32_t b;
+        int l;
+        lit_code((uint32_t)v, b, l);
+        put(pos, b, l);
+        pos += l;
+        int m = len - 1;
+        while (m >= 258) {
+            match_code(258, b, l);
+            put(pos, b, l);
+            pos += l;
+            m -= 258;
+        }
+        if (m >= 3) {
+            match_code(m, b, l);
+            put(pos, b, l);
+        } else {
+            for (int k = 0; k < m; ++k) {
+                lit_code((uint32_t)v, b, l);
+                put(pos, b, l);
+                pos += l;
+            }
+        }
+    });
+    __syncthreads();
+    // OR the row's bits into the output at its offset
+    const uint64_t off = ws.row_off[r];
+    uint32_t* out = (uint32_t*)png;
+    const uint64_t w_first = off >> 5, w_last = (off + nbits - 1) >> 5;
+    const uint32_t sh = (uint32_t)(off & 31);
+    for (uint64_t w = w_first + threadIdx.x; w <= w_last; w += 64) {
+        const uint64_t k = w - w_first;   // output word k of this row
+        // bits [32k - sh, 32k - sh + 32) of the row stream
+        uint32_t val;
+        if (sh == 0) val = s_bits[k];
+        else {
+            const uint32_t lo = k > 0 ? s_bits[k - 1] >> (32 - sh) : 0u;
+            const uint32_t hi = k < nwords ? s_bits[k] << sh : 0u;
+            val = lo | hi;
+        }
+        if (w == w_first || w == w_last) atomicOr(&out[w], val);
+        else out[w] = val;
+    }
+}
+
+// One thread: end-of-block code, padding, Adler-32, IDAT length.
+__global__ void k_png_tail(PngWs ws, uint8_t* png) {
+    const uint64_t end = ws.totals[0];   // bit after the last row
+    // end-of-block: symbol 256, 7 zero bits; the deflate stream ends at end + 7
+    const uint64_t dbytes_end = (end + 7 + 7) / 8;   // byte after the deflate data
+    const uint32_t adler = (uint32_t)ws.totals[3];
+    uint8_t* a = png + dbytes_end;
+    a[0] = (uint8_t)(adler >> 24); a[1] = (uint8_t)(adler >> 16); a[2] = (uint8_t)(adler >> 8); a[3] = (uint8_t)adler;
+    const uint64_t zlib_len = dbytes_end + 4 - 41;
+    png[33] = (uint8_t)(zlib_len >> 24); png[34] = (uint8_t)(zlib_len >> 16);
+    png[35] = (uint8_t)(zlib_len >> 8); png[36] = (uint8_t)zlib_len;
+    ws.totals[1] = zlib_len;
+}
+
+// Raw CRC (init 0, no final xor) of 4096-byte blocks of the IDAT type + data.
+__global__ void __launch_bounds__(256) k_crc_blocks(const uint8_t* png, PngWs ws, uint64_t max_blocks) {
+    __shared__ uint32_t table[256];
+    __shared__ uint32_t part[256];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+        uint32_t c = (uint32_t)i;
+        for (int k = 0; k < 8; ++k) c = (c & 1u) ? (c >> 1) ^ kCrcPoly : c >> 1;
+        table[i] = c;
+    }
+    __syncthreads();
+    const uint64_t len = 4 + ws.totals[1];   // "IDAT" + zlib data
+    const uint64_t blk = blockIdx.x;
+    if (blk >= max_blocks || blk * kCrcBlock >= len) return;
+    const uint8_t* base = png + 37 + blk * kCrcBlock;
+    const uint64_t bl = min((uint64_t)kCrcBlock, len - blk * kCrcBlock);
+    // each thread: 16 bytes
+    constexpr int kPer = kCrcBlock / 256;
+    const uint64_t s = (uint64_t)threadIdx.x * kPer;
+    uint32_t c = 0;
+    int cnt = 0;
+    for (int k = 0; k < kPer; ++k) {
+        if (s + k < bl) {
+            c = crc_update_raw(c, base[s + k], table);
+            ++cnt;
+        }
+    }
+    part[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // combine in order: crc(A||B) = crc(A) * x^(8|B|) + crc(B)
+        uint32_t acc = 0;
+        const uint32_t step = x_pow_8n(kPer);
+        for (int t = 0; t < 256; ++t) {
+            const uint64_t ts = (uint64_t)t * kPer;
+            if (ts >= bl) break;
+            const uint64_t tl = min((uint64_t)kPer, bl - ts);
+            acc = gf2_mulmod(acc, tl == kPer ? step : x_pow_8n(tl)) ^ part[t];
+        }
+        ws.crc_part[blk] = acc;
+    }
+}
+
+// One thread: combine block CRCs, conditioning, CRC bytes, IEND, length.
+__global__ void k_crc_final(uint8_t* png, PngWs ws, uint64_t* png_len) {
+    const uint64_t len = 4 + ws.totals[1];
+    const uint64_t nblk = (len + kCrcBlock - 1) / kCrcBlock;
+    const uint32_t step = x_pow_8n(kCrcBlock);
+    uint32_t acc = 0;
+    for (uint64_t b = 0; b < nblk; ++b) {
+        const uint64_t bl = min((uint64_t)kCrcBlock, len - b * kCrcBlock);
+        acc = gf2_mulmod(acc, bl == (uint64_t)kCrcBlock ? step : x_pow_8n(bl)) ^ ws.crc_part[b];
+    }
+    // standard CRC-32: init 0xFFFFFFFF (its effect after len bytes) and final xor
+    const uint32_t crc = acc ^ gf2_mulmod(0xFFFFFFFFu, x_pow_8n(len)) ^ 0xFFFFFFFFu;
+    uint8_t* c = png + 37 + len;
+    c[0] = (uint8_t)(crc >> 24); c[1] = (uint8_t)(crc >> 16); c[2] = (uint8_t)(crc >> 8); c[3] = (uint8_t)crc;
+    const uint8_t iend[12] = {0, 0, 0, 0, 'I', 'E', 'N', 'D', 0xAE, 0x42, 0x60, 0x82};
+    for (int i = 0; i < 12; ++i) c[4 + i] = iend[i];
+    const uint64_t total = 37 + len + 4 + 12;
+    ws.totals[2] = total;
+    *png_len = total;
+}
+
+__global__ void __launch_bounds__(256) k_base64(const uint8_t* __restrict__ in, const uint64_t* in_len, uint64_t in_cap,
+                                                uint8_t* __restrict__ out, uint64_t* out_len) {
+    const char tab[] = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+    uint64_t n = *in_len;
+    if (n > in_cap) n = in_cap;
+    const uint64_t groups = (n + 2) / 3;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = 3 * g;
+        const uint32_t b0 = in[i];
+        const uint32_t b1 = i + 1 < n ? in[i + 1] : 0u;
+        const uint32_t b2 = i + 2 < n ? in[i + 2] : 0u;
+        const uint32_t v = (b0 << 16) | (b1 << 8) | b2;
+        uint8_t* o = out + 4 * g;
+        o[0] = (uint8_t)tab[(v >> 18) & 63];
+        o[1] = (uint8_t)tab[(v >> 12) & 63];
+        o[2] = i + 1 < n ? (uint8_t)tab[(v >> 6) & 63] : (uint8_t)'=';
+        o[3] = i + 2 < n ? (uint8_t)tab[v & 63] : (uint8_t)'=';
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out_len = 4 * groups;
+}
+
+// host-side CRC of the IHDR chunk (type + 13 data bytes)
+uint32_t host_crc32(const uint8_t* p, size_t n) {
+    uint32_t c = 0xFFFFFFFFu;
+    for (size_t i = 0; i < n; ++i) {
+        c ^= p[i];
+        for (int k = 0; k < 8; ++k) c = (c & 1u) ? (c >> 1) ^ kCrcPoly : c >> 1;
+    }
+    return c ^ 0xFFFFFFFFu;
+}
+
+struct WsLayout {
+    size_t filt, row_bits, row_s, row_t, row_off, totals, crc_part, bytes;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+uint64_t png_bound(int w, int h, int ch) {
+    const uint64_t n = (uint64_t)w * ch + 1;
+    const uint64_t bits = 3 + (uint64_t)h * n * 9 + 7;
+    return kHeadBytes + 2 + (bits + 7) / 8 + 4 + 4 + 12 + 8;   // + slack for the last shared word
+}
+
+WsLayout ws_layout(int w, int h, int ch, uint64_t bound) {
+    WsLayout L;
+    size_t o = 0;
+    L.filt = o; o = align256(o + (size_t)h * ((size_t)w * ch + 1));
+    L.row_bits = o; o = align256(o + (size_t)h * 4);
+    L.row_s = o; o = align256(o + (size_t)h * 8);
+    L.row_t = o; o = align256(o + (size_t)h * 8);
+    L.row_off = o; o = align256(o + (size_t)h * 8);
+    L.totals = o; o = align256(o + 8 * 8);
+    L.crc_part = o; o = align256(o + (size_t)((bound + kCrcBlock - 1) / kCrcBlock + 1) * 4);
+    L.bytes = o;
+    return L;
+}
+
+bool bad_dims(int w, int h, int ch) {
+    return w <= 0 || h <= 0 || (ch != 3 && ch != 4) || (int64_t)w * ch > kMaxRowBytes || h > 65536;
+}
+
+}  // namespace
+
+extern "C" uint64_t mk_png_max_bytes(int width, int height, int channels) {
+    if (bad_dims(width, height, channels)) return 0;
+    return png_bound(width, height, channels);
+}
+
+extern "C" uint64_t mk_png_workspace_bytes(int width, int height, int channels) {
+    if (bad_dims(width, height, channels)) return 0;
+    return ws_layout(width, height, channels, png_bound(width, height, channels)).bytes;
+}
+
+extern "C" mk_status mk_png_encode_async(const uint8_t* dev_pixels, int width, int height, int channels,
+                                         void* dev_workspace, uint64_t workspace_bytes, uint8_t* dev_png,
+                                         uint64_t png_capacity, uint64_t* dev_png_len, void* stream) {
+    if (bad_dims(width, height, channels))
+        return mk_fail(nullptr, MK_ERR_ARG, "mk_png_encode_async: bad image %dx%dx%d (channels 3 or 4, row <= %d bytes)",
+                       width, height, channels, kMaxRowBytes);
+    if (!dev_pixels || !dev_workspace || !dev_png || !dev_png_len)
+        return mk_fail(nullptr, MK_ERR_ARG, "mk_png_encode_async: null buffer");
+    const uint64_t bound = png_bound(width, height, channels);
+    if (png_capacity < bound)
+        return mk_fail(nullptr, MK_ERR_ARG, "mk_png_encode_async: capacity %llu below the bound %llu",
+                       (unsigned long long)png_capacity, (unsigned long long)bound);
+    if (((uintptr_t)dev_png & 3) != 0) return mk_fail(nullptr, MK_ERR_ARG, "mk_png_encode_async: output not 4-byte aligned");
+    const WsLayout L = ws_layout(width, height, channels, bound);
+    if (workspace_bytes < L.bytes || ((uintptr_t)dev_workspace & 255) != 0)
+        return mk_fail(nullptr, MK_ERR_ARG, "mk_png_encode_async: workspace needs %zu bytes, 256-byte aligned", L.bytes);
+    uint8_t* w = (uint8_t*)dev_workspace;
+    PngWs ws;
+    ws.filt = w + L.filt;
+    ws.row_bits = (uint32_t*)(w + L.row_bits);
+    ws.row_s = (uint64_t*)(w + L.row_s);
+    ws.row_t = (uint64_t*)(w + L.row_t);
+    ws.row_off = (uint64_t*)(w + L.row_off);
+    ws.totals = (uint64_t*)(w + L.totals);
+    ws.crc_part = (uint32_t*)(w + L.crc_part);
+    // IHDR CRC on the host (the header depends only on the dims)
+    uint8_t ih[17] = {'I', 'H', 'D', 'R', (uint8_t)(width >> 24), (uint8_t)(width >> 16), (uint8_t)(width >> 8),
+                      (uint8_t)width, (uint8_t)(height >> 24), (uint8_t)(height >> 16), (uint8_t)(height >> 8),
+                      (uint8_t)height, 8, (uint8_t)(channels == 3 ? 2 : 6), 0, 0, 0};
+    const uint32_t ihdr_crc = host_crc32(ih, sizeof(ih));
+    const int stride = width * channels;
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t words = bound / 4;
+    hipLaunchKernelGGL(k_zero_words, dim3((uint32_t)std::min<uint64_t>((words + 255) / 256, 65535)), dim3(256), 0, s,
+                       (uint32_t*)dev_png, words);
+    const size_t row_lds = ((size_t)stride + 1 + 3) / 4 * 4;
+    hipLaunchKernelGGL(k_png_filter_count, dim3(height), dim3(256), row_lds, s, dev_pixels, width, height, channels,
+                       ws);
+    hipLaunchKernelGGL(k_png_scan, dim3(1), dim3(1024), 0, s, height, stride, ws, dev_png, ihdr_crc, width, channels);
+    const size_t emit_lds = row_lds + ((size_t)(stride + 1) * 9 / 32 + 2) * 4;
+    hipLaunchKernelGGL(k_png_emit, dim3(height), dim3(64), emit_lds, s, height, stride, ws, dev_png);
+    hipLaunchKernelGGL(k_png_tail, dim3(1), dim3(1), 0, s, ws, dev_png);
+    const uint64_t max_blocks = (bound + kCrcBlock - 1) / kCrcBlock;
+    hipLaunchKernelGGL(k_crc_blocks, dim3((uint32_t)max_blocks), dim3(256), 0, s, dev_png, ws, max_blocks);
+    hipLaunchKernelGGL(k_crc_final, dim3(1), dim3(1), 0, s, dev_png, ws, dev_png_len);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return mk_hip_fail(nullptr, e, "mk_png_encode_async launch");
+    return MK_OK;
+}
+
+extern "C" uint64_t mk_base64_max_bytes(uint64_t n) { return 4 * ((n + 2) / 3); }
+
+extern "C" mk_status mk_base64_encode_async(const uint8_t* dev_in, const uint64_t* dev_in_len, uint64_t in_capacity,
+                                            uint8_t* dev_out, uint64_t out_capacity, uint64_t* dev_out_len,
+                                            void* stream) {
+    if (!dev_in || !dev_in_len || !dev_out || !dev_out_len)
+        return mk_fail(nullptr, MK_ERR_ARG, "mk_base64_encode_async: null buffer");
+    if (out_capacity < mk_base64_max_bytes(in_capacity))
+        return mk_fail(nullptr, MK_ERR_ARG, "mk_base64_encode_async: output capacity below 4 * ceil(in_capacity / 3)");
+    const uint64_t groups = (in_capacity + 2) / 3;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((groups + 255) / 256 + 1, 65535);
+    hipLaunchKernelGGL(k_base64, dim3(blocks), dim3(256), 0, (hipStream_t)stream, dev_in, dev_in_len, in_capacity,
+                       dev_out, dev_out_len);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return mk_hip_fail(nullptr, e, "mk_base64_encode_async launch");
+    return MK_OK;
+}

@@ -79,8 +79,20 @@ class Request:
         w, h = self.size
         return ((w + TILE - 1) // TILE) * ((h + TILE - 1) // TILE)
 
-    def arena_bytes(self) -> int:
-        return _align(self.out_bytes()) + _align(self.geo_bytes())
+    def encode_dims(self) -> tuple:
+        """(width, height, channels) of the PNG the reference makes of this
+        result: the RGB frame, or the int16 image read as RGBA of width W/2."""
+        w, h = self.size
+        return (w, h, 3) if self.kind == KIND_RENDER else (w // 2, h, 4)
+
+    def b64_bytes(self) -> int:
+        w, h, ch = self.encode_dims()
+        lib = _capi.load()
+        return int(lib.mk_base64_max_bytes(lib.mk_png_max_bytes(w, h, ch)))
+
+    def arena_bytes(self, encode: bool = False) -> int:
+        n = _align(self.out_bytes()) + _align(self.geo_bytes())
+        return n + (_align(self.b64_bytes()) if encode else 0)
 
 
 def _align(n: int) -> int:
@@ -91,24 +103,27 @@ def _render_key(p: RenderParams) -> bytes:
     return bytes(p.alpha_wwwl)
 
 
-def plan_batch(queue: list, max_items: int, max_tiles: int, arena_bytes: int):
+def plan_batch(queue: list, max_items: int, max_tiles: int, arena_bytes: int, encode: bool = False):
     """Admission rule. Returns (batch, rejected): batch is the longest prefix
     of ``queue`` that fits one slot (items, tiles, arena bytes) and whose render
     items agree per volume on step_scale and per (volume, render type) on the
     label alpha/window table (the batch kernels share those per-volume tables);
     rejected holds requests that could never fit a slot. A request marked
-    ``solo`` goes in a batch of its own. Neither list is removed from
-    ``queue``."""
+    ``solo`` goes in a batch of its own. With ``encode`` a request also
+    holds the bound of its base64 PNG text in the arena. Neither list is
+    removed from ``queue``."""
     batch, rejected = [], []
     tiles = nbytes = 0
     steps, tables = {}, {}
     for r in queue:
-        if r.tiles() > max_tiles or r.arena_bytes() > arena_bytes:
+        need = r.arena_bytes(encode)
+        unencodable = encode and ((r.kind != KIND_RENDER and r.size[0] % 2) or r.b64_bytes() == 0)
+        if r.tiles() > max_tiles or need > arena_bytes or unencodable:
             rejected.append(r)
             continue
         if batch and (r.solo or batch[0].solo):
             break
-        if len(batch) == max_items or tiles + r.tiles() > max_tiles or nbytes + r.arena_bytes() > arena_bytes:
+        if len(batch) == max_items or tiles + r.tiles() > max_tiles or nbytes + need > arena_bytes:
             break
         if r.kind == KIND_RENDER:
             p = r.render
@@ -120,7 +135,7 @@ def plan_batch(queue: list, max_items: int, max_tiles: int, arena_bytes: int):
             tables[(r.volume, p.render_type)] = _render_key(p)
         batch.append(r)
         tiles += r.tiles()
-        nbytes += r.arena_bytes()
+        nbytes += need
     return batch, rejected
 
 
@@ -132,6 +147,13 @@ class _Slot:
     event: object = None     # torch.cuda.Event after the copy back
     batch: list = field(default_factory=list)
     offsets: list = field(default_factory=list)
+    # encoding: 1 = waiting for the text lengths, 2 = waiting for the texts
+    phase: int = 0
+    b64_offsets: list = field(default_factory=list)
+    lens_dev: object = None
+    lens_host: object = None
+    png: object = None
+    workspace: object = None
 
 
 class RenderServer:
@@ -139,10 +161,16 @@ class RenderServer:
 
     hbm_budget: device bytes for request outputs and CPR geometry, split
     evenly over ``nslots`` launch slots; a batch never exceeds its slot's
-    share, and a request larger than a share is rejected."""
+    share, and a request larger than a share is rejected.
+
+    encode: results come back as the reference's wire format, base64 text of
+    a PNG (RGB frames; int16 images as RGBA of width W/2), built on the device
+    after the batch (mk_png_encode_async + mk_base64_encode_async); only the
+    texts are copied back, in two steps (their lengths, then exactly their
+    bytes)."""
 
     def __init__(self, contexts: list, hbm_budget: int = 256 << 20, nslots: int = 2, max_items: int = 64,
-                 max_tiles: int = 32768):
+                 max_tiles: int = 32768, encode: bool = False):
         import torch
 
         if nslots < 1:
@@ -160,6 +188,12 @@ class RenderServer:
                   stream=torch.cuda.Stream())
             for _ in range(nslots)
         ]
+        self.encode = encode
+        self.lib = _capi.load()
+        if encode:
+            for sl in self.slots:
+                sl.lens_dev = torch.zeros(2 * max_items, dtype=torch.int64, device="cuda")
+                sl.lens_host = torch.zeros(2 * max_items, dtype=torch.int64, pin_memory=True)
         self.queue: list = []
         self.done: list = []
         self.batches = 0
@@ -178,9 +212,57 @@ class RenderServer:
     def busy(self) -> bool:
         return bool(self.queue) or any(s.batch for s in self.slots)
 
+    def _grow(self, s: _Slot, name: str, nbytes: int) -> None:
+        t = getattr(s, name)
+        if t is None or t.numel() < nbytes:
+            setattr(s, name, self.torch.empty((nbytes + 255) // 256 * 256, dtype=self.torch.uint8, device="cuda"))
+
+    def _encode_batch(self, s: _Slot, batch: list, offs: list, b64_offs: list) -> None:
+        """PNG + base64 of every result on the slot's stream, then the text
+        lengths to pinned memory."""
+        lib = self.lib
+        dims = [r.encode_dims() for r in batch]
+        self._grow(s, "png", max(int(lib.mk_png_max_bytes(*d)) for d in dims))
+        self._grow(s, "workspace", max(int(lib.mk_png_workspace_bytes(*d)) for d in dims))
+        st = s.stream.cuda_stream
+        lens = s.lens_dev.data_ptr()
+        base = s.arena.data_ptr()
+        for i, (r, off, bo, d) in enumerate(zip(batch, offs, b64_offs, dims)):
+            cap = int(lib.mk_png_max_bytes(*d))
+            ws = int(lib.mk_png_workspace_bytes(*d))
+            st_ = lib.mk_png_encode_async(base + off, d[0], d[1], d[2], s.workspace.data_ptr(), ws, s.png.data_ptr(),
+                                          cap, lens + 16 * i, st)
+            if st_ == 0:
+                st_ = lib.mk_base64_encode_async(s.png.data_ptr(), lens + 16 * i, cap, base + bo,
+                                                 int(lib.mk_base64_max_bytes(cap)), lens + 16 * i + 8, st)
+            if st_ != 0:
+                raise _capi.MkError(st_, lib.mk_last_error(None).decode())
+        s.lens_host[:2 * len(batch)].copy_(s.lens_dev[:2 * len(batch)], non_blocking=True)
+
+    def _texts_back(self, s: _Slot) -> None:
+        """Phase 2 of an encoded batch: copy exactly the texts back."""
+        torch = self.torch
+        lens = s.lens_host.numpy()
+        with torch.cuda.stream(s.stream):
+            for i, bo in enumerate(s.b64_offsets):
+                n = int(lens[2 * i + 1])
+                s.host[bo:bo + n].copy_(s.arena[bo:bo + n], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(s.stream)
+        s.event, s.phase = ev, 2
+
     def _finish(self, s: _Slot) -> None:
         host = s.host.numpy()
         now = time.perf_counter()
+        if self.encode:
+            lens = s.lens_host.numpy()
+            for i, (r, bo) in enumerate(zip(s.batch, s.b64_offsets)):
+                n = int(lens[2 * i + 1])
+                r.result = host[bo:bo + n].tobytes().decode("ascii")
+                r.t_done = now
+                self.done.append(r)
+            s.batch, s.offsets, s.b64_offsets, s.event, s.phase = [], [], [], None, 0
+            return
         for r, off in zip(s.batch, s.offsets):
             w, h = r.size
             n = r.out_bytes()
@@ -226,6 +308,11 @@ class RenderServer:
                 it.cpr_points_offset, it.cpr_dirs_offset = cur, cur + nb
                 cur = _align(cur + 2 * nb)
             items.append(it)
+        b64_offs = []
+        if self.encode:
+            for r in batch:
+                b64_offs.append(cur)
+                cur = _align(cur + r.b64_bytes())
         now = time.perf_counter()
         for r in batch:
             r.t_admit = now
@@ -252,10 +339,14 @@ class RenderServer:
                     batch[0].t_done = time.perf_counter()
                     self.done.append(batch[0])
                 return
-            s.host[:out_end].copy_(s.arena[:out_end], non_blocking=True)
+            if self.encode:
+                self._encode_batch(s, batch, offs, b64_offs)
+            else:
+                s.host[:out_end].copy_(s.arena[:out_end], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(s.stream)
-        s.event, s.batch, s.offsets = ev, list(batch), offs
+        s.event, s.batch, s.offsets, s.b64_offsets = ev, list(batch), offs, b64_offs
+        s.phase = 1 if self.encode else 0
         self.batches += 1
         self.batch_sizes.append(len(batch))
 
@@ -263,14 +354,19 @@ class RenderServer:
         """Retires finished batches, then fills every free slot from the queue."""
         for s in self.slots:
             if s.batch and s.event.query():
-                self._finish(s)
+                if s.phase == 1:
+                    self._texts_back(s)
+                else:
+                    self._finish(s)
         for si, s in enumerate(self.slots):
             if s.batch or not self.queue:
                 continue
-            batch, rejected = plan_batch(self.queue, self.max_items, self.max_tiles, self.slot_bytes)
+            batch, rejected = plan_batch(self.queue, self.max_items, self.max_tiles, self.slot_bytes, self.encode)
             for r in rejected:
-                r.error = (f"request needs {r.arena_bytes()} arena bytes / {r.tiles()} tiles; a slot holds "
-                           f"{self.slot_bytes} bytes / {self.max_tiles} tiles")
+                r.error = (f"request needs {r.arena_bytes(self.encode)} arena bytes / {r.tiles()} tiles; a slot "
+                           f"holds {self.slot_bytes} bytes / {self.max_tiles} tiles"
+                           + ("; encoded images need an even plane width and rows of at most 24576 bytes"
+                              if self.encode else ""))
                 r.t_done = time.perf_counter()
                 self.done.append(r)
             gone = {id(r) for r in batch} | {id(r) for r in rejected}

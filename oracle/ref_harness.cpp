@@ -13,6 +13,8 @@
 //   Projection_Point2Plane (core/Methods.cpp:582-598), GetTransferPoint
 //   (core/Methods.cpp:667-679), PlaneInfo::GetNormDirection / GetLeftTopPoint
 //   (core/PlaneInfo.h:37-45).
+//   fpng_encode_image_to_memory (core/fpng/fpng.cpp): the reference's PNG
+//   encoder, as the CPU baseline of the device encoder.
 // Used only to write golden vectors (tests/golden/make_golden.py) and to
 // cross-check the restatements where the reference is present.
 #include <cstring>
@@ -27,6 +29,9 @@
 #include "Methods.h"
 #include "PlaneInfo.h"
 #include "TransferFunction.h"
+#include "fpng/fpng.h"
+
+#include <vector>
 
 using namespace MonkeyGL;
 
@@ -145,6 +150,22 @@ void ref_plane_info(const double* h0, const double* h1, const double* v0, const 
     norm[1] = n.y();
     norm[2] = n.z();
     out3(info.GetLeftTopPoint(P3(center)), left_top);
+}
+
+// fpng encode into out (capacity cap); returns the PNG size, 0 on failure or
+// when cap is too small.
+unsigned long long ref_fpng_encode(const unsigned char* img, int w, int h, int chans, unsigned char* out,
+                                   unsigned long long cap) {
+    static bool inited = false;
+    if (!inited) {
+        fpng::fpng_init();
+        inited = true;
+    }
+    std::vector<uint8_t> buf;
+    if (!fpng::fpng_encode_image_to_memory(img, (uint32_t)w, (uint32_t)h, (uint32_t)chans, buf)) return 0;
+    if (buf.size() > cap) return 0;
+    std::memcpy(out, buf.data(), buf.size());
+    return buf.size();
 }
 
 }  // extern "C"

@@ -151,3 +151,29 @@ def test_open_loop_poisson(served):
     for a, b in zip(reqs, seq):
         assert b.t_done >= b.t_arrival and a.t_done >= a.t_arrival
         assert np.array_equal(a.result, b.result), a.tag
+
+
+def test_encoded_mix_decodes_to_plain(served):
+    """encode=True: every result is base64 PNG text (RGB frames; int16 images
+    as RGBA of width W/2) that decodes to the plain entry point's image."""
+    import base64
+
+    from tests.png_util import read_png
+
+    scene, ctxs = served
+    reqs = _requests(scene, n=30, seed=23)
+    srv = RenderServer(ctxs, hbm_budget=64 << 20, nslots=2, max_items=8, max_tiles=4096, encode=True)
+    try:
+        srv.run(reqs)
+    finally:
+        srv.close()
+    for r in reqs:
+        assert r.error is None, r.error
+        assert isinstance(r.result, str)
+        pix, info = read_png(base64.b64decode(r.result, validate=True))
+        plain = render_one(ctxs[r.volume], r)
+        if r.kind == KIND_RENDER:
+            assert info["channels"] == 3 and np.array_equal(pix, plain), r.tag
+        else:
+            assert info["channels"] == 4 and info["width"] * 2 == r.size[0]
+            assert np.array_equal(pix.reshape(-1).view(np.int16).reshape(plain.shape), plain), r.tag
