@@ -445,10 +445,23 @@ __global__ void k_png_tail(PngWs ws, uint8_t* png) {
     ws.totals[1] = zlib_len;
 }
 
-// Raw CRC (init 0, no final xor) of 4096-byte blocks of the IDAT type + data.
+// XOR over the workgroup (waves of 64, at most 16 waves).
+__device__ inline uint32_t block_xor(uint32_t v) {
+    __shared__ uint32_t s_x[16];
+    for (int o = 32; o > 0; o >>= 1) v ^= __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) s_x[wave] = v;
+    __syncthreads();
+    uint32_t r = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r ^= s_x[w];
+    return r;
+}
+
+// Raw CRC (init 0, no final xor) of 4096-byte blocks of the IDAT type + data:
+// each thread's 16-byte CRC moved past the bytes that follow it in the block
+// (times x^(8 n) mod P), then XORed over the block.
 __global__ void __launch_bounds__(256) k_crc_blocks(const uint8_t* png, PngWs ws, uint64_t max_blocks) {
     __shared__ uint32_t table[256];
-    __shared__ uint32_t part[256];
     for (int i = threadIdx.x; i < 256; i += blockDim.x) {
         uint32_t c = (uint32_t)i;
         for (int k = 0; k < 8; ++k) c = (c & 1u) ? (c >> 1) ^ kCrcPoly : c >> 1;
@@ -460,43 +473,31 @@ __global__ void __launch_bounds__(256) k_crc_blocks(const uint8_t* png, PngWs ws
     if (blk >= max_blocks || blk * kCrcBlock >= len) return;
     const uint8_t* base = png + 37 + blk * kCrcBlock;
     const uint64_t bl = min((uint64_t)kCrcBlock, len - blk * kCrcBlock);
-    // each thread: 16 bytes
     constexpr int kPer = kCrcBlock / 256;
     const uint64_t s = (uint64_t)threadIdx.x * kPer;
-    uint32_t c = 0;
-    int cnt = 0;
-    for (int k = 0; k < kPer; ++k) {
-        if (s + k < bl) {
-            c = crc_update_raw(c, base[s + k], table);
-            ++cnt;
-        }
+    uint32_t v = 0;
+    if (s < bl) {
+        const uint64_t e = min(s + kPer, bl);
+        uint32_t c = 0;
+        for (uint64_t k = s; k < e; ++k) c = crc_update_raw(c, base[k], table);
+        v = gf2_mulmod(c, x_pow_8n(bl - e));
     }
-    part[threadIdx.x] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        // combine in order: crc(A||B) = crc(A) * x^(8|B|) + crc(B)
-        uint32_t acc = 0;
-        const uint32_t step = x_pow_8n(kPer);
-        for (int t = 0; t < 256; ++t) {
-            const uint64_t ts = (uint64_t)t * kPer;
-            if (ts >= bl) break;
-            const uint64_t tl = min((uint64_t)kPer, bl - ts);
-            acc = gf2_mulmod(acc, tl == kPer ? step : x_pow_8n(tl)) ^ part[t];
-        }
-        ws.crc_part[blk] = acc;
-    }
+    const uint32_t crc = block_xor(v);
+    if (threadIdx.x == 0) ws.crc_part[blk] = crc;
 }
 
-// One thread: combine block CRCs, conditioning, CRC bytes, IEND, length.
-__global__ void k_crc_final(uint8_t* png, PngWs ws, uint64_t* png_len) {
+// One workgroup: block CRCs moved past the bytes after them and XORed, the
+// standard conditioning, CRC bytes, IEND, length.
+__global__ void __launch_bounds__(1024) k_crc_final(uint8_t* png, PngWs ws, uint64_t* png_len) {
     const uint64_t len = 4 + ws.totals[1];
     const uint64_t nblk = (len + kCrcBlock - 1) / kCrcBlock;
-    const uint32_t step = x_pow_8n(kCrcBlock);
-    uint32_t acc = 0;
-    for (uint64_t b = 0; b < nblk; ++b) {
-        const uint64_t bl = min((uint64_t)kCrcBlock, len - b * kCrcBlock);
-        acc = gf2_mulmod(acc, bl == (uint64_t)kCrcBlock ? step : x_pow_8n(bl)) ^ ws.crc_part[b];
+    uint32_t v = 0;
+    for (uint64_t b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const uint64_t e = min((b + 1) * (uint64_t)kCrcBlock, len);
+        v ^= gf2_mulmod(ws.crc_part[b], x_pow_8n(len - e));
     }
+    const uint32_t acc = block_xor(v);
+    if (threadIdx.x != 0) return;
     // standard CRC-32: init 0xFFFFFFFF (its effect after len bytes) and final xor
     const uint32_t crc = acc ^ gf2_mulmod(0xFFFFFFFFu, x_pow_8n(len)) ^ 0xFFFFFFFFu;
     uint8_t* c = png + 37 + len;
@@ -626,7 +627,7 @@ extern "C" mk_status mk_png_encode_async(const uint8_t* dev_pixels, int width, i
     hipLaunchKernelGGL(k_png_tail, dim3(1), dim3(1), 0, s, ws, dev_png);
     const uint64_t max_blocks = (bound + kCrcBlock - 1) / kCrcBlock;
     hipLaunchKernelGGL(k_crc_blocks, dim3((uint32_t)max_blocks), dim3(256), 0, s, dev_png, ws, max_blocks);
-    hipLaunchKernelGGL(k_crc_final, dim3(1), dim3(1), 0, s, dev_png, ws, dev_png_len);
+    hipLaunchKernelGGL(k_crc_final, dim3(1), dim3(1024), 0, s, dev_png, ws, dev_png_len);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return mk_hip_fail(nullptr, e, "mk_png_encode_async launch");
     return MK_OK;

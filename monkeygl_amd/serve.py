@@ -58,6 +58,7 @@ class Request:
     result: np.ndarray | None = None
     error: str | None = None
     t_admit: float = 0.0
+    t_rendered: float = 0.0   # encode mode: when the host saw the texts' lengths
     t_done: float = 0.0
 
     @property
@@ -242,6 +243,9 @@ class RenderServer:
     def _texts_back(self, s: _Slot) -> None:
         """Phase 2 of an encoded batch: copy exactly the texts back."""
         torch = self.torch
+        now = time.perf_counter()
+        for r in s.batch:
+            r.t_rendered = now
         lens = s.lens_host.numpy()
         with torch.cuda.stream(s.stream):
             for i, bo in enumerate(s.b64_offsets):
