@@ -114,16 +114,27 @@ __host__ __device__ inline uint32_t gf2_mulmod(uint32_t a, uint32_t b) {
     return p;
 }
 
-// x^(8 n) mod P by squaring from x^8.
-__host__ __device__ inline uint32_t x_pow_8n(uint64_t n) {
-    uint32_t result = 0x80000000u;   // 1
-    uint32_t sq = 0x00800000u;       // x^8
-    while (n) {
-        if (n & 1) result = gf2_mulmod(result, sq);
+// x^(8 * 2^k) mod P for k < 48, built once on the host: x^(8 n) is then a
+// product over the set bits of n.
+struct PowTable {
+    uint32_t p[48];
+};
+
+PowTable make_pow_table() {
+    PowTable t;
+    uint32_t sq = 0x00800000u;   // x^8
+    for (int k = 0; k < 48; ++k) {
+        t.p[k] = sq;
         sq = gf2_mulmod(sq, sq);
-        n >>= 1;
     }
-    return result;
+    return t;
+}
+
+__device__ inline uint32_t x_pow_8n_tab(const PowTable& t, uint64_t n) {
+    uint32_t r = 0x80000000u;
+    for (int k = 0; n; ++k, n >>= 1)
+        if (n & 1) r = gf2_mulmod(r, t.p[k]);
+    return r;
 }
 
 __device__ inline uint32_t crc_update_raw(uint32_t c, uint8_t byte, const uint32_t* table) {
@@ -457,10 +468,10 @@ __device__ inline uint32_t block_xor(uint32_t v) {
     return r;
 }
 
-// Raw CRC (init 0, no final xor) of 4096-byte blocks of the IDAT type + data:
+// CRC (no final xor) of 4096-byte blocks of the IDAT type + data:
 // each thread's 16-byte CRC moved past the bytes that follow it in the block
 // (times x^(8 n) mod P), then XORed over the block.
-__global__ void __launch_bounds__(256) k_crc_blocks(const uint8_t* png, PngWs ws, uint64_t max_blocks) {
+__global__ void __launch_bounds__(256) k_crc_blocks(const uint8_t* png, PngWs ws, uint64_t max_blocks, PowTable pw) {
     __shared__ uint32_t table[256];
     for (int i = threadIdx.x; i < 256; i += blockDim.x) {
         uint32_t c = (uint32_t)i;
@@ -478,9 +489,11 @@ __global__ void __launch_bounds__(256) k_crc_blocks(const uint8_t* png, PngWs ws
     uint32_t v = 0;
     if (s < bl) {
         const uint64_t e = min(s + kPer, bl);
-        uint32_t c = 0;
+        // the standard initial value 0xFFFFFFFF enters with the first byte;
+        // by linearity it then travels to the end like any other term
+        uint32_t c = (blk == 0 && threadIdx.x == 0) ? 0xFFFFFFFFu : 0u;
         for (uint64_t k = s; k < e; ++k) c = crc_update_raw(c, base[k], table);
-        v = gf2_mulmod(c, x_pow_8n(bl - e));
+        v = gf2_mulmod(c, x_pow_8n_tab(pw, bl - e));
     }
     const uint32_t crc = block_xor(v);
     if (threadIdx.x == 0) ws.crc_part[blk] = crc;
@@ -488,18 +501,18 @@ __global__ void __launch_bounds__(256) k_crc_blocks(const uint8_t* png, PngWs ws
 
 // One workgroup: block CRCs moved past the bytes after them and XORed, the
 // standard conditioning, CRC bytes, IEND, length.
-__global__ void __launch_bounds__(1024) k_crc_final(uint8_t* png, PngWs ws, uint64_t* png_len) {
+__global__ void __launch_bounds__(1024) k_crc_final(uint8_t* png, PngWs ws, uint64_t* png_len, PowTable pw) {
     const uint64_t len = 4 + ws.totals[1];
     const uint64_t nblk = (len + kCrcBlock - 1) / kCrcBlock;
     uint32_t v = 0;
     for (uint64_t b = threadIdx.x; b < nblk; b += blockDim.x) {
         const uint64_t e = min((b + 1) * (uint64_t)kCrcBlock, len);
-        v ^= gf2_mulmod(ws.crc_part[b], x_pow_8n(len - e));
+        v ^= gf2_mulmod(ws.crc_part[b], x_pow_8n_tab(pw, len - e));
     }
     const uint32_t acc = block_xor(v);
     if (threadIdx.x != 0) return;
-    // standard CRC-32: init 0xFFFFFFFF (its effect after len bytes) and final xor
-    const uint32_t crc = acc ^ gf2_mulmod(0xFFFFFFFFu, x_pow_8n(len)) ^ 0xFFFFFFFFu;
+    // standard CRC-32: the initial value went in with the first block; final xor
+    const uint32_t crc = acc ^ 0xFFFFFFFFu;
     uint8_t* c = png + 37 + len;
     c[0] = (uint8_t)(crc >> 24); c[1] = (uint8_t)(crc >> 16); c[2] = (uint8_t)(crc >> 8); c[3] = (uint8_t)crc;
     const uint8_t iend[12] = {0, 0, 0, 0, 'I', 'E', 'N', 'D', 0xAE, 0x42, 0x60, 0x82};
@@ -626,8 +639,9 @@ extern "C" mk_status mk_png_encode_async(const uint8_t* dev_pixels, int width, i
     hipLaunchKernelGGL(k_png_emit, dim3(height), dim3(64), emit_lds, s, height, stride, ws, dev_png);
     hipLaunchKernelGGL(k_png_tail, dim3(1), dim3(1), 0, s, ws, dev_png);
     const uint64_t max_blocks = (bound + kCrcBlock - 1) / kCrcBlock;
-    hipLaunchKernelGGL(k_crc_blocks, dim3((uint32_t)max_blocks), dim3(256), 0, s, dev_png, ws, max_blocks);
-    hipLaunchKernelGGL(k_crc_final, dim3(1), dim3(1024), 0, s, dev_png, ws, dev_png_len);
+    static const PowTable pw = make_pow_table();
+    hipLaunchKernelGGL(k_crc_blocks, dim3((uint32_t)max_blocks), dim3(256), 0, s, dev_png, ws, max_blocks, pw);
+    hipLaunchKernelGGL(k_crc_final, dim3(1), dim3(1024), 0, s, dev_png, ws, dev_png_len, pw);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return mk_hip_fail(nullptr, e, "mk_png_encode_async launch");
     return MK_OK;
