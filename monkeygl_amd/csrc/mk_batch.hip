@@ -16,9 +16,10 @@
 // Launch: each slot owns a pinned host descriptor block, its device copy and
 // a hipGraph built once at creation: [copy the descriptor block] -> [variant 0
 // kernel] -> ... -> [variant N-1 kernel]. Submitting fills the pinned block,
-// sets the copy size and each kernel's grid (its variant's tile count; one
-// idle workgroup when the variant is absent) in the instantiated graph, and
-// replays it on the caller's stream. A slot is reused only after its
+// sets the copy size and each present variant's grid (its tile count) in the
+// instantiated graph, disables the nodes of absent variants (a disabled node
+// passes its dependency through and launches nothing), and replays the graph
+// on the caller's stream. A slot is reused only after its
 // completion event has fired, so the descriptors of a batch in flight are
 // never overwritten. Several slots in flight on separate streams overlap on
 // the device.
@@ -107,6 +108,7 @@ struct mk_batcher {
         hipGraphExec_t exec = nullptr;
         hipGraphNode_t copy = nullptr;
         hipGraphNode_t kern[V_COUNT] = {};
+        bool enabled[V_COUNT];   // kernel node state in the instantiated graph
         hipEvent_t done = nullptr;
         bool busy = false;
     };
@@ -176,6 +178,7 @@ static mk_status build_slot(mk_batcher* b, mk_batcher::Slot& s) {
         prev = s.kern[v];
     }
     B_HIP(b, hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
+    for (int v = 0; v < V_COUNT; ++v) s.enabled[v] = true;
     return MK_OK;
 }
 
@@ -479,8 +482,15 @@ extern "C" mk_status mk_batch_submit(mk_batcher* b, int slot, const mk_batch_ite
 
     B_HIP(b, hipGraphExecMemcpyNodeSetParams1D(s.exec, s.copy, s.d, s.h, b->lay.bytes(n), hipMemcpyHostToDevice));
     for (int v = 0; v < V_COUNT; ++v) {
-        hipKernelNodeParams kp = kernel_params(s, v, H->var[v].tiles);
-        B_HIP(b, hipGraphExecKernelNodeSetParams(s.exec, s.kern[v], &kp));
+        const bool used = H->var[v].tiles > 0;
+        if (used) {
+            hipKernelNodeParams kp = kernel_params(s, v, H->var[v].tiles);
+            B_HIP(b, hipGraphExecKernelNodeSetParams(s.exec, s.kern[v], &kp));
+        }
+        if (used != s.enabled[v]) {
+            B_HIP(b, hipGraphNodeSetEnabled(s.exec, s.kern[v], used ? 1u : 0u));
+            s.enabled[v] = used;
+        }
     }
     hipStream_t st = (hipStream_t)stream;
     for (mk_context* c : b->ctxs) {

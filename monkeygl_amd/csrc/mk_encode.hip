@@ -18,12 +18,12 @@
 // CRC-32 from per-block raw CRCs (CRC is linear: a block's contribution is
 // moved past the following bytes by multiplying with x^(8n) mod P).
 //
-// Launch sequence (one stream, no host round trip): zero the output ->
-// filter -> count -> scan (header bytes, row offsets) -> emit rows -> tail
-// (end-of-block code, Adler-32) -> block CRCs -> combine (CRC, IEND, length).
-// The encoded length is left in device memory, so base64 can follow on the
-// same stream. Rows are limited to 24576 bytes: the count and emit kernels
-// hold a filtered row and its bits in LDS.
+// Launch sequence (one stream, no host round trip): filter + count (one
+// workgroup per row) -> scan (row offsets, every byte around the deflate
+// data) -> emit rows -> CRC (workgroups of 4 KiB; the last one to finish
+// writes the CRC, IEND and the length). The encoded length is left in device
+// memory, so base64 can follow on the same stream. Rows are limited to 24576
+// bytes: the count and emit kernels hold a filtered row and its bits in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -153,11 +153,6 @@ struct PngWs {
     uint32_t* crc_part; // per CRC block
 };
 
-__global__ void __launch_bounds__(256) k_zero_words(uint32_t* p, uint64_t n) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = 0u;
-}
-
 // PNG filter per row; one workgroup per row.
 __device__ inline int paeth(int a, int b, int c) {
     const int p = a + b - c;
@@ -179,67 +174,104 @@ __device__ inline uint8_t filt_byte(int f, const uint8_t* cur, const uint8_t* pr
     }
 }
 
-// Runs of one filtered row (in LDS), swept by one wave in windows of 64
-// bytes. A run starts where the byte differs from the previous one (and at
-// the row start). In each window: the run carried from earlier windows closes
-// at the window's first start; runs starting in the window close at the next
-// start in the window; the window's last start opens the carried run. The
-// run open at the row's end closes there. Counting and emitting use this same
-// parse, so a row's bits match its count. fn(lane_run_start, value, length,
-// carried) is called for every closed run on the lane that owns it (lane 0
-// for carried runs); it returns the run's bits for the offset bookkeeping.
-template <typename Fn>
-__device__ inline uint32_t sweep_row(const uint8_t* row, int n, Fn fn) {
-    const int lane = threadIdx.x & 63;
-    uint32_t base = 0;
-    int open_start = 0, open_val = row[0];
-    for (int w0 = 0; w0 < n; w0 += 64) {
-        const int i = w0 + lane;
-        const bool in = i < n;
-        const int v = in ? (int)row[i] : -1;
-        const int pv = i == 0 ? -2 : (in ? (int)row[i - 1] : -1);
-        const bool start = in && v != pv;
-        const uint64_t smask = __ballot(start);
-        uint32_t carried = 0;
-        if (smask) {
-            const int first = w0 + __ffsll((long long)smask) - 1;
-            if (!(w0 == 0 && first == 0)) {
-                const int len = first - open_start;
-                carried = run_bits((uint32_t)open_val, len);
-                if (lane == 0) fn(base, open_val, len);
-            }
-        }
-        uint32_t my_bits = 0;
-        int my_len = 0;
-        if (start) {
-            const uint64_t later = lane == 63 ? 0ull : (smask >> (lane + 1)) << (lane + 1);
-            if (later) {
-                my_len = w0 + __ffsll((long long)later) - 1 - i;
-                my_bits = run_bits((uint32_t)v, my_len);
-            }
-        }
-        uint32_t incl = my_bits;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += y;
-        }
-        const uint32_t total = __shfl(incl, 63, 64);
-        if (my_len) fn(base + carried + incl - my_bits, v, my_len);
-        base += carried + total;
-        if (smask) {
-            const int last = w0 + 63 - __clzll((long long)smask);
-            open_start = last;
-            open_val = row[last];
-        }
+// Workgroup-wide helpers for the row kernels (kRowThreads threads, 4 waves).
+constexpr int kRowThreads = 256;
+constexpr int kRowWaves = kRowThreads / 64;
+
+// Exclusive prefix sum over the workgroup; total gets the sum of all.
+__device__ inline uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
     }
-    const uint32_t tail = run_bits((uint32_t)open_val, n - open_start);
-    if (lane == 0) fn(base, open_val, n - open_start);
-    return base + tail;
+    if (lane == 63) s_w[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+    for (int w = 0; w < kRowWaves; ++w) {
+        const uint32_t x = s_w[w];
+        if (w < wave) before += x;
+        total += x;
+    }
+    __syncthreads();   // s_w may be reused
+    return before + incl - v;
 }
 
-// Filter, count and Adler partials of one row; one workgroup of 256 per row,
-// the filtered row staged in LDS for the run sweep of wave 0.
-__global__ void __launch_bounds__(256) k_png_filter_count(const uint8_t* __restrict__ img, int width, int height,
+// Minimum of v over the threads after this one (INT_MAX for the last).
+__device__ inline int block_min_after(int v, int* s_w) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = v;   // suffix minimum within the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_down(incl, o, 64);
+        if (lane + o < 64) incl = min(incl, y);
+    }
+    int after = __shfl_down(incl, 1, 64);
+    if (lane == 63) after = INT_MAX;
+    if (lane == 0) s_w[wave] = incl;
+    __syncthreads();
+    for (int w = wave + 1; w < kRowWaves; ++w) after = min(after, s_w[w]);
+    __syncthreads();
+    return after;
+}
+
+// Runs of one filtered row (n bytes in LDS), parsed by the whole workgroup.
+// A run starts at byte 0 and wherever a byte differs from the previous one,
+// and ends at the next start (or the row end). Thread t owns the bytes
+// [t k, (t + 1) k), k = ceil(n / kRowThreads), and the runs starting there;
+// the end of its last run is the first start owned by a later thread. With
+// kEmit, fn(bit_pos, value, length) is called for each run by its owner at
+// the run's bit offset in the row (exclusive scan of the owners' bit counts).
+// Counting and emitting use this same parse, so a row's bits match its count.
+// Returns the row's bit length (on every thread).
+template <bool kEmit, typename Fn>
+__device__ inline uint32_t parse_row(const uint8_t* row, int n, uint32_t* s_u, int* s_i, Fn fn) {
+    const int k = (n + kRowThreads - 1) / kRowThreads;
+    const int lo = min(n, (int)threadIdx.x * k), hi = min(n, lo + k);
+    int first = INT_MAX;
+    for (int i = lo; i < hi; ++i)
+        if (i == 0 || row[i] != row[i - 1]) {
+            first = i;
+            break;
+        }
+    int next = block_min_after(first, s_i);
+    if (next == INT_MAX) next = n;
+    uint32_t bits = 0;
+    if (first != INT_MAX) {
+        int st = first, v = row[first];
+        for (int i = first + 1; i < hi; ++i) {
+            const int b = row[i];
+            if (b != v) {
+                bits += run_bits((uint32_t)v, i - st);
+                st = i;
+                v = b;
+            }
+        }
+        bits += run_bits((uint32_t)v, next - st);
+    }
+    uint32_t total;
+    const uint32_t off = block_excl_sum(bits, s_u, total);
+    if (kEmit && first != INT_MAX) {
+        uint32_t pos = off;
+        int st = first, v = row[first];
+        for (int i = first + 1; i < hi; ++i) {
+            const int b = row[i];
+            if (b != v) {
+                fn(pos, v, i - st);
+                pos += run_bits((uint32_t)v, i - st);
+                st = i;
+                v = b;
+            }
+        }
+        fn(pos, v, next - st);
+    }
+    return total;
+}
+
+// Filter, count and Adler partials of one row; one workgroup per row, the
+// filtered row staged in LDS for the run parse.
+__global__ void __launch_bounds__(kRowThreads) k_png_filter_count(const uint8_t* __restrict__ img, int width, int height,
                                                           int ch, PngWs ws) {
     extern __shared__ uint8_t s_row[];
     const int r = blockIdx.x;
@@ -294,19 +326,22 @@ __global__ void __launch_bounds__(256) k_png_filter_count(const uint8_t* __restr
         s_ad[1][wave] = sb;
     }
     __syncthreads();
-    if (wave == 0) {
-        uint32_t bits = sweep_row(s_row, n, [](uint32_t, int, int) {});
-        if (lane == 0) {
-            ws.row_bits[r] = bits;
-            ws.row_s[r] = s_ad[0][0] + s_ad[0][1] + s_ad[0][2] + s_ad[0][3];
-            ws.row_t[r] = s_ad[1][0] + s_ad[1][1] + s_ad[1][2] + s_ad[1][3];
-        }
+    __shared__ uint32_t s_u[kRowWaves];
+    __shared__ int s_i[kRowWaves];
+    const uint32_t bits = parse_row<false>(s_row, n, s_u, s_i, [](uint32_t, int, int) {});
+    if (threadIdx.x == 0) {
+        ws.row_bits[r] = bits;
+        ws.row_s[r] = s_ad[0][0] + s_ad[0][1] + s_ad[0][2] + s_ad[0][3];
+        ws.row_t[r] = s_ad[1][0] + s_ad[1][1] + s_ad[1][2] + s_ad[1][3];
     }
 }
 
-// One workgroup: row bit offsets, the deflate end, Adler-32, and the bytes
-// before the deflate data (signature, IHDR, IDAT type, zlib header, block
-// header). The IDAT length is written by the tail kernel.
+// One workgroup: row bit offsets, the deflate end, Adler-32, every byte of the
+// PNG before the deflate data (signature, IHDR, IDAT length and type, zlib
+// header, block header) and after it up to the CRC (end-of-block code,
+// padding, Adler-32). The output is not cleared beforehand: the emit kernel
+// stores a row's interior words whole and ORs only the first and last word of
+// each row, which this kernel zeroes, with the words of the end-of-block code.
 __global__ void __launch_bounds__(1024) k_png_scan(int height, int stride, PngWs ws, uint8_t* png, uint32_t ihdr_crc,
                                                    int width, int ch) {
     __shared__ uint64_t s_part[1024];
@@ -344,14 +379,38 @@ __global__ void __launch_bounds__(1024) k_png_scan(int height, int stride, PngWs
     }
     const uint64_t start = kDeflateStartBit + 3;   // after BFINAL + BTYPE
     uint64_t off = start + (t > 0 ? s_part[t - 1] : 0);
+    uint32_t* words = (uint32_t*)png;
     for (int r = r0; r < r1; ++r) {
+        const uint32_t nb = ws.row_bits[r];
         ws.row_off[r] = off;
-        off += ws.row_bits[r];
+        words[off >> 5] = 0u;
+        words[(off + nb - 1) >> 5] = 0u;
+        off += nb;
     }
+    const uint64_t end = start + s_part[nt - 1];   // end of the row bits
+    // end-of-block: symbol 256 is seven zero bits; with the padding to the
+    // byte boundary it lies in the (at most two) words from the end bit on
+    const uint64_t dbytes_end = (end + 7 + 7) / 8;   // byte after the deflate data
     if (t == 0) {
-        ws.totals[0] = start + s_part[nt - 1];   // end of the row bits (the end-of-block code follows)
+        words[end >> 5] = 0u;
+        words[(dbytes_end * 8 - 1) >> 5] = 0u;
+    }
+    // the zeroed words complete before any thread writes bytes into them (one
+    // workgroup: the barrier orders its global stores)
+    __syncthreads();
+    if (t == 0) {
+        ws.totals[0] = end;
         const uint64_t Af = (1 + s_a[0]) % kAdlerMod, Bf = (N % kAdlerMod + s_b[0]) % kAdlerMod;
-        ws.totals[3] = (Bf << 16) | Af;
+        const uint32_t adler = (uint32_t)((Bf << 16) | Af);
+        ws.totals[3] = adler;
+        uint8_t* a = png + dbytes_end;
+        a[0] = (uint8_t)(adler >> 24); a[1] = (uint8_t)(adler >> 16); a[2] = (uint8_t)(adler >> 8);
+        a[3] = (uint8_t)adler;
+        const uint64_t zlib_len = dbytes_end + 4 - 41;
+        png[33] = (uint8_t)(zlib_len >> 24); png[34] = (uint8_t)(zlib_len >> 16);
+        png[35] = (uint8_t)(zlib_len >> 8); png[36] = (uint8_t)zlib_len;
+        ws.totals[1] = zlib_len;
+        ws.totals[4] = 0;   // CRC workgroup ticket
         // signature
         const uint8_t sig[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
         for (int i = 0; i < 8; ++i) png[i] = sig[i];
@@ -365,7 +424,7 @@ __global__ void __launch_bounds__(1024) k_png_scan(int height, int stride, PngWs
         h[16] = 8; h[17] = ch == 3 ? 2 : 6; h[18] = 0; h[19] = 0; h[20] = 0;
         h[21] = (uint8_t)(ihdr_crc >> 24); h[22] = (uint8_t)(ihdr_crc >> 16); h[23] = (uint8_t)(ihdr_crc >> 8);
         h[24] = (uint8_t)ihdr_crc;
-        // IDAT type; the length (bytes 33..36) is known after the tail
+        // IDAT type
         png[37] = 'I'; png[38] = 'D'; png[39] = 'A'; png[40] = 'T';
         // zlib header: deflate, 32K window, no dictionary, fastest (0x78 0x01)
         png[41] = 0x78; png[42] = 0x01;
@@ -374,11 +433,14 @@ __global__ void __launch_bounds__(1024) k_png_scan(int height, int stride, PngWs
     }
 }
 
-// Emit: one wave per row stages the filtered row in LDS, builds the row's
-// bits there, then ORs them into the output at the row's bit offset (the
-// first and last words, shared with neighbours, through atomics).
-__global__ void __launch_bounds__(64) k_png_emit(int height, int stride, PngWs ws, uint8_t* png) {
+// Emit: one workgroup per row stages the filtered row in LDS, builds the
+// row's bits there, then writes them at the row's bit offset: interior words
+// whole, the first and last words (shared with the neighbouring rows or the
+// bytes around the deflate data, and zeroed by the scan) through atomic OR.
+__global__ void __launch_bounds__(kRowThreads) k_png_emit(int height, int stride, PngWs ws, uint8_t* png) {
     extern __shared__ uint32_t s_mem[];
+    __shared__ uint32_t s_u[kRowWaves];
+    __shared__ int s_i[kRowWaves];
     const int r = blockIdx.x;
     if (r >= height) return;
     const int n = stride + 1;
@@ -388,8 +450,8 @@ __global__ void __launch_bounds__(64) k_png_emit(int height, int stride, PngWs w
     const uint8_t* row = ws.filt + (size_t)r * n;
     const uint32_t nbits = ws.row_bits[r];
     const uint32_t nwords = (nbits + 31) / 32 + 1;
-    for (int i = threadIdx.x; i < n; i += 64) s_row[i] = row[i];
-    for (uint32_t i = threadIdx.x; i < nwords; i += 64) s_bits[i] = 0u;
+    for (int i = threadIdx.x; i < n; i += kRowThreads) s_row[i] = row[i];
+    for (uint32_t i = threadIdx.x; i < nwords; i += kRowThreads) s_bits[i] = 0u;
     __syncthreads();
     auto put = [&](uint32_t pos, uint32_t bits, int len) {
         // len <= 32 (a match: at most 8 + 5 + 5 bits)
@@ -397,7 +459,7 @@ __global__ void __launch_bounds__(64) k_png_emit(int height, int stride, PngWs w
         atomicOr(&s_bits[w], bits << o);
         if (o + len > 32) atomicOr(&s_bits[w + 1], bits >> (32 - o));
     };
-    sweep_row(s_row, n, [&](uint32_t pos, int v, int len) {
+    parse_row<true>(s_row, n, s_u, s_i, [&](uint32_t pos, int v, int len) {
         uint32_t b;
         int l;
         lit_code((uint32_t)v, b, l);
@@ -422,12 +484,11 @@ __global__ void __launch_bounds__(64) k_png_emit(int height, int stride, PngWs w
         }
     });
     __syncthreads();
-    // OR the row's bits into the output at its offset
     const uint64_t off = ws.row_off[r];
     uint32_t* out = (uint32_t*)png;
     const uint64_t w_first = off >> 5, w_last = (off + nbits - 1) >> 5;
     const uint32_t sh = (uint32_t)(off & 31);
-    for (uint64_t w = w_first + threadIdx.x; w <= w_last; w += 64) {
+    for (uint64_t w = w_first + threadIdx.x; w <= w_last; w += kRowThreads) {
         const uint64_t k = w - w_first;   // output word k of this row
         // bits [32k - sh, 32k - sh + 32) of the row stream
         uint32_t val;
@@ -442,81 +503,80 @@ __global__ void __launch_bounds__(64) k_png_emit(int height, int stride, PngWs w
     }
 }
 
-// One thread: end-of-block code, padding, Adler-32, IDAT length.
-__global__ void k_png_tail(PngWs ws, uint8_t* png) {
-    const uint64_t end = ws.totals[0];   // bit after the last row
-    // end-of-block: symbol 256, 7 zero bits; the deflate stream ends at end + 7
-    const uint64_t dbytes_end = (end + 7 + 7) / 8;   // byte after the deflate data
-    const uint32_t adler = (uint32_t)ws.totals[3];
-    uint8_t* a = png + dbytes_end;
-    a[0] = (uint8_t)(adler >> 24); a[1] = (uint8_t)(adler >> 16); a[2] = (uint8_t)(adler >> 8); a[3] = (uint8_t)adler;
-    const uint64_t zlib_len = dbytes_end + 4 - 41;
-    png[33] = (uint8_t)(zlib_len >> 24); png[34] = (uint8_t)(zlib_len >> 16);
-    png[35] = (uint8_t)(zlib_len >> 8); png[36] = (uint8_t)zlib_len;
-    ws.totals[1] = zlib_len;
-}
-
-// XOR over the workgroup (waves of 64, at most 16 waves).
-__device__ inline uint32_t block_xor(uint32_t v) {
-    __shared__ uint32_t s_x[16];
-    for (int o = 32; o > 0; o >>= 1) v ^= __shfl_xor(v, o, 64);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) s_x[wave] = v;
-    __syncthreads();
-    uint32_t r = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r ^= s_x[w];
-    return r;
-}
-
-// CRC (no final xor) of 4096-byte blocks of the IDAT type + data:
-// each thread's 16-byte CRC moved past the bytes that follow it in the block
-// (times x^(8 n) mod P), then XORed over the block.
-__global__ void __launch_bounds__(256) k_crc_blocks(const uint8_t* png, PngWs ws, uint64_t max_blocks, PowTable pw) {
+// CRC-32 of the IDAT type + data (len bytes from png + 37), in workgroups of
+// kCrcBlock bytes. The stream is taken as nblk * kCrcBlock bytes with
+// pad = nblk * kCrcBlock - len zero bytes in front: a CRC register at zero
+// stays zero over zero bytes, and the standard initial value 0xFFFFFFFF is
+// XORed into the register at the first real byte, so the result is the CRC
+// of the real bytes. Every workgroup and every thread then covers a
+// power-of-two span, and spans combine as crc(A B) = crc(A) x^(8|B|) + crc(B)
+// with the powers taken from the table. The last workgroup to finish (atomic
+// ticket; nothing waits) combines the workgroups' CRCs and writes the CRC,
+// IEND and the PNG length.
+__global__ void __launch_bounds__(256) k_crc(uint8_t* png, PngWs ws, uint64_t max_blocks, uint64_t* png_len,
+                                             PowTable pw) {
     __shared__ uint32_t table[256];
+    __shared__ uint32_t s_w[4];
+    __shared__ int s_last;
+    constexpr int kPer = kCrcBlock / 256;   // 16 bytes per thread
+    static_assert(kPer == 16 && kCrcBlock == 4096, "the tree levels below assume 16-byte spans in 4 KiB blocks");
     for (int i = threadIdx.x; i < 256; i += blockDim.x) {
         uint32_t c = (uint32_t)i;
         for (int k = 0; k < 8; ++k) c = (c & 1u) ? (c >> 1) ^ kCrcPoly : c >> 1;
         table[i] = c;
     }
     __syncthreads();
-    const uint64_t len = 4 + ws.totals[1];   // "IDAT" + zlib data
-    const uint64_t blk = blockIdx.x;
-    if (blk >= max_blocks || blk * kCrcBlock >= len) return;
-    const uint8_t* base = png + 37 + blk * kCrcBlock;
-    const uint64_t bl = min((uint64_t)kCrcBlock, len - blk * kCrcBlock);
-    constexpr int kPer = kCrcBlock / 256;
-    const uint64_t s = (uint64_t)threadIdx.x * kPer;
-    uint32_t v = 0;
-    if (s < bl) {
-        const uint64_t e = min(s + kPer, bl);
-        // the standard initial value 0xFFFFFFFF enters with the first byte;
-        // by linearity it then travels to the end like any other term
-        uint32_t c = (blk == 0 && threadIdx.x == 0) ? 0xFFFFFFFFu : 0u;
-        for (uint64_t k = s; k < e; ++k) c = crc_update_raw(c, base[k], table);
-        v = gf2_mulmod(c, x_pow_8n_tab(pw, bl - e));
-    }
-    const uint32_t crc = block_xor(v);
-    if (threadIdx.x == 0) ws.crc_part[blk] = crc;
-}
-
-// One workgroup: block CRCs moved past the bytes after them and XORed, the
-// standard conditioning, CRC bytes, IEND, length.
-__global__ void __launch_bounds__(1024) k_crc_final(uint8_t* png, PngWs ws, uint64_t* png_len, PowTable pw) {
     const uint64_t len = 4 + ws.totals[1];
     const uint64_t nblk = (len + kCrcBlock - 1) / kCrcBlock;
-    uint32_t v = 0;
-    for (uint64_t b = threadIdx.x; b < nblk; b += blockDim.x) {
-        const uint64_t e = min((b + 1) * (uint64_t)kCrcBlock, len);
-        v ^= gf2_mulmod(ws.crc_part[b], x_pow_8n_tab(pw, len - e));
+    const uint64_t blk = blockIdx.x;
+    if (blk >= max_blocks || blk >= nblk) return;
+    const uint64_t pad = nblk * kCrcBlock - len;
+    const uint8_t* base = png + 37;
+    const uint64_t q0 = blk * kCrcBlock + (uint64_t)threadIdx.x * kPer;
+    uint32_t c = 0;
+    for (int k = 0; k < kPer; ++k) {
+        const uint64_t q = q0 + k;
+        if (q == pad) c ^= 0xFFFFFFFFu;
+        const uint8_t b = q >= pad ? base[q - pad] : (uint8_t)0;
+        c = crc_update_raw(c, b, table);
     }
-    const uint32_t acc = block_xor(v);
+    // tree over the wave: level j joins spans of 16 * 2^j bytes
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = 0; j < 6; ++j) {
+        const uint32_t right = __shfl_down(c, 1 << j, 64);
+        c = gf2_mulmod(c, pw.p[4 + j]) ^ right;   // meaningful on lanes that are multiples of 2^(j+1)
+    }
+    if (lane == 0) s_w[wave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t b = s_w[0];
+        for (int w = 1; w < 4; ++w) b = gf2_mulmod(b, pw.p[10]) ^ s_w[w];   // 1024-byte spans
+        ws.crc_part[blk] = b;
+        __threadfence();
+        const unsigned long long tk = atomicAdd((unsigned long long*)&ws.totals[4], 1ull);
+        s_last = tk == nblk - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // the last workgroup: thread t folds blocks [t m, (t + 1) m) by Horner's
+    // rule, moves the result past the blocks after them, then XOR over threads
+    const uint64_t m = (nblk + 255) / 256;
+    const uint64_t b0 = min(nblk, (uint64_t)threadIdx.x * m), b1 = min(nblk, b0 + m);
+    uint32_t acc = 0;
+    for (uint64_t b = b0; b < b1; ++b)
+        acc = gf2_mulmod(acc, pw.p[12]) ^ __hip_atomic_load(&ws.crc_part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t v = b1 > b0 ? gf2_mulmod(acc, x_pow_8n_tab(pw, (nblk - b1) * (uint64_t)kCrcBlock)) : 0u;
+    for (int o = 32; o > 0; o >>= 1) v ^= __shfl_xor(v, o, 64);
+    __syncthreads();
+    if (lane == 0) s_w[wave] = v;
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    // standard CRC-32: the initial value went in with the first block; final xor
-    const uint32_t crc = acc ^ 0xFFFFFFFFu;
-    uint8_t* c = png + 37 + len;
-    c[0] = (uint8_t)(crc >> 24); c[1] = (uint8_t)(crc >> 16); c[2] = (uint8_t)(crc >> 8); c[3] = (uint8_t)crc;
+    const uint32_t crc = (s_w[0] ^ s_w[1] ^ s_w[2] ^ s_w[3]) ^ 0xFFFFFFFFu;   // final XOR
+    uint8_t* cp = png + 37 + len;
+    cp[0] = (uint8_t)(crc >> 24); cp[1] = (uint8_t)(crc >> 16); cp[2] = (uint8_t)(crc >> 8); cp[3] = (uint8_t)crc;
     const uint8_t iend[12] = {0, 0, 0, 0, 'I', 'E', 'N', 'D', 0xAE, 0x42, 0x60, 0x82};
-    for (int i = 0; i < 12; ++i) c[4 + i] = iend[i];
+    for (int i = 0; i < 12; ++i) cp[4 + i] = iend[i];
     const uint64_t total = 37 + len + 4 + 12;
     ws.totals[2] = total;
     *png_len = total;
@@ -628,20 +688,15 @@ extern "C" mk_status mk_png_encode_async(const uint8_t* dev_pixels, int width, i
     const uint32_t ihdr_crc = host_crc32(ih, sizeof(ih));
     const int stride = width * channels;
     hipStream_t s = (hipStream_t)stream;
-    const uint64_t words = bound / 4;
-    hipLaunchKernelGGL(k_zero_words, dim3((uint32_t)std::min<uint64_t>((words + 255) / 256, 65535)), dim3(256), 0, s,
-                       (uint32_t*)dev_png, words);
     const size_t row_lds = ((size_t)stride + 1 + 3) / 4 * 4;
-    hipLaunchKernelGGL(k_png_filter_count, dim3(height), dim3(256), row_lds, s, dev_pixels, width, height, channels,
-                       ws);
+    hipLaunchKernelGGL(k_png_filter_count, dim3(height), dim3(kRowThreads), row_lds, s, dev_pixels, width, height,
+                       channels, ws);
     hipLaunchKernelGGL(k_png_scan, dim3(1), dim3(1024), 0, s, height, stride, ws, dev_png, ihdr_crc, width, channels);
     const size_t emit_lds = row_lds + ((size_t)(stride + 1) * 9 / 32 + 2) * 4;
-    hipLaunchKernelGGL(k_png_emit, dim3(height), dim3(64), emit_lds, s, height, stride, ws, dev_png);
-    hipLaunchKernelGGL(k_png_tail, dim3(1), dim3(1), 0, s, ws, dev_png);
+    hipLaunchKernelGGL(k_png_emit, dim3(height), dim3(kRowThreads), emit_lds, s, height, stride, ws, dev_png);
     const uint64_t max_blocks = (bound + kCrcBlock - 1) / kCrcBlock;
     static const PowTable pw = make_pow_table();
-    hipLaunchKernelGGL(k_crc_blocks, dim3((uint32_t)max_blocks), dim3(256), 0, s, dev_png, ws, max_blocks, pw);
-    hipLaunchKernelGGL(k_crc_final, dim3(1), dim3(1024), 0, s, dev_png, ws, dev_png_len, pw);
+    hipLaunchKernelGGL(k_crc, dim3((uint32_t)max_blocks), dim3(256), 0, s, dev_png, ws, max_blocks, dev_png_len, pw);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return mk_hip_fail(nullptr, e, "mk_png_encode_async launch");
     return MK_OK;

@@ -312,6 +312,7 @@ class RenderServer:
                 it.cpr_points_offset, it.cpr_dirs_offset = cur, cur + nb
                 cur = _align(cur + 2 * nb)
             items.append(it)
+        geo_end = cur
         b64_offs = []
         if self.encode:
             for r in batch:
@@ -321,8 +322,8 @@ class RenderServer:
         for r in batch:
             r.t_admit = now
         with torch.cuda.stream(s.stream):
-            if cur > out_end:
-                s.arena[out_end:cur].copy_(s.host[out_end:cur], non_blocking=True)
+            if geo_end > out_end:   # CPR geometry; the text region is written on the device
+                s.arena[out_end:geo_end].copy_(s.host[out_end:geo_end], non_blocking=True)
             try:
                 self.batcher.submit(si, items, s.arena, self.slot_bytes, stream=s.stream.cuda_stream)
             except _capi.MkError as e:
