@@ -172,6 +172,53 @@ def fit_main(args) -> int:
     return 0
 
 
+def series_section(ctx, spec, n_images: int = 256, size: int = 512):
+    """GetBatchData: a series of n_images parallel 3 mm MIP slabs of size^2 on
+    the body volume (an oblique frame through the z axis, 1 mm apart), through
+    the batcher against one mk_render_plane_async launch per image."""
+    import torch
+
+    from monkeygl_amd import mpr as M
+    from monkeygl_amd._capi import MK_MPR_MIP
+    from monkeygl_amd.series import SeriesRenderer
+
+    h, v = M.batch_direction_3d((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), 30.0)
+    ps = min(spec.spacing)
+    c = tuple(0.5 * d * s for d, s in zip(spec.dims, spec.spacing))
+    info = M.BatchInfo(h, v, c, size * ps, size * ps, ps, 3.0, 1.0, n_images, MK_MPR_MIP)
+    params = M.batch_plane_params(info, spec.spacing)
+    w, hgt = params[0].width, params[0].height
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    a = torch.empty(n_images * w * hgt, dtype=torch.int16, device="cuda")
+    b = torch.empty_like(a)
+    sr = SeriesRenderer(ctx)
+    try:
+        sr.render_async(params, a, stream=sh)   # warm
+        for k, p in enumerate(params):
+            ctx.render_plane_async(p, b[k * w * hgt:(k + 1) * w * hgt], stream=sh)
+        torch.cuda.synchronize()
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        e[0].record(stream)
+        sr.render_async(params, a, stream=sh)
+        e[1].record(stream)
+        e[2].record(stream)
+        for k, p in enumerate(params):
+            ctx.render_plane_async(p, b[k * w * hgt:(k + 1) * w * hgt], stream=sh)
+        e[3].record(stream)
+        torch.cuda.synchronize()
+    finally:
+        sr.close()
+    same = int((a.view(n_images, -1) == b.view(n_images, -1)).all(dim=1).sum().item())
+    tb, ts = e[0].elapsed_time(e[1]), e[2].elapsed_time(e[3])
+    out = {"images": n_images, "image": [w, hgt], "slab_mm": 3.0, "slice_distance_mm": 1.0, "mpr_type": "MIP",
+           "batched_ms": round(tb, 3), "batched_images_per_s": round(n_images / tb * 1e3, 1),
+           "per_image_launches_ms": round(ts, 3), "per_image_launches_images_per_s": round(n_images / ts * 1e3, 1),
+           "identical": f"{same}/{n_images}"}
+    log(f"batch series: {out}")
+    return out
+
+
 def serve_section(args, body_ctx, spec, vol, mask, luts, labels):
     """Served vs sequential plain rendering of one seeded request stream."""
     import dataclasses
@@ -459,6 +506,7 @@ def main() -> int:
             encode["fpng_cores"] = 1
             log(f"fpng on the host: {encode}")
 
+    series = series_section(ctx, spec)
     served = None if args.no_serve else serve_section(args, ctx, spec, vol, mask, luts, labels)
 
     line = {
@@ -487,6 +535,7 @@ def main() -> int:
         "parity": parity,
         "cpu_baseline": cpu,
         "encode": encode,
+        "series": series,
         "served": served,
     }
     print(json.dumps(line), flush=True)

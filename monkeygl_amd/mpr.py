@@ -402,3 +402,91 @@ class MPRInfo:
         p.pixel_spacing = _f(ps)
         p.half_num = half
         return p
+
+
+# -- batch series (core/BatchInfo.h, DataManager::GetBatchDirection3D) ------------
+
+def batch_direction_3d(dir_h, dir_v, angle: float):
+    """DataManager::GetBatchDirection3D (core/DataManager.cpp:613-644) for a
+    plane frame (dir_h, dir_v): dir_h turned by ``angle`` degrees about the
+    plane normal a = dir_h x dir_v (the axis-angle rotation matrix, applied by
+    GetTransferPoint), and a itself as the vertical direction."""
+    a = cross(dir_h, dir_v)
+    t = angle / 180 * PI
+    x, y, z = a
+    xx, yy, zz = x * x, y * y, z * z
+    s, c = math.sin(t), math.cos(t)
+    ct = 1.0 - c
+    m = [[xx + (1 - xx) * c, x * y * ct - z * s, x * z * ct + y * s],
+         [x * y * ct + z * s, yy + (1 - yy) * c, y * z * ct - x * s],
+         [x * z * ct - y * s, y * z * ct + x * s, zz + (1 - zz) * c]]
+    return _dir(*transfer_point(m, dir_h)), a
+
+
+class BatchInfo:
+    """core/BatchInfo.h:29-54: a series of ``num`` slab images of one frame.
+    Points and lengths are in mm, like the MPR state."""
+
+    def __init__(self, dir_h=(1.0, 0.0, 0.0), dir_v=(0.0, 1.0, 0.0), center=(0.0, 0.0, 0.0), length_h=0.0,
+                 length_v=0.0, pixel_spacing=0.0, slice_thickness=0.0, slice_distance=0.0, num=0,
+                 mpr_type=MK_MPR_AVERAGE):
+        self.dir_h, self.dir_v, self.center = tuple(dir_h), tuple(dir_v), tuple(center)
+        self.length_h, self.length_v = float(length_h), float(length_v)
+        self.pixel_spacing = float(pixel_spacing)
+        self.slice_thickness, self.slice_distance = float(slice_thickness), float(slice_distance)
+        self.num = int(num)
+        self.mpr_type = int(mpr_type)
+
+    def width(self) -> int:
+        return int(self.length_h / self.pixel_spacing)
+
+    def height(self) -> int:
+        return int(self.length_v / self.pixel_spacing)
+
+
+def batch_plane_params(info: BatchInfo, spacing, invert_z: bool = False,
+                       sampling: int = MK_SAMPLE_LINEAR) -> list | None:
+    """Kernel arguments of every image of a batch series, or None when the
+    series is empty or degenerate.
+
+    The reference declares the series (core/BatchInfo.h) and the entry point
+    (Render::GetBatchData, core/Render.cpp:117-120) but leaves it unimplemented.
+    The series here: ``num`` parallel slabs spanned by (dir_h, dir_v), image k
+    centred at center + (k - (num - 1) / 2) * slice_distance * n with
+    n = dir_h x dir_v, each width() x height() pixels of pixel_spacing and a
+    slab of slice_thickness combined by mpr_type, with the per-image arithmetic
+    of the MPR planes (left-top corner from the centre, slab half count from
+    thickness / pixel spacing, everything rounded to float)."""
+    if info.num <= 0 or info.pixel_spacing <= 0:
+        return None
+    w, h = info.width(), info.height()
+    if w <= 0 or h <= 0:
+        return None
+    dh, dv = _dir(*info.dir_h), _dir(*info.dir_v)
+    n = cross(dh, dv)
+    if any(math.isnan(c) for c in dh + dv + n):
+        return None
+    ps = info.pixel_spacing
+    n_slice = int(info.slice_thickness / ps)
+    n_slice = 1 if n_slice < 1 else n_slice
+    half = _f(_f(1.0 * (n_slice - 1)) / 2)
+    out = []
+    for k in range(info.num):
+        center = _add(info.center, _mul(n, (k - (info.num - 1) / 2.0) * info.slice_distance))
+        lt = _sub(center, _mul(dh, 0.5 * w * ps))
+        lt = _sub(lt, _mul(dv, 0.5 * h * ps))
+        p = PlaneParams()
+        p.width, p.height = w, h
+        p.mpr_type = info.mpr_type
+        p.sampling = sampling
+        p.invert_z = int(invert_z)
+        for i in range(3):
+            p.spacing[i] = _f(spacing[i])
+            p.dir_h[i] = _f(dh[i])
+            p.dir_v[i] = _f(dv[i])
+            p.dir_n[i] = _f(n[i])
+            p.pt_left_top[i] = _f(lt[i])
+        p.pixel_spacing = _f(ps)
+        p.half_num = half
+        out.append(p)
+    return out

@@ -210,6 +210,37 @@ class Direction3d:
         return self._c
 
 
+class BatchInfo:
+    """core/BatchInfo.h:29-54 (the members without their m_ prefix); every
+    field starts at zero, as the reference's memset constructor leaves it."""
+
+    def __init__(self):
+        self.dirH = (0.0, 0.0, 0.0)
+        self.dirV = (0.0, 0.0, 0.0)
+        self.ptCenter = (0.0, 0.0, 0.0)
+        self.fLengthH = 0.0
+        self.fLengthV = 0.0
+        self.fPixelSpacing = 0.0
+        self.fSliceThickness = 0.0
+        self.fSliceDistance = 0.0
+        self.nNum = 0
+        self.mprType = MPRType.MPRTypeAverage
+
+    def Width(self) -> int:
+        return int(self.fLengthH / self.fPixelSpacing)
+
+    def Height(self) -> int:
+        return int(self.fLengthV / self.fPixelSpacing)
+
+    def _series(self) -> _mpr.BatchInfo:
+        def t(v):
+            return v.tuple() if isinstance(v, (Direction3d, Point3d)) else tuple(float(c) for c in v)
+
+        return _mpr.BatchInfo(t(self.dirH), t(self.dirV), t(self.ptCenter), self.fLengthH, self.fLengthV,
+                              self.fPixelSpacing, self.fSliceThickness, self.fSliceDistance, self.nNum,
+                              int(self.mprType))
+
+
 class DeviceInfo:
     def GetCount(self, count: int = 0) -> bool:
         from . import _capi
@@ -270,6 +301,7 @@ class HelloMonkey:
         self._plane_label = 0
         self._dirty_vol = self._dirty_mask = self._dirty_lut = False
         self._encoders: dict = {}
+        self._series_renderer = None
 
     # -- device sync ----------------------------------------------------------
     def _device(self):
@@ -747,6 +779,35 @@ class HelloMonkey:
         dev = torch.tensor(sl, device="cuda")
         return self._int16_png_b64(dev, W, H)
 
+    # -- batch series (core/HelloMonkey.cpp:415-420, 614-619) ---------------------
+    def GetBatchDirection3D(self, fAngle: float, planeType):
+        """(dirH, dirV) of a batch plane: the plane's dirH turned by fAngle
+        degrees about its normal, and the normal (DataManager::GetBatchDirection3D);
+        None for a plane without MPR state."""
+        d = self._mpr_state().get_direction_3d(int(planeType))
+        if d is None:
+            return None
+        h, v = _mpr.batch_direction_3d(d[0], d[1], float(fAngle))
+        return Direction3d(*h), Direction3d(*v)
+
+    def GetBatchData(self, batchInfo) -> list:
+        """The images of a batch series as (height, width) int16 arrays, one
+        per slice, or [] for an empty or degenerate series. The reference
+        leaves this unimplemented (core/Render.cpp:117-120); the series is
+        nNum parallel slabs centred on ptCenter, fSliceDistance apart along
+        dirH x dirV, rendered in batches on the GPU."""
+        from .series import SeriesRenderer
+
+        params = _mpr.batch_plane_params(batchInfo._series(), self._spacing, invert_z=self._need_invert_z(),
+                                         sampling=MK_SAMPLE_LINEAR)
+        if not params:
+            return []
+        c = self._device()
+        if self._series_renderer is None:
+            self._series_renderer = SeriesRenderer(c)
+        arr = self._series_renderer.render(params)
+        return [arr[k] for k in range(arr.shape[0])]
+
     def Transfer2Base64Array(self, npData) -> None:
         """Prints the base64 of width * height bytes as a quoted line (core/HelloMonkey.cpp:54-61)."""
         a = np.asarray(npData)
@@ -754,7 +815,7 @@ class HelloMonkey:
         print(f'"{_b64.b64encode(_raw_memory(a).astype(np.uint8)[:n].tobytes()).decode()}",')
 
 
-__all__ = ["HelloMonkey", "RGBA", "Point2d", "Point3d", "Direction3d", "DeviceInfo", "LogLevel", "PlaneType",
-           "Orientation", "MPRType", "RenderType", "LayerType", "FontSize", "AnnotationFormat"]
+__all__ = ["HelloMonkey", "RGBA", "Point2d", "Point3d", "Direction3d", "DeviceInfo", "BatchInfo", "LogLevel",
+           "PlaneType", "Orientation", "MPRType", "RenderType", "LayerType", "FontSize", "AnnotationFormat"]
 __all__ += [m for e in (LogLevel, PlaneType, Orientation, MPRType, RenderType, LayerType, FontSize,
                         AnnotationFormat) for m in e.__members__]
