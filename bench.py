@@ -24,7 +24,14 @@ The JSON line carries:
     p50/p99 latency (arrival to result in host memory) and frames/s of the
     continuous-batching server next to one-at-a-time plain rendering of the
     same request stream, the same two at saturation (every request queued at
-    once), and the count of served images bit-identical to the plain ones.
+    once), and the count of served images bit-identical to the plain ones,
+    then the same stream served as base64 PNG text (served.encoded);
+  * encode: the headline frame to PNG + base64 on the device, next to the
+    reference's fpng on one host core;
+  * series: a GetBatchData series of parallel MIP slabs through the batcher
+    against one plane launch per image;
+  * mask_ingest: an object-mask edit merged on the device against the
+    reference's host merge + full mask upload.
 """
 from __future__ import annotations
 
@@ -216,6 +223,47 @@ def series_section(ctx, spec, n_images: int = 256, size: int = 512):
            "per_image_launches_ms": round(ts, 3), "per_image_launches_images_per_s": round(n_images / ts * 1e3, 1),
            "identical": f"{same}/{n_images}"}
     log(f"batch series: {out}")
+    return out
+
+
+def mask_ingest_section(spec, vol, mask):
+    """An object-mask edit on the body volume with its 4-label mask: the device
+    merge (mk_merge_object_mask from host memory, and from a device copy)
+    against the reference's path, a host merge of the whole mask followed by a
+    full mask upload (VolumeInfo::AddNewObjectMask + CopyMaskData)."""
+    import torch
+
+    from monkeygl_amd import workloads as W
+    from monkeygl_amd._capi import MK_MASK_ADD, Context
+
+    obj = (W.ellipsoid_mask(spec.dims, 1, seed=9) > 0).astype(np.uint8)
+    c = Context(0)
+    try:
+        c.set_volume(vol)
+        c.set_mask(mask)
+        t = time.perf_counter()
+        c.merge_mask(obj, 5, MK_MASK_ADD)
+        t_host = time.perf_counter() - t
+        merged = c.get_mask()
+        dev = torch.from_numpy(obj).cuda()
+        torch.cuda.synchronize()
+        c.set_mask(mask)
+        t = time.perf_counter()
+        c.merge_mask_device(dev, obj.shape, 5, MK_MASK_ADD)
+        t_dev = time.perf_counter() - t
+        # the reference's path: merge on the host, upload the whole mask
+        t = time.perf_counter()
+        m = mask.copy()
+        m[obj > 0] = 5
+        c.set_mask(m)
+        t_ref = time.perf_counter() - t
+        same = bool(np.array_equal(merged, m))
+    finally:
+        c.close()
+    out = {"voxels": int(obj.size), "object_voxels": int(obj.sum()), "device_merge_from_host_ms": round(t_host * 1e3, 2),
+           "device_merge_from_device_ms": round(t_dev * 1e3, 2), "host_merge_and_upload_ms": round(t_ref * 1e3, 2),
+           "same_mask": same}
+    log(f"mask ingest: {out}")
     return out
 
 
@@ -507,6 +555,7 @@ def main() -> int:
             log(f"fpng on the host: {encode}")
 
     series = series_section(ctx, spec)
+    mask_ingest = mask_ingest_section(spec, vol, mask)
     served = None if args.no_serve else serve_section(args, ctx, spec, vol, mask, luts, labels)
 
     line = {
@@ -536,6 +585,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "encode": encode,
         "series": series,
+        "mask_ingest": mask_ingest,
         "served": served,
     }
     print(json.dumps(line), flush=True)

@@ -158,6 +158,31 @@ mk_status mk_copy_mask_data_device(mk_context* ctx, const uint8_t* dev, int nx, 
 /* Removes the mask (reference: mask texture 0 => every sample has label 0). */
 mk_status mk_clear_mask(mk_context* ctx);
 
+/* Incremental object-mask edit on the device. Replaces the reference's host
+ * merge and full re-upload: VolumeInfo::AddNewObjectMask / UpdateObjectMask
+ * (core/VolumeInfo.cpp:94-133) followed by CuDataManager::CopyMaskData
+ * (core/DataManager.cpp:280-335). src is a binary mask in the volume's
+ * layout and dims (nonzero = inside). MK_MASK_ADD: voxels inside take label
+ * (without a mask yet the mask starts at 0, where the reference leaves the
+ * rest of a new mask uninitialised). MK_MASK_UPDATE: also voxels outside that
+ * hold label become 0; MK_ERR_STATE without a mask. The brick label sets are
+ * rebuilt in the same pass; label must be 0..MK_MAX_OBJECTS. Like the
+ * upload calls, run it when no render of this context is in flight. */
+typedef enum { MK_MASK_ADD = 0, MK_MASK_UPDATE = 1 } mk_mask_mode;
+mk_status mk_merge_object_mask(mk_context* ctx, const uint8_t* host, int nx, int ny, int nz, int label, int mode);
+mk_status mk_merge_object_mask_device(mk_context* ctx, const uint8_t* dev, int nx, int ny, int nz, int label,
+                                      int mode);
+
+/* The label mask back in the volume's layout (nx*ny*nz bytes) into host
+ * memory: DataManager::GetMaskData (core/DataManager.h:70). MK_ERR_STATE
+ * without a mask. */
+mk_status mk_get_mask_data(mk_context* ctx, uint8_t* host);
+
+/* The image of HelloMonkey::GetOriginData_pngString (core/HelloMonkey.cpp:261-319):
+ * axial slice z (clamped to the volume) as nx*ny int16, x fastest, with
+ * -2048 where the mask holds 0 when there is a mask, into dev_out on stream. */
+mk_status mk_origin_slice_async(mk_context* ctx, int z, int16_t* dev_out, void* stream);
+
 /* cu_setTransferFunc (core/CuDataManager.cu:122-163) with the label of
  * CuDataManager::SetTransferFunction (core/CuDataManager.cpp:80-83).
  * rgba holds n RGBA float entries. A label with no LUT reads as all zero. */

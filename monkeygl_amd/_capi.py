@@ -20,12 +20,14 @@ MK_RENDER_VR, MK_RENDER_MIP, MK_RENDER_SURFACE = range(3)
 MK_MPR_AVERAGE, MK_MPR_MIP, MK_MPR_MINIP = range(3)
 MK_SAMPLE_LINEAR, MK_SAMPLE_NEAREST = range(2)
 MK_ITEM_RENDER, MK_ITEM_PLANE, MK_ITEM_CPR = range(3)
+MK_MASK_ADD, MK_MASK_UPDATE = range(2)
 
 # exported symbols, in header order
 SYMBOLS = [
     "mk_device_count", "mk_context_create", "mk_context_destroy", "mk_last_error",
     "mk_context_stream", "mk_synchronize", "mk_copy_volume_data", "mk_copy_volume_data_device",
-    "mk_copy_mask_data", "mk_copy_mask_data_device", "mk_clear_mask", "mk_set_transfer_func",
+    "mk_copy_mask_data", "mk_copy_mask_data_device", "mk_clear_mask", "mk_merge_object_mask",
+    "mk_merge_object_mask_device", "mk_get_mask_data", "mk_origin_slice_async", "mk_set_transfer_func",
     "mk_render", "mk_render_async", "mk_render_footprint", "mk_render_plane", "mk_render_plane_async",
     "mk_render_cpr", "mk_render_cpr_async", "mk_marching_cubes", "mk_marching_cubes_async",
     "mk_tf_forward", "mk_tf_backward", "mk_png_max_bytes", "mk_png_workspace_bytes", "mk_png_encode_async",
@@ -108,6 +110,10 @@ def load() -> C.CDLL:
         "mk_copy_mask_data": (st, [vp, vp, i, i, i]),
         "mk_copy_mask_data_device": (st, [vp, vp, i, i, i]),
         "mk_clear_mask": (st, [vp]),
+        "mk_merge_object_mask": (st, [vp, vp, i, i, i, i, i]),
+        "mk_merge_object_mask_device": (st, [vp, vp, i, i, i, i, i]),
+        "mk_get_mask_data": (st, [vp, vp]),
+        "mk_origin_slice_async": (st, [vp, i, vp, vp]),
         "mk_set_transfer_func": (st, [vp, i, vp, i]),
         "mk_render": (st, [vp, C.POINTER(RenderParams), vp]),
         "mk_render_async": (st, [vp, C.POINTER(RenderParams), vp, vp]),
@@ -203,6 +209,26 @@ class Context:
 
     def clear_mask(self) -> None:
         self._check(self.lib.mk_clear_mask(self.h))
+
+    def merge_mask(self, obj_zyx: np.ndarray, label: int, mode: int = 0) -> None:
+        """mk_merge_object_mask: obj_zyx nonzero = inside (uint8, the volume's
+        shape); MK_MASK_ADD or MK_MASK_UPDATE."""
+        m = np.ascontiguousarray(obj_zyx, dtype=np.uint8)
+        nz, ny, nx = m.shape
+        self._check(self.lib.mk_merge_object_mask(self.h, _ptr(m), nx, ny, nz, int(label), int(mode)))
+
+    def merge_mask_device(self, t, shape_zyx, label: int, mode: int = 0) -> None:
+        nz, ny, nx = shape_zyx
+        self._check(self.lib.mk_merge_object_mask_device(self.h, _ptr(t), nx, ny, nz, int(label), int(mode)))
+
+    def get_mask(self) -> np.ndarray:
+        """The label mask as a (z, y, x) uint8 array."""
+        out = np.empty(self.shape, np.uint8)
+        self._check(self.lib.mk_get_mask_data(self.h, _ptr(out)))
+        return out
+
+    def origin_slice_async(self, z: int, dev_out, stream: int = 0) -> None:
+        self._check(self.lib.mk_origin_slice_async(self.h, int(z), _ptr(dev_out), stream))
 
     def set_lut(self, label: int, lut: np.ndarray) -> None:
         a = np.ascontiguousarray(lut, dtype=np.float32).reshape(-1, 4)

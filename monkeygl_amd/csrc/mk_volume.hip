@@ -5,6 +5,7 @@
 // pick of DeviceInfo (core/DeviceInfo.cpp:28-70).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -198,43 +199,146 @@ __global__ void __launch_bounds__(256) k_brick_minmax(MkVolView v, int16_t* __re
     }
 }
 
-// One wave per brick: label bits of the brick's own voxels. flags[0..1]
-// accumulate the union (lo 64 bits, bit 0 of flags[2] = label 64) and
-// flags[3] != 0 flags a label above MK_MAX_OBJECTS.
-__global__ void __launch_bounds__(256) k_brick_labels(MkVolView v, uint64_t* __restrict__ lab_lo, uint8_t* __restrict__ lab_hi,
-                                                       uint8_t* __restrict__ blabel, uint32_t nbricks,
-                                                       unsigned long long* __restrict__ flags) {
-    uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6);
-    int lane = threadIdx.x & 63;
-    if (b >= nbricks) return;
-    int bx = b % v.nbx, by = (b / v.nbx) % v.nby, bz = b / (v.nbx * v.nby);
-    uint32_t lo0 = 0, lo1 = 0, hi = 0, bad = 0;
-    for (int o = lane; o < 512; o += 64) {
-        int x = bx * 8 + (o & 7), y = by * 8 + ((o >> 3) & 7), z = bz * 8 + (o >> 6);
-        if (x >= v.nx || y >= v.ny || z >= v.nz) continue;
-        uint32_t l = v.mask[((size_t)b << 9) | o];
+// Label statistics. A wave handles a brick, each lane one 8-voxel x-row of it
+// (lane = z * 8 + y within the brick, the row's bytes at lane * 8); nvalid is
+// the number of the row's voxels inside the volume (0 for rows beyond it).
+__device__ inline void mk_row_labels(uint64_t row, int nvalid, uint32_t& lo0, uint32_t& lo1, uint32_t& hi,
+                                     uint32_t& bad) {
+    for (int k = 0; k < nvalid; ++k) {
+        const uint32_t l = (uint32_t)(row >> (8 * k)) & 0xFFu;
         if (l < 32) lo0 |= 1u << l;
         else if (l < 64) lo1 |= 1u << (l - 32);
         else if (l == 64) hi = 1;
         else bad = 1;
     }
+}
+
+__device__ inline int mk_row_valid(const MkVolView& v, uint32_t b, int lane, int& x0, int& y, int& z) {
+    const int bx = b % v.nbx, by = (b / v.nbx) % v.nby, bz = b / (v.nbx * v.nby);
+    x0 = bx * 8;
+    y = by * 8 + (lane & 7);
+    z = bz * 8 + (lane >> 3);
+    return (y < v.ny && z < v.nz) ? min(8, v.nx - x0) : 0;
+}
+
+// Reduces one brick's row labels over the wave, writes the brick's entries,
+// and ORs them into the wave's running union u[4].
+__device__ inline void mk_brick_label_out(uint32_t b, uint32_t lo0, uint32_t lo1, uint32_t hi, uint32_t bad,
+                                          uint64_t* lab_lo, uint8_t* lab_hi, uint8_t* blabel, uint32_t* u) {
     for (int off = 32; off > 0; off >>= 1) {
         lo0 |= __shfl_xor(lo0, off);
         lo1 |= __shfl_xor(lo1, off);
         hi |= __shfl_xor(hi, off);
         bad |= __shfl_xor(bad, off);
     }
-    if (lane == 0) {
-        uint64_t lo = ((uint64_t)lo1 << 32) | lo0;
+    if ((threadIdx.x & 63) == 0) {
+        const uint64_t lo = ((uint64_t)lo1 << 32) | lo0;
         lab_lo[b] = lo;
         lab_hi[b] = (uint8_t)hi;
         const int n = __popcll(lo) + (hi ? 1 : 0);
         blabel[b] = n == 1 ? (uint8_t)(hi ? 64 : __ffsll((long long)lo) - 1) : (uint8_t)0xFF;
-        if (lo) atomicOr(&flags[0], (unsigned long long)lo);
-        if (hi) atomicOr(&flags[2], 1ull);
-        if (bad) atomicOr(&flags[3], 1ull);
+    }
+    u[0] |= lo0;
+    u[1] |= lo1;
+    u[2] |= hi;
+    u[3] |= bad;
+}
+
+// The workgroup's union into flags with one atomic per word: flags[0] the
+// labels 0..63, flags[2] bit 0 label 64, flags[3] != 0 a label above it.
+__device__ inline void mk_label_union_out(const uint32_t* u, unsigned long long* flags) {
+    __shared__ uint32_t s_u[4][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int i = 0; i < 4; ++i) s_u[wave][i] = u[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t r[4] = {0, 0, 0, 0};
+        for (int w = 0; w < 4; ++w)
+            for (int i = 0; i < 4; ++i) r[i] |= s_u[w][i];
+        const unsigned long long lo = ((unsigned long long)r[1] << 32) | r[0];
+        if (lo) atomicOr(&flags[0], lo);
+        if (r[2]) atomicOr(&flags[2], 1ull);
+        if (r[3]) atomicOr(&flags[3], 1ull);
     }
 }
+
+// Label sets of every brick (grid-stride, one wave per brick).
+__global__ void __launch_bounds__(256) k_brick_labels(MkVolView v, uint64_t* __restrict__ lab_lo, uint8_t* __restrict__ lab_hi,
+                                                       uint8_t* __restrict__ blabel, uint32_t nbricks,
+                                                       unsigned long long* __restrict__ flags) {
+    const int lane = threadIdx.x & 63;
+    uint32_t u[4] = {0, 0, 0, 0};
+    for (uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6); b < nbricks; b += gridDim.x * 4) {
+        int x0, y, z;
+        const int nvalid = mk_row_valid(v, b, lane, x0, y, z);
+        const uint64_t row = *(const uint64_t*)(v.mask + ((size_t)b << 9) + lane * 8);
+        uint32_t lo0 = 0, lo1 = 0, hi = 0, bad = 0;
+        mk_row_labels(row, nvalid, lo0, lo1, hi, bad);
+        mk_brick_label_out(b, lo0, lo1, hi, bad, lab_lo, lab_hi, blabel, u);
+    }
+    mk_label_union_out(u, flags);
+}
+
+// Merges a binary object mask (src, x fastest, the volume's dims) into the
+// bricked label mask in place, with the label statistics of the result.
+// Voxels with src > 0 take label; with update, voxels with src == 0 holding
+// label become 0 (VolumeInfo::AddNewObjectMask / UpdateObjectMask,
+// core/VolumeInfo.cpp:94-133). Without an old mask (has_old == 0) the mask
+// starts at 0. Voxels of the brick padding stay 0.
+__global__ void __launch_bounds__(256) k_mask_merge(MkVolView v, const uint8_t* __restrict__ src, uint8_t* __restrict__ mask,
+                                                     int has_old, uint32_t label, int update, uint64_t* __restrict__ lab_lo,
+                                                     uint8_t* __restrict__ lab_hi, uint8_t* __restrict__ blabel,
+                                                     uint32_t nbricks, unsigned long long* __restrict__ flags) {
+    const int lane = threadIdx.x & 63;
+    uint32_t u[4] = {0, 0, 0, 0};
+    for (uint32_t b = blockIdx.x * 4 + (threadIdx.x >> 6); b < nbricks; b += gridDim.x * 4) {
+        int x0, y, z;
+        const int nvalid = mk_row_valid(v, b, lane, x0, y, z);
+        uint64_t* rp = (uint64_t*)(mask + ((size_t)b << 9) + lane * 8);
+        const uint64_t old = has_old ? *rp : 0ull;
+        uint64_t nw = old;
+        if (nvalid > 0) {
+            const uint8_t* s = src + ((size_t)z * v.ny + y) * v.nx + x0;
+            for (int k = 0; k < nvalid; ++k) {
+                uint32_t m = (uint32_t)(old >> (8 * k)) & 0xFFu;
+                if (s[k] > 0) m = label;
+                else if (update && m == label) m = 0;
+                nw = (nw & ~(0xFFull << (8 * k))) | ((uint64_t)m << (8 * k));
+            }
+        }
+        *rp = nw;
+        uint32_t lo0 = 0, lo1 = 0, hi = 0, bad = 0;
+        mk_row_labels(nw, nvalid, lo0, lo1, hi, bad);
+        mk_brick_label_out(b, lo0, lo1, hi, bad, lab_lo, lab_hi, blabel, u);
+    }
+    mk_label_union_out(u, flags);
+}
+
+// Axial slice z of the volume, -2048 where the mask holds 0 (when there is a
+// mask): the image of HelloMonkey::GetOriginData_pngString
+// (core/HelloMonkey.cpp:261-319).
+__global__ void __launch_bounds__(256) k_origin_slice(MkVolView v, int z, int16_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)v.nx * (uint32_t)v.ny) return;
+    const int x = (int)(i % (uint32_t)v.nx), y = (int)(i / (uint32_t)v.nx);
+    const uint32_t o = mk_voxel_offset(v, x, y, z);
+    int16_t val = v.vox[o];
+    if (v.mask && v.mask[o] == 0) val = -2048;
+    out[i] = val;
+}
+
+// Bricked bytes back to the x-fastest layout; one workgroup per brick.
+__global__ void __launch_bounds__(512) k_unbrickify_u8(const uint8_t* __restrict__ br, uint8_t* __restrict__ lin, int nx,
+                                                       int ny, int nz, int nbx, int nby) {
+    const uint32_t b = blockIdx.x, o = threadIdx.x;
+    const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+    const int x = bx * 8 + (o & 7), y = by * 8 + ((o >> 3) & 7), z = bz * 8 + (o >> 6);
+    if (x < nx && y < ny && z < nz) lin[((size_t)z * ny + y) * nx + x] = br[((size_t)b << 9) | o];
+}
+
+// grid of the grid-stride label kernels
+static uint32_t mk_label_grid(size_t nb) { return (uint32_t)std::min<size_t>((nb + 3) / 4, 16384); }
 
 __global__ void k_fill_u64(uint64_t* p, uint64_t val, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -337,7 +441,7 @@ static mk_status mk_mask_from_device(mk_context* ctx, const uint8_t* d_lin, int 
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, (uint64_t*)d_flags, (uint64_t)0, 4u);
     MkVolView v = ctx->view();
     v.mask = d_mask;
-    hipLaunchKernelGGL(k_brick_labels, dim3((uint32_t)((nb + 3) / 4)), dim3(256), 0, ctx->stream, v, ctx->stats.lab_lo,
+    hipLaunchKernelGGL(k_brick_labels, dim3(mk_label_grid(nb)), dim3(256), 0, ctx->stream, v, ctx->stats.lab_lo,
                        ctx->stats.lab_hi, ctx->stats.blabel, (uint32_t)nb, d_flags);
     MK_HIP(ctx, hipGetLastError());
     unsigned long long flags[4];
@@ -359,7 +463,7 @@ static mk_status mk_mask_from_device(mk_context* ctx, const uint8_t* d_lin, int 
             unsigned long long* d_f2 = nullptr;
             MK_HIP(ctx, hipMalloc((void**)&d_f2, 4 * sizeof(unsigned long long)));
             hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, (uint64_t*)d_f2, (uint64_t)0, 4u);
-            hipLaunchKernelGGL(k_brick_labels, dim3((uint32_t)((nb + 3) / 4)), dim3(256), 0, ctx->stream, v2,
+            hipLaunchKernelGGL(k_brick_labels, dim3(mk_label_grid(nb)), dim3(256), 0, ctx->stream, v2,
                                ctx->stats.lab_lo, ctx->stats.lab_hi, ctx->stats.blabel, (uint32_t)nb, d_f2);
             MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
             (void)hipFree(d_f2);
@@ -418,6 +522,110 @@ extern "C" mk_status mk_clear_mask(mk_context* ctx) {
     ctx->used_hi = false;
     ctx->vol_version++;
     for (auto& t : ctx->skip) t.valid = false;
+    return MK_OK;
+}
+
+// ---- incremental mask edits, slices -----------------------------------------
+
+static mk_status mk_merge_from_device(mk_context* ctx, const uint8_t* d_lin, int nx, int ny, int nz, int label,
+                                      int mode) {
+    if (!ctx->d_vox) return mk_fail(ctx, MK_ERR_STATE, "mask merge before any volume");
+    if (nx != ctx->nx || ny != ctx->ny || nz != ctx->nz)
+        return mk_fail(ctx, MK_ERR_ARG, "mask dims (%d,%d,%d) differ from volume (%d,%d,%d)", nx, ny, nz, ctx->nx,
+                       ctx->ny, ctx->nz);
+    if (label < 0 || label > MK_MAX_OBJECTS)
+        return mk_fail(ctx, MK_ERR_RANGE, "label %d outside 0..%d", label, MK_MAX_OBJECTS);
+    if (mode != MK_MASK_ADD && mode != MK_MASK_UPDATE) return mk_fail(ctx, MK_ERR_ARG, "unknown merge mode %d", mode);
+    if (mode == MK_MASK_UPDATE && !ctx->d_mask) return mk_fail(ctx, MK_ERR_STATE, "mask update without a mask");
+    const size_t nb = ctx->nbricks();
+    uint8_t* d_mask = ctx->d_mask;
+    const int has_old = d_mask != nullptr;
+    if (!has_old) MK_HIP(ctx, hipMalloc((void**)&d_mask, nb * 512));
+    unsigned long long* d_flags = nullptr;
+    hipError_t e = hipMalloc((void**)&d_flags, 4 * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        if (!has_old) (void)hipFree(d_mask);
+        return mk_hip_fail(ctx, e, "hipMalloc");
+    }
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, (uint64_t*)d_flags, (uint64_t)0, 4u);
+    MkVolView v = ctx->view();
+    hipLaunchKernelGGL(k_mask_merge, dim3(mk_label_grid(nb)), dim3(256), 0, ctx->stream, v, d_lin, d_mask, has_old,
+                       (uint32_t)label, mode == MK_MASK_UPDATE ? 1 : 0, ctx->stats.lab_lo, ctx->stats.lab_hi,
+                       ctx->stats.blabel, (uint32_t)nb, d_flags);
+    unsigned long long flags[4] = {0, 0, 0, 0};
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_flags);
+    if (e != hipSuccess) {
+        if (!has_old) (void)hipFree(d_mask);
+        return mk_hip_fail(ctx, e, "mask merge");
+    }
+    ctx->d_mask = d_mask;
+    ctx->used_lo = flags[0];
+    ctx->used_hi = flags[2] != 0;
+    ctx->vol_version++;
+    for (auto& t : ctx->skip) t.valid = false;
+    return MK_OK;
+}
+
+extern "C" mk_status mk_merge_object_mask(mk_context* ctx, const uint8_t* host, int nx, int ny, int nz, int label,
+                                          int mode) {
+    if (!ctx) return MK_ERR_ARG;
+    if (!host) return mk_fail(ctx, MK_ERR_ARG, "mk_merge_object_mask: null data");
+    if (nx <= 0 || ny <= 0 || nz <= 0) return mk_fail(ctx, MK_ERR_ARG, "mask dims must be positive");
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)nx * ny * nz;
+    void* tmp = nullptr;
+    MK_HIP(ctx, hipMalloc(&tmp, bytes));
+    hipError_t e = hipMemcpyAsync(tmp, host, bytes, hipMemcpyHostToDevice, ctx->stream);
+    mk_status st = e == hipSuccess ? mk_merge_from_device(ctx, (const uint8_t*)tmp, nx, ny, nz, label, mode)
+                                   : mk_hip_fail(ctx, e, "hipMemcpyAsync");
+    e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp);
+    if (st) return st;
+    if (e != hipSuccess) return mk_hip_fail(ctx, e, "hipStreamSynchronize");
+    return MK_OK;
+}
+
+extern "C" mk_status mk_merge_object_mask_device(mk_context* ctx, const uint8_t* dev, int nx, int ny, int nz,
+                                                 int label, int mode) {
+    if (!ctx) return MK_ERR_ARG;
+    if (!dev) return mk_fail(ctx, MK_ERR_ARG, "mk_merge_object_mask_device: null data");
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    return mk_merge_from_device(ctx, dev, nx, ny, nz, label, mode);
+}
+
+extern "C" mk_status mk_get_mask_data(mk_context* ctx, uint8_t* host) {
+    if (!ctx) return MK_ERR_ARG;
+    if (!host) return mk_fail(ctx, MK_ERR_ARG, "mk_get_mask_data: null buffer");
+    if (!ctx->d_mask) return mk_fail(ctx, MK_ERR_STATE, "no mask");
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)ctx->nx * ctx->ny * ctx->nz;
+    void* tmp = nullptr;
+    MK_HIP(ctx, hipMalloc(&tmp, bytes));
+    hipLaunchKernelGGL(k_unbrickify_u8, dim3((uint32_t)ctx->nbricks()), dim3(512), 0, ctx->stream, ctx->d_mask,
+                       (uint8_t*)tmp, ctx->nx, ctx->ny, ctx->nz, ctx->nbx, ctx->nby);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, tmp, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return mk_hip_fail(ctx, e, "mk_get_mask_data");
+    return MK_OK;
+}
+
+extern "C" mk_status mk_origin_slice_async(mk_context* ctx, int z, int16_t* dev_out, void* stream) {
+    if (!ctx) return MK_ERR_ARG;
+    if (!dev_out) return mk_fail(ctx, MK_ERR_ARG, "mk_origin_slice_async: null output");
+    if (!ctx->d_vox) return mk_fail(ctx, MK_ERR_STATE, "no volume");
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    z = z < 0 ? 0 : (z >= ctx->nz ? ctx->nz - 1 : z);
+    hipStream_t s = (hipStream_t)stream;
+    mk_status st = mk_order_after_context(ctx, s);
+    if (st) return st;
+    const uint32_t n = (uint32_t)ctx->nx * (uint32_t)ctx->ny;
+    hipLaunchKernelGGL(k_origin_slice, dim3((n + 255) / 256), dim3(256), 0, s, ctx->view(), z, dev_out);
+    MK_HIP(ctx, hipGetLastError());
     return MK_OK;
 }
 

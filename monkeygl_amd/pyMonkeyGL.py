@@ -46,7 +46,7 @@ import numpy as np
 from . import cpr as _cpr
 from . import mpr as _mpr
 from . import params as _P
-from ._capi import (MK_NUM_LABELS, MK_SAMPLE_LINEAR, MkError)
+from ._capi import (MK_MASK_ADD, MK_MASK_UPDATE, MK_NUM_LABELS, MK_SAMPLE_LINEAR, MkError)
 
 MAXOBJECTCOUNT = 64
 
@@ -283,7 +283,7 @@ class HelloMonkey:
         self._log_level = LogLevel.LogLevelInfo
         self._dims = None                   # (W, H, D)
         self._vol = None                    # int16, x fastest
-        self._mask = None                   # uint8, x fastest, or None
+        self._has_mask = False              # the label mask lives on the device only
         self._spacing = (1.0, 1.0, 1.0)
         self._dir = ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, -1.0))
         self._origin = (0.0, 0.0, 0.0)
@@ -299,7 +299,7 @@ class HelloMonkey:
         self._layers: dict = {}
         self._annotations: list = []
         self._plane_label = 0
-        self._dirty_vol = self._dirty_mask = self._dirty_lut = False
+        self._dirty_vol = self._dirty_lut = False
         self._encoders: dict = {}
         self._series_renderer = None
 
@@ -314,15 +314,8 @@ class HelloMonkey:
             raise RuntimeError("no volume: call SetVolumeArray first")
         W, H, D = self._dims
         if self._dirty_vol:
-            c.set_volume(self._vol.reshape(D, H, W))
+            c.set_volume(self._vol.reshape(D, H, W))   # drops the device mask, as SetVolumeData does
             self._dirty_vol = False
-            self._dirty_mask = True
-        if self._dirty_mask:
-            if self._mask is None:
-                c.clear_mask()
-            else:
-                c.set_mask(self._mask.reshape(D, H, W))
-            self._dirty_mask = False
         if self._dirty_lut:
             # DataManager::UpdateTransferFunctions (core/DataManager.cpp:372-388)
             for label, info in self._objects.items():
@@ -375,11 +368,11 @@ class HelloMonkey:
         self._objects = {}
         self._dims = tuple(int(s) for s in a.shape)
         self._vol = _raw_memory(a).astype(np.int16)
-        self._mask = None
+        self._has_mask = False
         self._mpr = _mpr.MPRInfo(self._dims, self._spacing)
         self._active = 0
         self._objects[0] = _ObjectInfo()
-        self._dirty_vol = self._dirty_mask = True
+        self._dirty_vol = True
         return True
 
     def GetVolumeArray(self):
@@ -387,7 +380,8 @@ class HelloMonkey:
         return self._vol.reshape(W, H, D).copy()
 
     def AddNewObjectMaskArray(self, npData) -> int:
-        """DataManager::AddNewObjectMask (core/DataManager.cpp:280-315)."""
+        """DataManager::AddNewObjectMask (core/DataManager.cpp:280-315); the
+        merge runs on the device (mk_merge_object_mask, MK_MASK_ADD)."""
         a = _raw_memory(np.asarray(npData)).astype(np.uint8)
         if len(self._objects) <= 0:
             self._objects[0] = _ObjectInfo()
@@ -402,25 +396,33 @@ class HelloMonkey:
                 label += 1
         if self._vol is None or a.size < self._vol.size:
             return 0
-        if self._mask is None:
-            self._mask = np.zeros(self._vol.size, np.uint8)
-        sel = a[:self._vol.size] > 0
-        self._mask[sel] = label
+        W, H, D = self._dims
+        self._device().merge_mask(a[:self._vol.size].reshape(D, H, W), label, MK_MASK_ADD)
+        self._has_mask = True
         self._objects[label] = copy.deepcopy(self._objects[self._active])
         self._active = label
-        self._dirty_mask = True
         return label
 
     def UpdateMaskArray(self, npData, nLabel: int) -> bool:
-        """DataManager::UpdateObjectMask (core/DataManager.cpp:324-335, VolumeInfo.cpp:118-133)."""
+        """DataManager::UpdateObjectMask (core/DataManager.cpp:324-335,
+        VolumeInfo.cpp:118-133) as a device merge (MK_MASK_UPDATE)."""
         nLabel = int(nLabel)
-        if nLabel not in self._objects or self._mask is None:
+        if nLabel not in self._objects or not self._has_mask:
             return False
-        a = _raw_memory(np.asarray(npData)).astype(np.uint8)[:self._mask.size]
-        self._mask[a > 0] = nLabel
-        self._mask[(a == 0) & (self._mask == nLabel)] = 0
-        self._dirty_mask = True
+        a = _raw_memory(np.asarray(npData)).astype(np.uint8)
+        if a.size < self._vol.size:
+            return False
+        W, H, D = self._dims
+        self._device().merge_mask(a[:self._vol.size].reshape(D, H, W), nLabel, MK_MASK_UPDATE)
         return True
+
+    def GetMaskArray(self):
+        """The merged label mask (W, H, D) view of an x-fastest buffer, like
+        GetVolumeArray; None without a mask (DataManager::GetMaskData)."""
+        if not self._has_mask:
+            return None
+        W, H, D = self._dims
+        return self._device().get_mask().reshape(W, H, D)
 
     def SetSpacing(self, x: float, y: float, z: float) -> None:
         self._spacing = (float(x), float(y), float(z))
@@ -772,11 +774,9 @@ class HelloMonkey:
         import torch
 
         W, H, D = self._dims
-        s = min(max(int(slice_), 0), D - 1)
-        sl = self._vol[W * H * s:W * H * (s + 1)].copy()
-        if self._mask is not None:
-            sl[self._mask[W * H * s:W * H * (s + 1)] == 0] = -2048
-        dev = torch.tensor(sl, device="cuda")
+        c = self._device()
+        dev = torch.empty(W * H, dtype=torch.int16, device="cuda")
+        c.origin_slice_async(int(slice_), dev, stream=torch.cuda.current_stream().cuda_stream)
         return self._int16_png_b64(dev, W, H)
 
     # -- batch series (core/HelloMonkey.cpp:415-420, 614-619) ---------------------

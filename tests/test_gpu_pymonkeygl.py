@@ -75,13 +75,16 @@ def test_masks_labels_and_render_types(volume):
     hm.SetVRWWWL(800, 300, l2)
     hm.SetObjectAlpha(0.4, 0)
     hm.Anterior()
+    # the merged mask (the later object wins where they overlap), read back from the device
+    merged = np.where(m == 2, 2, np.where(m == 1, 1, 0)).astype(np.uint8)
+    assert np.array_equal(hm.GetMaskArray().reshape(-1), merged.reshape(-1))
     img = _vr_frame(hm)
     p = hm._render_params(128, 128)
-    ref = O.render(p, volume, _luts(hm), mask_zyx=hm._mask.reshape(volume.shape))
+    ref = O.render(p, volume, _luts(hm), mask_zyx=merged)
     assert _frac(img, ref) >= 0.999
     hm.SetRenderType(mk.RenderTypeMIP)
     mip = _vr_frame(hm)
-    ref = O.render(hm._render_params(128, 128), volume, _luts(hm), mask_zyx=hm._mask.reshape(volume.shape))
+    ref = O.render(hm._render_params(128, 128), volume, _luts(hm), mask_zyx=merged)
     assert _frac(mip, ref) >= 0.999
 
 

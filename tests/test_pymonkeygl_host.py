@@ -6,6 +6,7 @@ import pytest
 
 from monkeygl_amd import params as P
 from monkeygl_amd import pyMonkeyGL as mk
+from monkeygl_amd._capi import MK_MASK_ADD, MK_MASK_UPDATE
 
 
 def test_enum_values_and_exports():
@@ -58,38 +59,60 @@ def test_volume_read_as_raw_memory():
     assert np.array_equal(hm.GetVolumeArray().reshape(-1), zyx.reshape(-1))
 
 
-def test_label_allocation_and_mask_rules():
+class _Recorder:
+    """Stands in for the device context: records the mask merges the
+    module asks for (the merge itself runs on the GPU and is tested there)."""
+
+    def __init__(self):
+        self.merges = []
+
+    def merge_mask(self, arr, label, mode):
+        self.merges.append((arr.copy(), label, mode))
+
+
+def _recorded(hm):
+    rec = _Recorder()
+    hm._device = lambda: rec
+    return rec
+
+
+def test_label_allocation_and_mask_merges():
     zyx = _vol()
     hm = mk.HelloMonkey()
     hm.SetVolumeArray(zyx.swapaxes(2, 0))
+    rec = _recorded(hm)
     m1 = np.zeros_like(zyx, dtype=np.uint8)
     m1[2:6, 3:9, 4:10] = 1
     m2 = np.zeros_like(m1)
     m2[5:9, 5:12, 8:14] = 7
     hm.SetVRWWWL(300, 20)
+    assert not hm.UpdateMaskArray(m1.swapaxes(2, 0), 0)    # no mask yet
     assert hm.AddNewObjectMaskArray(m1.swapaxes(2, 0)) == 1
     # the new label copies the active object's settings
     assert hm._objects[1].ww == np.float32(300)
     assert hm.AddNewObjectMaskArray(m2.swapaxes(2, 0)) == 2
-    mask = hm._mask.reshape(zyx.shape)
-    assert (mask[m2 > 0] == 2).all()
-    assert (mask[(m1 > 0) & (m2 == 0)] == 1).all()
-    assert (mask[(m1 == 0) & (m2 == 0)] == 0).all()
-    # UpdateMaskArray: set where > 0, clear this label where 0
+    # the (W, H, D) views reach the device as the x-fastest buffers, add mode
+    assert [(l, md) for _, l, md in rec.merges] == [(1, MK_MASK_ADD), (2, MK_MASK_ADD)]
+    assert np.array_equal(rec.merges[0][0], m1) and np.array_equal(rec.merges[1][0], m2)
     m3 = np.zeros_like(m1)
     m3[0:2, 0:2, 0:2] = 1
     assert hm.UpdateMaskArray(m3.swapaxes(2, 0), 1)
-    mask = hm._mask.reshape(zyx.shape)
-    assert (mask == 1).sum() == 8 and (mask[m2 > 0] == 2).all()
+    assert rec.merges[-1][1:] == (1, MK_MASK_UPDATE) and np.array_equal(rec.merges[-1][0], m3)
     assert not hm.UpdateMaskArray(m3.swapaxes(2, 0), 9)   # no such object
+    assert not hm.UpdateMaskArray(m3[:2].swapaxes(2, 0), 1)   # smaller than the volume
+    assert hm.AddNewObjectMaskArray(m3[:2].swapaxes(2, 0)) == 0
     # a gap in the labels is filled first
     del hm._objects[1]
     assert hm.AddNewObjectMaskArray(m3.swapaxes(2, 0)) == 1
+    # a new volume drops the mask and the objects
+    hm.SetVolumeArray(zyx.swapaxes(2, 0))
+    assert not hm._has_mask and list(hm._objects) == [0]
 
 
 def test_label_capacity():
     hm = mk.HelloMonkey()
     hm.SetVolumeArray(_vol(8, 8, 8).swapaxes(2, 0))
+    _recorded(hm)
     one = np.ones((8, 8, 8), np.uint8)
     labels = [hm.AddNewObjectMaskArray(one) for _ in range(65)]
     assert labels[:64] == list(range(1, 65)) and labels[64] == 0
