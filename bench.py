@@ -31,7 +31,9 @@ The JSON line carries:
   * series: a GetBatchData series of parallel MIP slabs through the batcher
     against one plane launch per image;
   * mask_ingest: an object-mask edit merged on the device against the
-    reference's host merge + full mask upload.
+    reference's host merge + full mask upload;
+  * orientation_figure: the VR overlay's figure merged on the device against
+    the reference's host rasteriser.
 """
 from __future__ import annotations
 
@@ -264,6 +266,57 @@ def mask_ingest_section(spec, vol, mask):
            "device_merge_from_device_ms": round(t_dev * 1e3, 2), "host_merge_and_upload_ms": round(t_ref * 1e3, 2),
            "same_mask": same}
     log(f"mask ingest: {out}")
+    return out
+
+
+def overlay_section(ctx, size: int, with_cpu: bool):
+    """The orientation figure (a 35,124-facet synthetic figure, the size of the
+    reference's resources/human.stl) merged into a size^2 VR frame on the
+    device, against the reference's host rasteriser (MergeOrientationBox over
+    Methods::FillHoleInImage_Ch1, oracle/_ref) on the same frame."""
+    import ctypes as C
+    import struct
+
+    import torch
+
+    from monkeygl_amd import params as P
+    from monkeygl_amd import workloads as W
+
+    stl = W.stl_bytes(W.figure(0, fine=True))
+    n = ctx.set_orientation_mesh(stl)
+    info = P.RenderInfo()
+    info.rotate(30.0, 20.0)
+    rot = np.asarray(info.rot, np.float32)
+    ws = torch.empty(ctx.orientation_workspace_bytes(), dtype=torch.uint8, device="cuda")
+    frame = np.random.default_rng(5).integers(0, 256, (size, size, 3), dtype=np.uint8)
+    dev = torch.tensor(frame, device="cuda")
+    stream = torch.cuda.current_stream()
+    for _ in range(2):
+        ctx.merge_orientation_box_async(rot, dev, size, size, ws, stream=stream.cuda_stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(20):
+        ctx.merge_orientation_box_async(rot, dev, size, size, ws, stream=stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    out = {"facets": n, "image": [size, size], "gpu_ms": round(e0.elapsed_time(e1) / 20, 4)}
+    ref_lib = ROOT / "oracle" / "_ref" / "libmkref.so"
+    if with_cpu and ref_lib.exists():
+        L = C.CDLL(str(ref_lib))
+        L.ref_stl_normalize.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.ref_orientation_box.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        body = np.frombuffer(stl, np.uint8, 50 * n, 84).copy()
+        fac = np.zeros(n * 9, np.float32)
+        L.ref_stl_normalize(body.ctypes.data, n, fac.ctypes.data)
+        ref = frame.copy()
+        t = time.perf_counter()
+        for _ in range(5):
+            L.ref_orientation_box(fac.ctypes.data, n, rot.ctypes.data, ref.ctypes.data, size, size)
+        out["reference_cpu_ms"] = round((time.perf_counter() - t) / 5 * 1e3, 3)
+        out["reference_cpu_cores"] = 1
+        out["identical"] = bool(np.array_equal(dev.cpu().numpy(), ref))
+    ctx.set_orientation_mesh(None)
+    log(f"orientation figure: {out}")
     return out
 
 
@@ -556,6 +609,7 @@ def main() -> int:
 
     series = series_section(ctx, spec)
     mask_ingest = mask_ingest_section(spec, vol, mask)
+    overlay = overlay_section(ctx, W_, not args.no_cpu)
     served = None if args.no_serve else serve_section(args, ctx, spec, vol, mask, luts, labels)
 
     line = {
@@ -586,6 +640,7 @@ def main() -> int:
         "encode": encode,
         "series": series,
         "mask_ingest": mask_ingest,
+        "orientation_figure": overlay,
         "served": served,
     }
     print(json.dumps(line), flush=True)

@@ -259,6 +259,29 @@ mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, const float
                          float opacity_scale, const float* dev_grad_rgb, double* dev_grad_lut,
                          double* dev_grad_scale, void* stream);
 
+/* ---- VR overlay: the orientation figure --------------------------------- */
+
+/* The figure drawn into the corner of VR images (MarchingCube::LoadBinarySTL,
+ * core/MarchingCube.cpp:306-365): n_facets facets of a binary STL body (the
+ * file after its 84-byte header: 50 bytes per facet, normal and three
+ * vertices as float32, a 2-byte attribute), normalised as the reference does:
+ * centred on the bounding box, divided by the larger of the x and z extents,
+ * z negated. n_facets = 0 removes the figure. */
+mk_status mk_set_orientation_mesh(mk_context* ctx, const uint8_t* stl_facets, int n_facets);
+
+/* Device workspace mk_merge_orientation_box_async needs for ctx's figure. */
+uint64_t mk_orientation_workspace_bytes(mk_context* ctx);
+
+/* VRImageLayer::MergeOrientationBox (core/VRImageLayer.cpp:104-194) on an
+ * RGB24 device image: the figure seen through rotate9 (RenderInfo's rotate
+ * matrix, 9 floats, row-major), rasterised at 512x512 as
+ * Methods::FillHoleInImage_Ch1 does (core/Methods.cpp:308-368), every 8th
+ * pixel lighting the bottom-right 64x64 corner. No-op when width or height is
+ * at most 64 or no figure is set. The workspace (8-byte aligned) holds the
+ * per-call state, so calls with their own workspaces may overlap. */
+mk_status mk_merge_orientation_box_async(mk_context* ctx, const float* rotate9, uint8_t* dev_rgb, int width,
+                                         int height, void* dev_workspace, uint64_t workspace_bytes, void* stream);
+
 /* ---- output encoding: PNG + base64 on the device ------------------------- */
 /* Replaces the host encoding of served images: fpng_encode_image_to_memory
  * of the RGB VR frame (core/HelloMonkey.cpp:333-365) and of an int16 plane

@@ -30,7 +30,8 @@ SYMBOLS = [
     "mk_merge_object_mask_device", "mk_get_mask_data", "mk_origin_slice_async", "mk_set_transfer_func",
     "mk_render", "mk_render_async", "mk_render_footprint", "mk_render_plane", "mk_render_plane_async",
     "mk_render_cpr", "mk_render_cpr_async", "mk_marching_cubes", "mk_marching_cubes_async",
-    "mk_tf_forward", "mk_tf_backward", "mk_png_max_bytes", "mk_png_workspace_bytes", "mk_png_encode_async",
+    "mk_tf_forward", "mk_tf_backward", "mk_set_orientation_mesh", "mk_orientation_workspace_bytes",
+    "mk_merge_orientation_box_async", "mk_png_max_bytes", "mk_png_workspace_bytes", "mk_png_encode_async",
     "mk_base64_max_bytes", "mk_base64_encode_async", "mk_batcher_create", "mk_batcher_destroy", "mk_batcher_last_error",
     "mk_batch_submit", "mk_batch_query", "mk_batch_wait",
 ]
@@ -110,6 +111,9 @@ def load() -> C.CDLL:
         "mk_copy_mask_data": (st, [vp, vp, i, i, i]),
         "mk_copy_mask_data_device": (st, [vp, vp, i, i, i]),
         "mk_clear_mask": (st, [vp]),
+        "mk_set_orientation_mesh": (st, [vp, vp, i]),
+        "mk_orientation_workspace_bytes": (C.c_uint64, [vp]),
+        "mk_merge_orientation_box_async": (st, [vp, vp, vp, i, i, vp, C.c_uint64, vp]),
         "mk_merge_object_mask": (st, [vp, vp, i, i, i, i, i]),
         "mk_merge_object_mask_device": (st, [vp, vp, i, i, i, i, i]),
         "mk_get_mask_data": (st, [vp, vp]),
@@ -226,6 +230,29 @@ class Context:
         out = np.empty(self.shape, np.uint8)
         self._check(self.lib.mk_get_mask_data(self.h, _ptr(out)))
         return out
+
+    def set_orientation_mesh(self, stl: bytes | None) -> int:
+        """A binary STL file's bytes (or None to remove the figure); returns
+        the facet count."""
+        if not stl:
+            self._check(self.lib.mk_set_orientation_mesh(self.h, None, 0))
+            return 0
+        n = int.from_bytes(stl[80:84], "little")
+        if n <= 0 or len(stl) < 84 + 50 * n:
+            raise ValueError("not a binary STL")
+        body = np.frombuffer(stl, np.uint8, 50 * n, 84)
+        self._check(self.lib.mk_set_orientation_mesh(self.h, _ptr(np.ascontiguousarray(body)), n))
+        return n
+
+    def orientation_workspace_bytes(self) -> int:
+        return int(self.lib.mk_orientation_workspace_bytes(self.h))
+
+    def merge_orientation_box_async(self, rotate9, dev_rgb, width: int, height: int, dev_workspace,
+                                    stream: int = 0) -> None:
+        m = np.ascontiguousarray(rotate9, dtype=np.float32).reshape(9)
+        ws = dev_workspace.numel() * dev_workspace.element_size()
+        self._check(self.lib.mk_merge_orientation_box_async(self.h, _ptr(m), _ptr(dev_rgb), width, height,
+                                                            _ptr(dev_workspace), ws, stream))
 
     def origin_slice_async(self, z: int, dev_out, stream: int = 0) -> None:
         self._check(self.lib.mk_origin_slice_async(self.h, int(z), _ptr(dev_out), stream))

@@ -34,10 +34,10 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-I", str(ROOT / "include"),
 ]
 # files whose arithmetic must match the host restatement bit for bit
-NO_CONTRACT = {"mk_plane.hip", "mk_mc.hip", "mk_batch_plane.hip"}
+NO_CONTRACT = {"mk_plane.hip", "mk_mc.hip", "mk_batch_plane.hip", "mk_overlay.hip"}
 
 HIP_SOURCES = ["mk_volume.hip", "mk_render.hip", "mk_plane.hip", "mk_mc.hip", "mk_fit.hip", "mk_batch.hip",
-               "mk_batch_plane.hip", "mk_encode.hip"]
+               "mk_batch_plane.hip", "mk_encode.hip", "mk_overlay.hip"]
 
 
 def _run(cmd: list[str]) -> None:

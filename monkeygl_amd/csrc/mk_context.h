@@ -89,6 +89,10 @@ struct mk_context {
     size_t mc_scratch_cap = 0;
     void* d_mc_total = nullptr;
 
+    // orientation figure of the VR overlay: normalised facets (9 floats each)
+    float* d_orient = nullptr;
+    int orient_n = 0;
+
     // staging for synchronous host-output calls
     void* d_stage = nullptr;
     size_t stage_cap = 0;

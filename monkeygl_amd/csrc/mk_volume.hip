@@ -124,6 +124,7 @@ extern "C" void mk_context_destroy(mk_context* ctx) {
     if (ctx->d_mc_total) (void)hipFree(ctx->d_mc_total);
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
     if (ctx->d_stage2) (void)hipFree(ctx->d_stage2);
+    if (ctx->d_orient) (void)hipFree(ctx->d_orient);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

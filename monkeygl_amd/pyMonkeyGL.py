@@ -302,6 +302,15 @@ class HelloMonkey:
         self._dirty_vol = self._dirty_lut = False
         self._encoders: dict = {}
         self._series_renderer = None
+        # the orientation figure: MarchingCube loads ./resources/human.stl at
+        # construction (core/MarchingCube.cpp:297-298); without the file there
+        # is no figure
+        try:
+            with open("./resources/human.stl", "rb") as f:
+                self._orient_stl = f.read()
+        except OSError:
+            self._orient_stl = None
+        self._orient_ws = None
 
     # -- device sync ----------------------------------------------------------
     def _device(self):
@@ -309,6 +318,11 @@ class HelloMonkey:
 
         if self._ctx is None:
             self._ctx = Context(0)
+            if self._orient_stl:
+                try:
+                    self._ctx.set_orientation_mesh(self._orient_stl)
+                except ValueError:
+                    self._orient_stl = None   # not a binary STL: no figure, as the reference logs and goes on
         c = self._ctx
         if self._vol is None:
             raise RuntimeError("no volume: call SetVolumeArray first")
@@ -678,18 +692,24 @@ class HelloMonkey:
         w, h = self._ri.width, self._ri.height
         p = self._render_params(w, h)
         out = torch.empty(w * h * 3, dtype=torch.uint8, device="cuda")
-        c.render_async(p, out, stream=torch.cuda.current_stream().cuda_stream)
+        st = torch.cuda.current_stream().cuda_stream
+        c.render_async(p, out, stream=st)
+        if self._orient_stl:
+            # VRImageLayer::GetRGBData ends with MergeOrientationBox (core/VRImageLayer.cpp:99)
+            nb = c.orientation_workspace_bytes()
+            if self._orient_ws is None or self._orient_ws.numel() < nb:
+                self._orient_ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+            c.merge_orientation_box_async(self._ri.rot, out, w, h, self._orient_ws, stream=st)
         return out, w, h
 
     def GetVRArray(self, nWidth: int, nHeight: int):
         if not self._image_layer():
             raise RuntimeError("the image layer is disabled")
-        c = self._device()
         w, h = self._ri.width, self._ri.height
         if nWidth * nHeight != w * h:
             raise ValueError(f"GetVRArray reads the VR frame ({w}x{h}); ask for a shape of the same size")
-        img = c.render(self._render_params(w, h))
-        return img.reshape(-1).reshape(3, nWidth, nHeight).copy()
+        out, w, h = self._vr_device()
+        return out.cpu().numpy().reshape(3, nWidth, nHeight)
 
     def _vr_png(self) -> bytes:
         import torch

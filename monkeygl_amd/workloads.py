@@ -15,6 +15,7 @@ is false).
 """
 from __future__ import annotations
 
+import struct
 from dataclasses import dataclass
 
 import numpy as np
@@ -253,3 +254,57 @@ def serving_requests(scene: list, n: int, qps: float, seed: int = 0, sizes=SERVE
             reqs.append(Request(KIND_CPR, vi, cpr_width=w, cpr_points=pts, cpr_dirs=dirs, cpr_sampling=samp,
                                 t_arrival=t, tag=(i, "cpr")))
     return reqs
+
+
+# ---- orientation figures (binary STL) ------------------------------------------
+# The reference ships its figure as resources/human.stl (34,660 facets) and
+# loads it at start-up; these synthetic figures stand in for it: a body of UV
+# ellipsoids whose pole rings hold zero-area facets, plus two large facets.
+
+def uv_ellipsoid(center, radii, nu: int = 18, nv: int = 12) -> np.ndarray:
+    """(n, 3, 3) facets of a UV ellipsoid; the pole rows are degenerate
+    triangles (two vertices at the pole), as UV spheres have."""
+    c, r = np.asarray(center, np.float64), np.asarray(radii, np.float64)
+    th = np.linspace(0, np.pi, nv + 1)
+    ph = np.linspace(0, 2 * np.pi, nu + 1)
+    P = np.stack([np.outer(np.sin(th), np.cos(ph)), np.outer(np.sin(th), np.sin(ph)),
+                  np.outer(np.cos(th), np.ones_like(ph))], -1) * r + c
+    out = []
+    for i in range(nv):
+        for j in range(nu):
+            a, b, cc, d = P[i, j], P[i + 1, j], P[i + 1, j + 1], P[i, j + 1]
+            out.append((a, b, cc))
+            out.append((a, cc, d))
+    return np.asarray(out, np.float32)
+
+
+def figure(seed: int = 0, fine: bool = False) -> np.ndarray:
+    """A seeded body of UV ellipsoids (head, torso, arms, legs), 2,722 facets;
+    fine=True tessellates it into ~35,000, the size of the reference's figure."""
+    k = 3.6 if fine else 1.0
+    rng = np.random.default_rng(seed)
+
+    def part(c, r, nu, nv):
+        return uv_ellipsoid(c, r, int(round(nu * k)), int(round(nv * k)))
+
+    parts = [part((0, 0, 160), (12, 11, 14), 20, 14),          # head
+             part((0, 0, 100), (24, 14, 45), 28, 18),          # torso
+             part((-34, 0, 105), (6, 6, 38), 12, 10),          # arms
+             part((34, 0, 105), (6, 6, 38), 12, 10),
+             part((-12, 0, 30), (8, 8, 45), 14, 12),           # legs
+             part((12, 0, 30), (8, 8, 45), 14, 12)]
+    f = np.concatenate(parts)
+    f += rng.normal(0, 0.05, f.shape).astype(np.float32)   # break exact symmetries
+    big = np.array([[[-40, 20, -10], [40, 20, -10], [0, 20, 60]],
+                    [[-40, -25, 190], [40, -25, 190], [0, -25, -15]]], np.float32)
+    return np.concatenate([f, big])
+
+
+def stl_bytes(facets: np.ndarray) -> bytes:
+    f = np.asarray(facets, np.float32).reshape(-1, 3, 3)
+    out = bytearray(b"synthetic figure".ljust(80, b" "))
+    out += struct.pack("<I", f.shape[0])
+    n = np.cross(f[:, 1] - f[:, 0], f[:, 2] - f[:, 0])
+    for k in range(f.shape[0]):
+        out += np.asarray(n[k], np.float32).tobytes() + f[k].tobytes() + b"\0\0"
+    return bytes(out)

@@ -15,8 +15,14 @@
 //   (core/PlaneInfo.h:37-45).
 //   fpng_encode_image_to_memory (core/fpng/fpng.cpp): the reference's PNG
 //   encoder, as the CPU baseline of the device encoder.
+//   The orientation figure of VR images: Methods::FillHoleInImage_Ch1
+//   (core/Methods.cpp:308-368) under the few lines of
+//   VRImageLayer::MergeOrientationBox / GetMeshPoints and
+//   MarchingCube::LoadBinarySTL that call it (restated below: their files
+//   do not build here).
 // Used only to write golden vectors (tests/golden/make_golden.py) and to
 // cross-check the restatements where the reference is present.
+#include <cfloat>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -166,6 +172,95 @@ unsigned long long ref_fpng_encode(const unsigned char* img, int w, int h, int c
     if (buf.size() > cap) return 0;
     std::memcpy(out, buf.data(), buf.size());
     return buf.size();
+}
+
+// MarchingCube::LoadBinarySTL's normalisation (core/MarchingCube.cpp:306-365;
+// MarchingCube.cpp itself links the reference's logger, absent here) over the
+// reference's Facet3D / Point3f: body is the STL after its 84-byte header.
+void ref_stl_normalize(const unsigned char* body, int n, float* out9) {
+    std::vector<Facet3D> facets;
+    float xmin = FLT_MAX, xmax = FLT_MIN;
+    float ymin = FLT_MAX, ymax = FLT_MIN;
+    float zmin = FLT_MAX, zmax = FLT_MIN;
+    for (int i = 0; i < n; i++) {
+        Facet3D facet;
+        memcpy(&facet, body + (size_t)i * 50, 50);
+        facets.push_back(facet);
+        xmin = min(xmin, min(facet.v1.x(), min(facet.v2.x(), facet.v3.x())));
+        xmax = max(xmax, max(facet.v1.x(), max(facet.v2.x(), facet.v3.x())));
+        ymin = min(ymin, min(facet.v1.y(), min(facet.v2.y(), facet.v3.y())));
+        ymax = max(ymax, max(facet.v1.y(), max(facet.v2.y(), facet.v3.y())));
+        zmin = min(zmin, min(facet.v1.z(), min(facet.v2.z(), facet.v3.z())));
+        zmax = max(zmax, max(facet.v1.z(), max(facet.v2.z(), facet.v3.z())));
+    }
+    float xlen = xmax - xmin;
+    float zlen = zmax - zmin;
+    float maxlen = max(xlen, zlen);
+    Point3f ptCenter((xmin + xmax) / 2, (ymin + ymax) / 2, (zmin + zmax) / 2);
+    for (size_t i = 0; i < facets.size(); i++) {
+        Facet3D& facet = facets[i];
+        Point3f* v[3] = {&facet.v1, &facet.v2, &facet.v3};
+        for (int k = 0; k < 3; ++k) {
+            *v[k] -= ptCenter;
+            *v[k] /= maxlen;
+            v[k]->SetZ(-v[k]->z());
+            out9[i * 9 + k * 3 + 0] = v[k]->x();
+            out9[i * 9 + k * 3 + 1] = v[k]->y();
+            out9[i * 9 + k * 3 + 2] = v[k]->z();
+        }
+    }
+}
+
+// VRImageLayer::MergeOrientationBox + GetMeshPoints + the file's
+// TransferPoint2D (core/VRImageLayer.cpp:104-194; VRImageLayer.cpp pulls in
+// the CUDA data manager, so its few lines are restated here) over the
+// reference's Methods::GetTransferPointf, Direction3f, Point2f, RGBA and
+// Methods::FillHoleInImage_Ch1 (core/Methods.cpp:308-368), compiled from its
+// source. facets9: normalised facets; rgb: the VR image, modified in place.
+static Point2f ref_transfer_point_2d(Point2f pt, int nWidth, int nHeight) {
+    float r = nHeight;
+    Point2f ptOut = pt;
+    ptOut *= r;
+    ptOut += Point2f(nWidth / 2.0, nHeight / 2.0);
+    return ptOut;
+}
+
+void ref_orientation_box(const float* facets9, int n, const float* rot9, unsigned char* pVR, int nWidth, int nHeight) {
+    int nW_Box = 64, nH_Box = 64;
+    if (nWidth <= nW_Box || nHeight <= nH_Box) return;
+    int w = 512, h = 512;
+    float rot[9];
+    memcpy(rot, rot9, sizeof(rot));
+    std::vector<float> image(w * h, 0.0f), zb(w * h, 0.0f);
+    for (int i = 0; i < n; i++) {
+        const float* q = facets9 + (size_t)i * 9;
+        Point3f v1 = Methods::GetTransferPointf(rot, Point3f(q[0], q[1], q[2]));
+        Point3f v2 = Methods::GetTransferPointf(rot, Point3f(q[3], q[4], q[5]));
+        Point3f v3 = Methods::GetTransferPointf(rot, Point3f(q[6], q[7], q[8]));
+        Direction3f nrm = Direction3f(v3, v1).cross(Direction3f(v3, v2));
+        float d = nrm.dot(Direction3f(0, -1, 0));
+        if (d < 0) continue;
+        float zBuffer = (v1.y() + v2.y() + v3.y()) / 3 + 10000;
+        Point2f p1 = ref_transfer_point_2d(Point2f(v1.x(), v1.z()), w, h);
+        Point2f p2 = ref_transfer_point_2d(Point2f(v2.x(), v2.z()), w, h);
+        Point2f p3 = ref_transfer_point_2d(Point2f(v3.x(), v3.z()), w, h);
+        Methods::FillHoleInImage_Ch1(image.data(), zb.data(), w, h, d, zBuffer, p1, p2, p3);
+    }
+    RGBA clr(0.902, 0.902, 0.302);
+    for (int y = nHeight - nH_Box; y < nHeight; y++) {
+        int yIdx = (y + nH_Box - nHeight) * 8;
+        for (int x = nWidth - nW_Box; x < nWidth; x++) {
+            int xIdx = (x + nW_Box - nWidth) * 8;
+            float diffuse = image[yIdx * w + xIdx];
+            if (diffuse > 0) {
+                RGBA clrTemp = clr * diffuse;
+                clrTemp = clrTemp + 0.05;
+                pVR[3 * (y * nWidth + x)] = int(clrTemp.red * 255);
+                pVR[3 * (y * nWidth + x) + 1] = int(clrTemp.green * 255);
+                pVR[3 * (y * nWidth + x) + 2] = int(clrTemp.blue * 255);
+            }
+        }
+    }
 }
 
 }  // extern "C"
