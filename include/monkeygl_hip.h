@@ -282,6 +282,16 @@ uint64_t mk_orientation_workspace_bytes(mk_context* ctx);
 mk_status mk_merge_orientation_box_async(mk_context* ctx, const float* rotate9, uint8_t* dev_rgb, int width,
                                          int height, void* dev_workspace, uint64_t workspace_bytes, void* stream);
 
+/* CPRLineLayer::GetRGBData (core/CPRLineLayer.cpp:40-65) on an RGB24 device
+ * image: the polyline through n points (dev_xy: 2n floats, VR image
+ * coordinates, i.e. the CPR line's voxel points through
+ * DataManager::TransferVoxel2ImageInVR, core/DataManager.cpp:236-258, done by
+ * the caller), each segment drawn as Methods::DrawLineInImage24Bit of width 1
+ * (core/Methods.cpp:183-200) in color3 (RGB in 0..1, stored as int(c * 255)).
+ * Fewer than 2 points draw nothing. */
+mk_status mk_draw_polyline_async(mk_context* ctx, const float* dev_xy, int n, const float* color3, uint8_t* dev_rgb,
+                                 int width, int height, void* stream);
+
 /* ---- output encoding: PNG + base64 on the device ------------------------- */
 /* Replaces the host encoding of served images: fpng_encode_image_to_memory
  * of the RGB VR frame (core/HelloMonkey.cpp:333-365) and of an int16 plane

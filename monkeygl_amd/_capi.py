@@ -31,7 +31,7 @@ SYMBOLS = [
     "mk_render", "mk_render_async", "mk_render_footprint", "mk_render_plane", "mk_render_plane_async",
     "mk_render_cpr", "mk_render_cpr_async", "mk_marching_cubes", "mk_marching_cubes_async",
     "mk_tf_forward", "mk_tf_backward", "mk_set_orientation_mesh", "mk_orientation_workspace_bytes",
-    "mk_merge_orientation_box_async", "mk_png_max_bytes", "mk_png_workspace_bytes", "mk_png_encode_async",
+    "mk_merge_orientation_box_async", "mk_draw_polyline_async", "mk_png_max_bytes", "mk_png_workspace_bytes", "mk_png_encode_async",
     "mk_base64_max_bytes", "mk_base64_encode_async", "mk_batcher_create", "mk_batcher_destroy", "mk_batcher_last_error",
     "mk_batch_submit", "mk_batch_query", "mk_batch_wait",
 ]
@@ -114,6 +114,7 @@ def load() -> C.CDLL:
         "mk_set_orientation_mesh": (st, [vp, vp, i]),
         "mk_orientation_workspace_bytes": (C.c_uint64, [vp]),
         "mk_merge_orientation_box_async": (st, [vp, vp, vp, i, i, vp, C.c_uint64, vp]),
+        "mk_draw_polyline_async": (st, [vp, vp, i, vp, vp, i, i, vp]),
         "mk_merge_object_mask": (st, [vp, vp, i, i, i, i, i]),
         "mk_merge_object_mask_device": (st, [vp, vp, i, i, i, i, i]),
         "mk_get_mask_data": (st, [vp, vp]),
@@ -253,6 +254,11 @@ class Context:
         ws = dev_workspace.numel() * dev_workspace.element_size()
         self._check(self.lib.mk_merge_orientation_box_async(self.h, _ptr(m), _ptr(dev_rgb), width, height,
                                                             _ptr(dev_workspace), ws, stream))
+
+    def draw_polyline_async(self, dev_xy, n: int, color3, dev_rgb, width: int, height: int, stream: int = 0) -> None:
+        c = np.ascontiguousarray(color3, dtype=np.float32).reshape(3)
+        self._check(self.lib.mk_draw_polyline_async(self.h, _ptr(dev_xy), int(n), _ptr(c), _ptr(dev_rgb), width, height,
+                                                    stream))
 
     def origin_slice_async(self, z: int, dev_out, stream: int = 0) -> None:
         self._check(self.lib.mk_origin_slice_async(self.h, int(z), _ptr(dev_out), stream))

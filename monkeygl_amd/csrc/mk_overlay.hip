@@ -19,7 +19,9 @@
 // after adding 0.5). The sequential depth test
 // (paint when the stored depth is 0 or smaller) leaves each pixel to the first
 // facet with the largest depth, which one 64-bit atomic maximum of (depth
-// bits, inverted facet index) per sample point reproduces in parallel.
+// bits, inverted facet index) per sample point reproduces in parallel. A
+// thread handles a facet; the few facets spanning many sample rows are
+// queued and get a wave each, one lane per row.
 //
 // The arithmetic follows the reference's types and order (double rotation
 // sums, float directions and depth), so the file is built without
@@ -124,17 +126,30 @@ struct Rot9 {
     float m[9];
 };
 
-__global__ void __launch_bounds__(256) k_clear_keys(unsigned long long* keys) {
+// Per-call state in the workspace: the sample-point keys, each facet's
+// diffuse term, and the queue of facets spanning many sample rows.
+struct OrientWs {
+    unsigned long long* keys;   // kBox * kBox
+    float* diffuse;             // n
+    unsigned int* nbig;         // queue length
+    int* big;                   // n
+};
+
+__global__ void __launch_bounds__(256) k_clear_keys(OrientWs ws) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < kBox * kBox) keys[i] = 0ull;
+    if (i < kBox * kBox) ws.keys[i] = 0ull;
+    if (i == 0) *ws.nbig = 0u;
 }
 
-// One thread per facet: rotate, cull, project, then the painted sample points.
-__global__ void __launch_bounds__(256) k_orient_facets(const float* __restrict__ facets, int n, Rot9 rot,
-                                                       float* __restrict__ diffuse,
-                                                       unsigned long long* __restrict__ keys) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= n) return;
+// A facet after rotation, culling and projection, ready for its sample rows.
+struct FacetRows {
+    Line L[3];
+    unsigned long long key;
+    int xmin, ymin, nw;
+    int y0, y1;   // sample rows y0, y0 + 8, ..., <= y1
+};
+
+__device__ inline bool facet_setup(const float* __restrict__ facets, int f, const Rot9& rot, FacetRows& R, float& d) {
     const float* q = facets + (size_t)f * 9;
     const float3 v1 = rotate(rot.m, make_float3(q[0], q[1], q[2]));
     const float3 v2 = rotate(rot.m, make_float3(q[3], q[4], q[5]));
@@ -143,12 +158,11 @@ __global__ void __launch_bounds__(256) k_orient_facets(const float* __restrict__
     const float3 a = fdir(v1.x - v3.x, v1.y - v3.y, v1.z - v3.z);
     const float3 b = fdir(v2.x - v3.x, v2.y - v3.y, v2.z - v3.z);
     const float3 nn = fdir(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-    float d = 0.0f;
+    d = 0.0f;
     d += nn.x * 0.0f;
     d += nn.y * -1.0f;
     d += nn.z * 0.0f;
-    if (d < 0.0f) return;   // NaN passes, as in the reference
-    diffuse[f] = d;
+    if (d < 0.0f) return false;   // NaN passes, as in the reference
     const float zbuf = (v1.y + v2.y + v3.y) / 3 + 10000;
     // TransferPoint2D: (x, z) * 512 + (256, 256)
     const float r = (float)kRaster, c = (float)(kRaster / 2.0);
@@ -162,39 +176,76 @@ __global__ void __launch_bounds__(256) k_orient_facets(const float* __restrict__
     // there the reference's integer conversions are undefined and its mask of
     // the bounding box would not fit in memory; it also bounds every loop below
     const float lim = 1.0e6f;
-    if (!(fabsf(fxmin) < lim && fabsf(fymin) < lim && fabsf(fxmax) < lim && fabsf(fymax) < lim)) return;
+    if (!(fabsf(fxmin) < lim && fabsf(fymin) < lim && fabsf(fxmax) < lim && fabsf(fymax) < lim)) return false;
     const int xmin = (int)fxmin, ymin = (int)fymin;
     const int xmax = (int)((double)fxmax + 0.5), ymax = (int)((double)fymax + 0.5);
-    if (xmin >= kRaster || ymin >= kRaster || xmax < 0 || ymax < 0) return;
+    if (xmin >= kRaster || ymin >= kRaster || xmax < 0 || ymax < 0) return false;
     const int nw = xmax - xmin + 1, nh = ymax - ymin + 1;
-    if ((long long)nw * nh > (1ll << 26)) return;
+    if ((long long)nw * nh > (1ll << 26)) return false;
     const float ox = (float)xmin, oy = (float)ymin;
-    Line L[3];
-    L[0] = make_line(p1x - ox, p1y - oy, p2x - ox, p2y - oy);
-    L[1] = make_line(p1x - ox, p1y - oy, p3x - ox, p3y - oy);
-    L[2] = make_line(p2x - ox, p2y - oy, p3x - ox, p3y - oy);
-    const unsigned long long key = ((unsigned long long)__float_as_uint(zbuf) << 32) | (0xFFFFFFFFu - (uint32_t)f);
+    R.L[0] = make_line(p1x - ox, p1y - oy, p2x - ox, p2y - oy);
+    R.L[1] = make_line(p1x - ox, p1y - oy, p3x - ox, p3y - oy);
+    R.L[2] = make_line(p2x - ox, p2y - oy, p3x - ox, p3y - oy);
+    R.key = ((unsigned long long)__float_as_uint(zbuf) << 32) | (0xFFFFFFFFu - (uint32_t)f);
+    R.xmin = xmin;
+    R.ymin = ymin;
+    R.nw = nw;
     // sample rows Y = 8k inside the box rows and the raster
     int y0 = ymin > 0 ? ymin : 0;
-    y0 = (y0 + kStep - 1) / kStep * kStep;
-    const int y1 = ymax < kRaster - 1 ? ymax : kRaster - 1;
-    for (int Y = y0; Y <= y1; Y += kStep) {
-        const int yl = Y - ymin;
-        int lo = INT_MAX, hi = INT_MIN;
-        for (int e = 0; e < 3; ++e) {
-            int a, b;
-            if (row_extent(L[e], yl, nw, a, b)) {
-                lo = min(lo, a);
-                hi = max(hi, b);
-            }
+    R.y0 = (y0 + kStep - 1) / kStep * kStep;
+    R.y1 = ymax < kRaster - 1 ? ymax : kRaster - 1;
+    return true;
+}
+
+// One sample row Y of a facet: the painted sample points, lo <= x <= hi in
+// the facet's local row, inside the raster.
+__device__ inline void facet_row(const FacetRows& R, int Y, unsigned long long* __restrict__ keys) {
+    const int yl = Y - R.ymin;
+    int lo = INT_MAX, hi = INT_MIN;
+    for (int e = 0; e < 3; ++e) {
+        int a, b;
+        if (row_extent(R.L[e], yl, R.nw, a, b)) {
+            lo = min(lo, a);
+            hi = max(hi, b);
         }
-        if (lo > hi) continue;   // no edge pixel in this row
-        // painted: lo <= x <= hi, inside the raster
-        int X0 = lo + xmin;
-        X0 = X0 > 0 ? X0 : 0;
-        X0 = (X0 + kStep - 1) / kStep * kStep;
-        for (int X = X0; X <= hi + xmin && X < kRaster; X += kStep)
-            atomicMax(&keys[(Y / kStep) * kBox + X / kStep], key);
+    }
+    if (lo > hi) return;   // no edge pixel in this row
+    int X0 = lo + R.xmin;
+    X0 = X0 > 0 ? X0 : 0;
+    X0 = (X0 + kStep - 1) / kStep * kStep;
+    for (int X = X0; X <= hi + R.xmin && X < kRaster; X += kStep)
+        atomicMax(&keys[(Y / kStep) * kBox + X / kStep], R.key);
+}
+
+constexpr int kRowsPerThread = 2;   // facets with more sample rows go to k_orient_big
+
+// One thread per facet; facets spanning more than kRowsPerThread sample rows
+// are queued for a wave each.
+__global__ void __launch_bounds__(256) k_orient_facets(const float* __restrict__ facets, int n, Rot9 rot, OrientWs ws) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    FacetRows R;
+    float d;
+    if (!facet_setup(facets, f, rot, R, d)) return;
+    ws.diffuse[f] = d;
+    if (R.y1 - R.y0 >= kRowsPerThread * kStep) {
+        ws.big[atomicAdd(ws.nbig, 1u)] = f;
+        return;
+    }
+    for (int Y = R.y0; Y <= R.y1; Y += kStep) facet_row(R, Y, ws.keys);
+}
+
+// One wave per queued facet, one lane per sample row (at most 64 rows).
+__global__ void __launch_bounds__(256) k_orient_big(const float* __restrict__ facets, Rot9 rot, OrientWs ws) {
+    const unsigned int nbig = *ws.nbig;
+    const int lane = threadIdx.x & 63;
+    const unsigned int waves = gridDim.x * (blockDim.x >> 6);
+    for (unsigned int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < nbig; k += waves) {
+        FacetRows R;
+        float d;
+        if (!facet_setup(facets, ws.big[k], rot, R, d)) continue;
+        const int Y = R.y0 + lane * kStep;
+        if (Y <= R.y1) facet_row(R, Y, ws.keys);
     }
 }
 
@@ -223,7 +274,41 @@ __global__ void __launch_bounds__(256) k_orient_compose(const unsigned long long
     }
 }
 
+// A polyline drawn as Methods::DrawLineInImage24Bit of width 1 per segment
+// (core/Methods.cpp:183-200): one workgroup per segment, its threads over the
+// segment's dots. Every dot gets the same colour, so overlapping writes agree.
+__global__ void __launch_bounds__(64) k_polyline(const float* __restrict__ xy, int n, uint8_t r, uint8_t g, uint8_t b,
+                                                 uint8_t* __restrict__ rgb, int width, int height) {
+    const int k = blockIdx.x;
+    if (k + 1 >= n) return;
+    const Line L = make_line(xy[2 * k], xy[2 * k + 1], xy[2 * k + 2], xy[2 * k + 3]);
+    for (int i = threadIdx.x; i <= L.nsteps; i += blockDim.x) {
+        const int x = dot_x(L, i), y = dot_y(L, i);
+        if (x < 0 || x >= width || y < 0 || y >= height) continue;
+        uint8_t* px = rgb + 3 * ((size_t)y * width + x);
+        px[0] = r;
+        px[1] = g;
+        px[2] = b;
+    }
+}
+
 }  // namespace
+
+extern "C" mk_status mk_draw_polyline_async(mk_context* ctx, const float* dev_xy, int n, const float* color3,
+                                            uint8_t* dev_rgb, int width, int height, void* stream) {
+    if (!ctx) return MK_ERR_ARG;
+    if (n < 0 || (n > 0 && (!dev_xy || !color3)) || !dev_rgb || width <= 0 || height <= 0)
+        return mk_fail(ctx, MK_ERR_ARG, "mk_draw_polyline_async: bad arguments");
+    if (n < 2) return MK_OK;
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    // DrawDotInImage24Bit: int(c * 255) in float
+    const uint8_t r = (uint8_t)(int)(color3[0] * 255), g = (uint8_t)(int)(color3[1] * 255),
+                  b = (uint8_t)(int)(color3[2] * 255);
+    hipLaunchKernelGGL(k_polyline, dim3((uint32_t)(n - 1)), dim3(64), 0, (hipStream_t)stream, dev_xy, n, r, g, b,
+                       dev_rgb, width, height);
+    MK_HIP(ctx, hipGetLastError());
+    return MK_OK;
+}
 
 extern "C" mk_status mk_set_orientation_mesh(mk_context* ctx, const uint8_t* stl_facets, int n_facets) {
     if (!ctx) return MK_ERR_ARG;
@@ -266,7 +351,9 @@ extern "C" mk_status mk_set_orientation_mesh(mk_context* ctx, const uint8_t* stl
 
 extern "C" uint64_t mk_orientation_workspace_bytes(mk_context* ctx) {
     if (!ctx) return 0;
-    return (uint64_t)kBox * kBox * 8 + ((uint64_t)ctx->orient_n * 4 + 255) / 256 * 256;
+    // keys, diffuse (n floats), queue length, queue (n ints), each part 256-byte aligned
+    const uint64_t n = (uint64_t)ctx->orient_n;
+    return (uint64_t)kBox * kBox * 8 + (n * 4 + 255) / 256 * 256 + 256 + (n * 4 + 255) / 256 * 256;
 }
 
 extern "C" mk_status mk_merge_orientation_box_async(mk_context* ctx, const float* rotate9, uint8_t* dev_rgb, int width,
@@ -283,13 +370,22 @@ extern "C" mk_status mk_merge_orientation_box_async(mk_context* ctx, const float
     Rot9 rot;
     memcpy(rot.m, rotate9, sizeof(rot.m));
     hipStream_t s = (hipStream_t)stream;
-    unsigned long long* keys = (unsigned long long*)dev_workspace;
-    float* diffuse = (float*)((uint8_t*)dev_workspace + (size_t)kBox * kBox * 8);
-    hipLaunchKernelGGL(k_clear_keys, dim3(kBox * kBox / 256), dim3(256), 0, s, keys);
+    const uint64_t n = (uint64_t)ctx->orient_n;
+    uint8_t* w = (uint8_t*)dev_workspace;
+    OrientWs ws;
+    ws.keys = (unsigned long long*)w;
+    w += (size_t)kBox * kBox * 8;
+    ws.diffuse = (float*)w;
+    w += (n * 4 + 255) / 256 * 256;
+    ws.nbig = (unsigned int*)w;
+    w += 256;
+    ws.big = (int*)w;
+    hipLaunchKernelGGL(k_clear_keys, dim3(kBox * kBox / 256), dim3(256), 0, s, ws);
     hipLaunchKernelGGL(k_orient_facets, dim3((ctx->orient_n + 255) / 256), dim3(256), 0, s, ctx->d_orient,
-                       ctx->orient_n, rot, diffuse, keys);
-    hipLaunchKernelGGL(k_orient_compose, dim3(kBox * kBox / 256), dim3(256), 0, s, keys, diffuse, dev_rgb, width,
-                       height);
+                       ctx->orient_n, rot, ws);
+    hipLaunchKernelGGL(k_orient_big, dim3(64), dim3(256), 0, s, ctx->d_orient, rot, ws);
+    hipLaunchKernelGGL(k_orient_compose, dim3(kBox * kBox / 256), dim3(256), 0, s, ws.keys, ws.diffuse, dev_rgb,
+                       width, height);
     MK_HIP(ctx, hipGetLastError());
     return MK_OK;
 }

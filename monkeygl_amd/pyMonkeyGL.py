@@ -649,7 +649,7 @@ class HelloMonkey:
     def SetCPRLineColor(self, clr) -> None:
         self._cpr_color = clr
 
-    # -- layers and annotations (state only: the overlays are not drawn) ----------------
+    # -- layers and annotations (annotation text keeps its state only) ---------------------
     def ShowPlaneInVR(self, bShow: bool) -> None:
         """DataManager::ShowPlaneInVR (core/DataManager.cpp:769-812): three slabs
         added as a label at 0.6 alpha."""
@@ -700,7 +700,35 @@ class HelloMonkey:
             if self._orient_ws is None or self._orient_ws.numel() < nb:
                 self._orient_ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
             c.merge_orientation_box_async(self._ri.rot, out, w, h, self._orient_ws, stream=st)
+        # the annotation layer (text) keeps its state only; the CPR line layer
+        # draws the CPR line's voxel points projected into the image
+        line = self._cprinfo.line
+        if self._layers.get(int(LayerType.LayerTypeCPRLine), True) and len(line) >= 2:
+            xy = torch.tensor(self._voxel_to_vr(line, w, h), device="cuda")
+            c.draw_polyline_async(xy, len(line), self._cpr_color.tuple()[:3], out, w, h, stream=st)
+            out._mk_keep = xy
         return out, w, h
+
+    def _voxel_to_vr(self, pts, w: int, h: int) -> np.ndarray:
+        """DataManager::TransferVoxel2ImageInVR (core/DataManager.cpp:236-258)
+        for each voxel point: (x, y) float pairs in the w x h VR image."""
+        W, H, D = self._dims
+        sx, sy, sz = self._spacing
+        max_len = float(np.float32(max(W * sx, max(H * sy, D * sz))))
+        maxper = (max_len / (W * sx), max_len / (H * sy), max_len / (D * sz))
+        mt = [float(v) for v in np.asarray(self._ri.m_t, np.float32).reshape(9)]
+        inv = self._need_invert_z()
+        out = np.empty((len(pts), 2), np.float32)
+        for k, p in enumerate(pts):
+            x, y, z = p[0] / W, p[1] / H, p[2] / D
+            if inv:
+                z = 1.0 - z
+            x, y, z = (x - 0.5) / maxper[0], (y - 0.5) / maxper[1], (z - 0.5) / maxper[2]
+            rx = mt[0] * x + mt[1] * y + mt[2] * z
+            rz = mt[6] * x + mt[7] * y + mt[8] * z
+            out[k, 0] = (rx + 0.5) * w + float(self._ri.x_translate)
+            out[k, 1] = (rz + 0.5) * h + float(self._ri.y_translate)
+        return out
 
     def GetVRArray(self, nWidth: int, nHeight: int):
         if not self._image_layer():

@@ -19,7 +19,8 @@
 //   (core/Methods.cpp:308-368) under the few lines of
 //   VRImageLayer::MergeOrientationBox / GetMeshPoints and
 //   MarchingCube::LoadBinarySTL that call it (restated below: their files
-//   do not build here).
+//   do not build here); the CPR line of VR images (TransferVoxel2ImageInVR
+//   over Methods::MatrixMul, DrawLineInImage24Bit).
 // Used only to write golden vectors (tests/golden/make_golden.py) and to
 // cross-check the restatements where the reference is present.
 #include <cfloat>
@@ -260,6 +261,43 @@ void ref_orientation_box(const float* facets9, int n, const float* rot9, unsigne
                 pVR[3 * (y * nWidth + x) + 2] = int(clrTemp.blue * 255);
             }
         }
+    }
+}
+
+// DataManager::TransferVoxel2ImageInVR (core/DataManager.cpp:236-258; the
+// file does not build here) over the reference's Point3d and
+// Methods::MatrixMul, then CPRLineLayer::GetRGBData's polyline
+// (core/CPRLineLayer.cpp:40-65) through Methods::DrawLineInImage24Bit.
+// mt9: RenderInfo's transposed transform matrix.
+void ref_voxel_to_vr(const double* vox, int n, const int* dims, const double* spacing, int invert_z, const float* mt9,
+                     float xt, float yt, int nWidth, int nHeight, float* xy) {
+    float mt[9];
+    memcpy(mt, mt9, sizeof(mt));
+    for (int i = 0; i < n; ++i) {
+        double fxSpacing = spacing[0], fySpacing = spacing[1], fzSpacing = spacing[2];
+        float fMaxLen = max(dims[0] * fxSpacing, max(dims[1] * fySpacing, dims[2] * fzSpacing));
+        Point3d maxper(fMaxLen / (dims[0] * fxSpacing), fMaxLen / (dims[1] * fySpacing),
+                       fMaxLen / (dims[2] * fzSpacing));
+        Point3d ptVoxel(vox[3 * i], vox[3 * i + 1], vox[3 * i + 2]);
+        Point3d pt(ptVoxel.x() / dims[0], ptVoxel.y() / dims[1], ptVoxel.z() / dims[2]);
+        if (invert_z) pt.SetZ(1.0 - pt.z());
+        pt -= Point3d(0.5, 0.5, 0.5);
+        pt /= maxper;
+        pt = Methods::MatrixMul(mt, pt);
+        xy[2 * i] = (pt.x() + 0.5) * nWidth + xt;
+        xy[2 * i + 1] = (pt.z() + 0.5) * nHeight + yt;
+    }
+}
+
+void ref_polyline_24bit(const float* xy, int n, const float* rgb, unsigned char* img, int w, int h) {
+    RGBA clr(rgb[0], rgb[1], rgb[2]);
+    if (n < 2) return;
+    float x0 = xy[0], y0 = xy[1];
+    for (int i = 1; i < n; i++) {
+        float x1 = xy[2 * i], y1 = xy[2 * i + 1];
+        Methods::DrawLineInImage24Bit(img, w, h, x0, y0, x1, y1, 1, clr);
+        x0 = x1;
+        y0 = y1;
     }
 }
 

@@ -104,3 +104,54 @@ def test_drop_in_draws_the_figure(tmp_path, monkeypatch):
     assert not np.array_equal(exp, plain) and np.array_equal(img, exp)
     arr = hm.GetVRArray(160, 120)
     assert np.array_equal(arr.reshape(-1), exp.reshape(-1))
+
+
+def _ref_polyline(xy, rgb, img):
+    L = C.CDLL(str(REF))
+    L.ref_polyline_24bit.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+    out = img.copy()
+    a = np.ascontiguousarray(xy, np.float32)
+    c = np.ascontiguousarray(rgb, np.float32)
+    L.ref_polyline_24bit(a.ctypes.data, a.shape[0], c.ctypes.data, out.ctypes.data, img.shape[1], img.shape[0])
+    return out
+
+
+def test_polyline_matches_the_reference_line_drawer(gpu_ctx):
+    import torch
+
+    rng = np.random.default_rng(11)
+    h, w = 150, 210
+    img = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    xy = np.concatenate([rng.uniform(-40, 250, (30, 2)),
+                         [[10.5, 10.5], [10.5, 10.5], [199.99, 149.2], [-5.0, 160.0], [0.0, 0.0]]]).astype(np.float32)
+    for color in ((1.0, 1.0, 1.0), (0.9, 0.25, 0.0)):
+        dev = torch.tensor(img, device="cuda")
+        gpu_ctx.draw_polyline_async(torch.tensor(xy, device="cuda"), len(xy), color, dev, w, h,
+                                    stream=torch.cuda.current_stream().cuda_stream)
+        assert np.array_equal(dev.cpu().numpy(), _ref_polyline(xy, color, img)), color
+
+
+def test_drop_in_draws_the_cpr_line():
+    from monkeygl_amd import pyMonkeyGL as mk
+    from monkeygl_amd import workloads as W
+    from tests.png_util import read_png
+
+    spec = W.VolumeSpec("cprline", (48, 40, 36), (0.8, 0.8, 1.2), 3, (2, 6), (300, 1500), n_vessels=1, seed=6)
+    vol = W.make_volume(spec)
+    hm = mk.HelloMonkey()
+    hm.SetVolumeArray(vol.swapaxes(2, 0))
+    hm.SetSpacing(*spec.spacing)
+    hm.SetVRWWWL(500, 100)
+    hm.SetVRSize(180, 140)
+    hm.Rotate(25, 15)
+    line = np.stack([np.linspace(5, 40, 12), 20 + 8 * np.sin(np.linspace(0, 3, 12)), np.linspace(4, 30, 12)], 1)
+    assert hm.SetCPRLineVoxelArray(line.astype(np.float32))
+    hm.SetCPRLineColor(mk.RGBA(1.0, 0.2, 0.1))
+    img, _ = read_png(base64.b64decode(hm.GetVRData_pngString()))
+    plain = hm._ctx.render(hm._render_params(180, 140))
+    xy = hm._voxel_to_vr(hm._cprinfo.line, 180, 140)
+    exp = _ref_polyline(xy, (1.0, 0.2, 0.1), plain)
+    assert not np.array_equal(exp, plain) and np.array_equal(img, exp)
+    hm.EnableLayer(mk.LayerTypeCPRLine, False)
+    img, _ = read_png(base64.b64decode(hm.GetVRData_pngString()))
+    assert np.array_equal(img, plain)

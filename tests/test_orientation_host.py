@@ -89,3 +89,32 @@ def test_first_facet_wins_at_equal_depth():
     both = (ra.sum(-1) > 0) & (rb.sum(-1) > 0)
     assert both.any() and not np.array_equal(ra[both], rb[both])
     assert np.array_equal(ab[both], ra[both]) and np.array_equal(ba[both], rb[both])
+
+
+def test_cpr_line_projection_matches_the_reference():
+    """The drop-in's TransferVoxel2ImageInVR against the reference's own
+    Point3d / Methods::MatrixMul arithmetic, bit for bit."""
+    from monkeygl_amd import pyMonkeyGL as mk
+
+    L = C.CDLL(str(REF))
+    L.ref_voxel_to_vr.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_float,
+                                  C.c_float, C.c_int, C.c_int, C.c_void_p]
+    rng = np.random.default_rng(2)
+    for dims, sp, dirz in (((64, 48, 40), (0.7, 0.7, 1.3), -1.0), ((33, 70, 21), (1.0, 0.5, 2.0), 1.0)):
+        hm = mk.HelloMonkey()
+        hm.SetVolumeArray(np.zeros(dims, np.int16))
+        hm.SetSpacing(*sp)
+        hm.SetDirection(mk.Direction3d(1, 0, 0), mk.Direction3d(0, 1, 0), mk.Direction3d(0, 0, dirz))
+        hm.Rotate(33.0, -20.0)
+        hm.Zoom(1.3)
+        hm.Pan(4.5, -7.25)
+        pts = rng.uniform(0, 1, (25, 3)) * np.array(dims)
+        got = hm._voxel_to_vr([tuple(p) for p in pts], 300, 220)
+        exp = np.zeros((25, 2), np.float32)
+        d = np.array(dims, np.int32)
+        s = np.array(sp, np.float64)
+        mt = np.asarray(hm._ri.m_t, np.float32)
+        L.ref_voxel_to_vr(np.ascontiguousarray(pts).ctypes.data, 25, d.ctypes.data, s.ctypes.data,
+                          int(hm._need_invert_z()), mt.ctypes.data, float(hm._ri.x_translate),
+                          float(hm._ri.y_translate), 300, 220, exp.ctypes.data)
+        assert np.array_equal(got, exp)
