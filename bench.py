@@ -532,6 +532,21 @@ def main() -> int:
     tp = Path(args.traffic)
     if tp.exists() and args.size == 1024:
         traffic = json.loads(tp.read_text()).get("hbm_bytes_per_frame")
+    # achievable HBM bandwidth on this box: a 1 GiB device-to-device copy
+    # (read + write bytes over the copy's event time), beside the spec peak
+    cp_a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+    cp_b = torch.empty_like(cp_a)
+    cp_b.copy_(cp_a)
+    cs = torch.cuda.current_stream()   # the stream the copies run on
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record(cs)
+    for _ in range(5):
+        cp_b.copy_(cp_a)
+    c1.record(cs)
+    torch.cuda.synchronize()
+    copy_gbs = 2.0 * cp_a.numel() * 5 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    del cp_a, cp_b
+    log(f"device copy: {copy_gbs:.0f} GB/s achievable")
 
     # device PNG + base64 of the headline frame (the reference encodes on the host)
     from monkeygl_amd.encode import Encoder
@@ -634,7 +649,8 @@ def main() -> int:
         "roofline": None if achieved is None else {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-            "bytes_per_frame": int(b_frame), "kernel": "k_vr"},
+            "bytes_per_frame": int(b_frame), "kernel": "k_vr", "achievable_copy_gbs": round(copy_gbs, 1),
+            "frac_of_copy": round(achieved / copy_gbs, 5)},
         "parity": parity,
         "cpu_baseline": cpu,
         "encode": encode,

@@ -1,0 +1,116 @@
+"""Parity at the workload sizes the benchmark configurations name, through
+the C ABI: the cardiac-sized set (512x512x361: 1024^2 VR and MIP over 12 views
+rotating about z; axial, sagittal and coronal MPR at 512^2, nearest bit-exact
+and trilinear within 1 HU) and the body-sized set with its 4-label mask
+(1024^2 VR; stretched and straightened CPR along the seeded 200-point
+centreline). Full VR/MIP frames are checked on bands of rows against the FP64
+oracle (1/255 on >= 99.9% of the band's pixels); MPR and CPR images whole."""
+import numpy as np
+import pytest
+
+from monkeygl_amd import cpr as G
+from monkeygl_amd import params as P
+from monkeygl_amd import workloads as W
+from monkeygl_amd._capi import MK_MPR_AVERAGE, MK_RENDER_MIP, MK_RENDER_VR, MK_SAMPLE_LINEAR, MK_SAMPLE_NEAREST
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BANDS = ((256, 264), (508, 516), (760, 768))   # three 8-row bands of a 1024^2 frame
+
+
+@pytest.fixture(scope="module")
+def cardiac():
+    return W.make_volume(W.CARDIAC)
+
+
+@pytest.fixture(scope="module")
+def body():
+    vol = W.make_volume(W.BODY)
+    mask = W.ellipsoid_mask(W.BODY.dims, 4, seed=1)
+    return vol, mask
+
+
+def _views(spec, labels, rt, n=12, size=1024):
+    out = []
+    for k in range(n):
+        info = P.RenderInfo()
+        info.set_vr_size(size, size)
+        info.rotate(360.0 * k / n, 0.0)   # the x angle drives Rz: rotation about z
+        out.append(P.render_params(spec.dims, spec.spacing, info, labels, rt, width=size, height=size))
+    return out
+
+
+def _band_ok(img, p, vol, luts, mask=None):
+    worst = 1.0
+    for y0, y1 in BANDS:
+        ref = O.render_rows(p, vol, luts, y0, y1, mask_zyx=mask)
+        d = np.abs(img[y0:y1].astype(int) - ref.astype(int)).max(-1)
+        worst = min(worst, float((d <= 1).mean()))
+    return worst
+
+
+@pytest.mark.parametrize("rt", [MK_RENDER_VR, MK_RENDER_MIP])
+def test_cardiac_vr_mip_12_views(gpu_ctx, cardiac, rt):
+    O.set_threads(16)
+    gpu_ctx.set_volume(cardiac)
+    luts = {0: P.transfer_function(W.tf_points(1))}
+    gpu_ctx.set_lut(0, luts[0])
+    labels = {0: P.LabelState(1.0, 900.0, 250.0)}
+    for k, p in enumerate(_views(W.CARDIAC, labels, rt)):
+        img = gpu_ctx.render(p)
+        assert img.shape == (1024, 1024, 3)
+        frac = _band_ok(img, p, cardiac, luts)
+        assert frac >= 0.999, (k, frac)
+
+
+def test_cardiac_mpr_512(gpu_ctx, cardiac):
+    gpu_ctx.set_volume(cardiac)
+    for plane in (P.PLANE_AXIAL, P.PLANE_SAGITTAL, P.PLANE_CORONAL):
+        for sampling in (MK_SAMPLE_NEAREST, MK_SAMPLE_LINEAR):
+            q = P.plane_params(W.CARDIAC.dims, W.CARDIAC.spacing, plane, mpr_type=MK_MPR_AVERAGE,
+                               sampling=sampling, size=(512, 512))
+            got = gpu_ctx.render_plane(q)
+            assert got.shape == (512, 512)
+            if sampling == MK_SAMPLE_NEAREST:
+                assert np.array_equal(got, O.render_plane(q, cardiac, precision="f32")), plane
+            else:
+                d = np.abs(got.astype(np.int32) - O.render_plane(q, cardiac).astype(np.int32))
+                assert (d <= 1).mean() >= 0.999, (plane, int(d.max()))
+
+
+def test_body_vr_with_mask(gpu_ctx, body):
+    O.set_threads(16)
+    vol, mask = body
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_mask(mask)
+    luts, labels = {}, {}
+    for l in range(5):
+        luts[l] = P.transfer_function(W.tf_points(100 + l))
+        gpu_ctx.set_lut(l, luts[l])
+        labels[l] = P.LabelState(alpha=0.4 if l == 0 else 1.0, ww=1200.0 + 200.0 * l, wl=-200.0 + 60.0 * l)
+    for k, p in enumerate(_views(W.BODY, labels, MK_RENDER_VR, n=4)):
+        img = gpu_ctx.render(p)
+        frac = _band_ok(img, p, vol, luts, mask)
+        assert frac >= 0.999, (k, frac)
+    gpu_ctx.clear_mask()
+
+
+@pytest.mark.parametrize("which", [G.PLANE_STRETCHED_CPR, G.PLANE_STRAIGHTENED_CPR])
+def test_body_cpr_200_point_centreline(gpu_ctx, body, which):
+    vol, _ = body
+    gpu_ctx.set_volume(vol)
+    info = G.CPRInfo()
+    info.set_spacing(W.BODY.spacing)
+    assert info.set_cpr_line_voxel(W.centerline(W.BODY.dims, 200, seed=4))
+    pts, dirs, w, h = info.get_cpr_info(which)
+    for width in (w, 512):
+        for sampling in (MK_SAMPLE_NEAREST, MK_SAMPLE_LINEAR):
+            got = gpu_ctx.render_cpr(width, pts, dirs, False, sampling)
+            if sampling == MK_SAMPLE_NEAREST:
+                ref = O.render_cpr(width, pts, dirs, False, vol, sampling, precision="f32")
+                assert np.array_equal(got, ref), (which, width)
+            else:
+                ref = O.render_cpr(width, pts, dirs, False, vol, sampling, precision="f64")
+                d = np.abs(got.astype(np.int32) - ref.astype(np.int32))
+                assert (d <= 1).mean() >= 0.999, (which, width, int(d.max()))
