@@ -298,8 +298,8 @@ mk_status mk_draw_polyline_async(mk_context* ctx, const float* dev_xy, int n, co
  * read as an RGBA image of width W/2 (core/HelloMonkey.cpp:212-259), then
  * Base64::Encode (core/Base64.hpp). The PNG differs in its bytes from fpng's
  * but decodes to the same pixels (8-bit RGB or RGBA, one IDAT chunk, zlib
- * stream with one fixed-Huffman deflate block). Rows hold at most 24576 bytes
- * (width * channels), height at most 65536. */
+ * stream with one deflate block coded with the image's own Huffman code).
+ * Rows hold at most 24576 bytes (width * channels), height at most 65536. */
 
 /* Upper bound of the encoded PNG size (0 for unsupported dims). */
 uint64_t mk_png_max_bytes(int width, int height, int channels);

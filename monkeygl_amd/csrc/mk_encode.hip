@@ -7,23 +7,29 @@
 // there; any PNG decoder returns the same pixels, which is the contract the
 // client (a PNG decoder after atob) relies on.
 //
-// Format. One IDAT chunk holding a zlib stream with a single fixed-Huffman
-// deflate block. Each row gets the PNG filter (None, Sub, Up, Average, Paeth)
-// with the smallest sum of |signed byte| (the usual heuristic); the filtered
-// row is coded as runs of equal bytes: a run's first byte is a literal and the
-// rest are distance-1 copies in matches of up to 258 (a remainder below 3
-// stays literal). Rows never reference each other, so every row is coded
-// independently: the bit length of each row, an exclusive scan, then every
-// row writes its bits at its offset. Adler-32 is combined from per-row sums,
-// CRC-32 from per-block raw CRCs (CRC is linear: a block's contribution is
-// moved past the following bytes by multiplying with x^(8n) mod P).
+// Format. One IDAT chunk holding a zlib stream with a single deflate block
+// coded with Huffman codes built for the image (dynamic block). Each row gets
+// the PNG filter (None, Sub, Up, Average, Paeth) with the smallest sum of
+// |signed byte| (the usual heuristic); the filtered row is coded as runs of
+// equal bytes: a run's first byte is a literal and the rest are distance-1
+// copies in matches of up to 258 (a remainder below 3 stays literal). The
+// literal/length code is a Huffman code of the image's symbol counts, limited
+// to 12 bits (so no byte costs more than 12 bits); the one distance code has
+// length 1; the code lengths are sent with a fixed 4-bit code-length code.
+// Rows never reference each other, so every row is coded independently: its
+// bit length follows from its symbol counts, an exclusive scan gives the
+// offsets, then every row writes its bits at its offset. Adler-32 is combined
+// from per-row sums, CRC-32 from per-block raw CRCs (CRC is linear: a block's
+// contribution is moved past the following bytes by multiplying with x^(8n)
+// mod P).
 //
-// Launch sequence (one stream, no host round trip): filter + count (one
-// workgroup per row) -> scan (row offsets, every byte around the deflate
-// data) -> emit rows -> CRC (workgroups of 4 KiB; the last one to finish
-// writes the CRC, IEND and the length). The encoded length is left in device
-// memory, so base64 can follow on the same stream. Rows are limited to 24576
-// bytes: the count and emit kernels hold a filtered row and its bits in LDS.
+// Launch sequence (one stream, no host round trip): filter + symbol counts
+// (one workgroup per row) -> code + scan (one workgroup: the Huffman code,
+// row offsets, every byte around the row data) -> emit rows -> CRC
+// (workgroups of 4 KiB; the last one to finish writes the CRC, IEND and the
+// length). The encoded length is left in device memory, so base64 can follow
+// on the same stream. Rows are limited to 24576 bytes: the emit kernel holds
+// a filtered row and its bits in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,9 +44,14 @@ constexpr uint32_t kCrcPoly = 0xEDB88320u;   // reflected CRC-32 polynomial
 constexpr uint32_t kAdlerMod = 65521u;
 constexpr int kCrcBlock = 4096;              // bytes per CRC workgroup
 constexpr uint64_t kHeadBytes = 8 + 25 + 8;  // signature, IHDR, IDAT length + type
-constexpr uint32_t kDeflateStartBit = (uint32_t)(kHeadBytes + 2) * 8;  // after the zlib header
 
-// ---- bit codes of the fixed Huffman table (RFC 1951 3.2.6) -----------------
+constexpr int kNumLitLen = 286;     // literal/length symbols 0..285 (256 = end of block)
+constexpr int kMaxCodeBits = 12;    // literal/length code length limit
+constexpr int kHistStride = 288;    // uint16 counts per row histogram (16-byte rows)
+// dynamic block header: BFINAL + BTYPE, HLIT, HDIST, HCLEN, 19 code-length
+// code lengths, then 286 literal/length and 1 distance code lengths of 4 bits
+constexpr int kHeaderBits = 3 + 5 + 5 + 4 + 19 * 3 + (kNumLitLen + 1) * 4;
+constexpr uint32_t kBlockStartBit = (uint32_t)(kHeadBytes + 2) * 8;   // after the zlib header (bit 344)
 
 __host__ __device__ inline uint32_t bit_reverse(uint32_t c, int n) {
     uint32_t r = 0;
@@ -48,23 +59,9 @@ __host__ __device__ inline uint32_t bit_reverse(uint32_t c, int n) {
     return r;
 }
 
-// Literal byte v: (bits as written LSB first, length).
-__device__ inline void lit_code(uint32_t v, uint32_t& bits, int& len) {
-    if (v < 144) {
-        len = 8;
-        bits = bit_reverse(0x30u + v, 8);
-    } else {
-        len = 9;
-        bits = bit_reverse(0x190u + (v - 144u), 9);
-    }
-}
-
-__device__ inline int lit_len(uint32_t v) { return v < 144 ? 8 : 9; }
-
-// Length L in 3..258 with distance 1: symbol code, extra bits, distance code
-// 0 (five zero bits).
-__device__ inline void match_code(int L, uint32_t& bits, int& len) {
-    int sym, nextra, base;
+// Length symbol (257..285), extra-bit count and base of a match of length L
+// in 3..258 (RFC 1951 3.2.5).
+__device__ inline void len_symbol(int L, int& sym, int& nextra, int& base) {
     if (L == 258) {
         sym = 285; nextra = 0; base = 258;
     } else if (L <= 10) {
@@ -76,30 +73,6 @@ __device__ inline void match_code(int L, uint32_t& bits, int& len) {
         const int k = (L - start) >> e;
         sym = s0 + k; nextra = e; base = start + (k << e);
     }
-    uint32_t code;
-    int clen;
-    if (sym <= 279) { code = (uint32_t)(sym - 256); clen = 7; }
-    else { code = 0xC0u + (uint32_t)(sym - 280); clen = 8; }
-    bits = bit_reverse(code, clen) | ((uint32_t)(L - base) << clen);
-    len = clen + nextra + 5;   // distance code 0: 00000
-}
-
-__device__ inline int match_len(int L) {
-    uint32_t b;
-    int n;
-    match_code(L, b, n);
-    return n;
-}
-
-// Bits of a run of n equal bytes v (n >= 1): one literal, then n-1 copies.
-__device__ inline uint32_t run_bits(uint32_t v, int n) {
-    const int ll = lit_len(v);
-    uint32_t t = (uint32_t)ll;
-    int m = n - 1;
-    t += (uint32_t)(m / 258) * (uint32_t)match_len(258);
-    const int r = m % 258;
-    t += r >= 3 ? (uint32_t)match_len(r) : (uint32_t)(r * ll);
-    return t;
 }
 
 // ---- GF(2) arithmetic for CRC combination (reflected representation) -------
@@ -146,11 +119,14 @@ __device__ inline uint32_t crc_update_raw(uint32_t c, uint8_t byte, const uint32
 struct PngWs {
     uint8_t* filt;      // H * (1 + stride)
     uint32_t* row_bits; // H
-    uint64_t* row_s;    // H: sum of filtered bytes
-    uint64_t* row_t;    // H: sum of (len - k) * byte, k = 0..len-1
+    uint64_t* row_s;    // H: sum of filtered bytes, mod 65521
+    uint64_t* row_t;    // H: sum of (len - k) * byte, k = 0..len-1, mod 65521
     uint64_t* row_off;  // H: bit offset of each row
-    uint64_t* totals;   // [0] deflate end bit, [1] zlib length, [2] png length, [3] adler
+    uint64_t* totals;   // [0] end of the row bits, [1] zlib length, [2] png length, [3] adler, [4] CRC ticket
     uint32_t* crc_part; // per CRC block
+    uint16_t* hist;     // H * kHistStride: each row's literal/length symbol counts
+    uint32_t* extra;    // H: each row's extra and distance bits
+    uint32_t* code;     // kNumLitLen: bit-reversed code | length << 24
 };
 
 // PNG filter per row; one workgroup per row.
@@ -220,13 +196,15 @@ __device__ inline int block_min_after(int v, int* s_w) {
 // A run starts at byte 0 and wherever a byte differs from the previous one,
 // and ends at the next start (or the row end). Thread t owns the bytes
 // [t k, (t + 1) k), k = ceil(n / kRowThreads), and the runs starting there;
-// the end of its last run is the first start owned by a later thread. With
-// kEmit, fn(bit_pos, value, length) is called for each run by its owner at
-// the run's bit offset in the row (exclusive scan of the owners' bit counts).
-// Counting and emitting use this same parse, so a row's bits match its count.
-// Returns the row's bit length (on every thread).
-template <bool kEmit, typename Fn>
-__device__ inline uint32_t parse_row(const uint8_t* row, int n, uint32_t* s_u, int* s_i, Fn fn) {
+// the end of its last run is the first start owned by a later thread.
+struct RowRuns {
+    int first;   // the thread's first run start (INT_MAX: none)
+    int next;    // the first start after the thread's bytes (n: none)
+    int hi;      // end of the thread's bytes
+};
+
+// Every thread of the workgroup calls it (it holds barriers).
+__device__ inline RowRuns row_runs(const uint8_t* row, int n, int* s_i) {
     const int k = (n + kRowThreads - 1) / kRowThreads;
     const int lo = min(n, (int)threadIdx.x * k), hi = min(n, lo + k);
     int first = INT_MAX;
@@ -237,41 +215,43 @@ __device__ inline uint32_t parse_row(const uint8_t* row, int n, uint32_t* s_u, i
         }
     int next = block_min_after(first, s_i);
     if (next == INT_MAX) next = n;
-    uint32_t bits = 0;
-    if (first != INT_MAX) {
-        int st = first, v = row[first];
-        for (int i = first + 1; i < hi; ++i) {
-            const int b = row[i];
-            if (b != v) {
-                bits += run_bits((uint32_t)v, i - st);
-                st = i;
-                v = b;
-            }
-        }
-        bits += run_bits((uint32_t)v, next - st);
-    }
-    uint32_t total;
-    const uint32_t off = block_excl_sum(bits, s_u, total);
-    if (kEmit && first != INT_MAX) {
-        uint32_t pos = off;
-        int st = first, v = row[first];
-        for (int i = first + 1; i < hi; ++i) {
-            const int b = row[i];
-            if (b != v) {
-                fn(pos, v, i - st);
-                pos += run_bits((uint32_t)v, i - st);
-                st = i;
-                v = b;
-            }
-        }
-        fn(pos, v, next - st);
-    }
-    return total;
+    return RowRuns{first, next, hi};
 }
 
-// Filter, count and Adler partials of one row; one workgroup per row, the
-// filtered row staged in LDS for the run parse.
-__global__ void __launch_bounds__(kRowThreads) k_png_filter_count(const uint8_t* __restrict__ img, int width, int height,
+// fn(value, length) for each run of the calling thread, in row order.
+template <typename Fn>
+__device__ inline void visit_runs(const uint8_t* row, const RowRuns& R, Fn fn) {
+    if (R.first == INT_MAX) return;
+    int st = R.first, v = row[st];
+    for (int i = st + 1; i < R.hi; ++i) {
+        const int b = row[i];
+        if (b != v) {
+            fn(v, i - st);
+            st = i;
+            v = b;
+        }
+    }
+    fn(v, R.next - st);
+}
+
+// The symbols of a run of n equal bytes v: one literal, then n - 1 copies in
+// matches of up to 258 (a remainder below 3 stays literal). lit(v, count),
+// match(length) for each match.
+template <typename Lit, typename Match>
+__device__ inline void run_symbols(int v, int n, Lit lit, Match match) {
+    lit(v, 1);
+    int m = n - 1;
+    while (m >= 258) {
+        match(258);
+        m -= 258;
+    }
+    if (m >= 3) match(m);
+    else if (m > 0) lit(v, m);
+}
+
+// Filter, symbol counts and Adler partials of one row; one workgroup per
+// row, the filtered row staged in LDS for the run parse.
+__global__ void __launch_bounds__(kRowThreads) k_png_filter_hist(const uint8_t* __restrict__ img, int width, int height,
                                                           int ch, PngWs ws) {
     extern __shared__ uint8_t s_row[];
     const int r = blockIdx.x;
@@ -325,39 +305,304 @@ __global__ void __launch_bounds__(kRowThreads) k_png_filter_count(const uint8_t*
         s_ad[0][wave] = sa;
         s_ad[1][wave] = sb;
     }
-    __syncthreads();
     __shared__ uint32_t s_u[kRowWaves];
     __shared__ int s_i[kRowWaves];
-    const uint32_t bits = parse_row<false>(s_row, n, s_u, s_i, [](uint32_t, int, int) {});
+    __shared__ uint32_t s_hist[kHistStride];
+    for (int i = threadIdx.x; i < kHistStride; i += blockDim.x) s_hist[i] = 0u;
+    __syncthreads();
+    const RowRuns R = row_runs(s_row, n, s_i);
+    uint32_t extra = 0;   // length extra bits and one distance bit per match
+    visit_runs(s_row, R, [&](int v, int len) {
+        run_symbols(
+            v, len, [&](int b, int count) { atomicAdd(&s_hist[b], (uint32_t)count); },
+            [&](int L) {
+                int sym, ne, base;
+                len_symbol(L, sym, ne, base);
+                atomicAdd(&s_hist[sym], 1u);
+                extra += (uint32_t)ne + 1u;
+            });
+    });
+    uint32_t total_extra;
+    (void)block_excl_sum(extra, s_u, total_extra);   // ends with a barrier: the counts are complete
+    uint16_t* h = ws.hist + (size_t)r * kHistStride;
+    for (int i = threadIdx.x; i < kHistStride; i += blockDim.x) h[i] = (uint16_t)s_hist[i];
     if (threadIdx.x == 0) {
-        ws.row_bits[r] = bits;
-        ws.row_s[r] = s_ad[0][0] + s_ad[0][1] + s_ad[0][2] + s_ad[0][3];
-        ws.row_t[r] = s_ad[1][0] + s_ad[1][1] + s_ad[1][2] + s_ad[1][3];
+        ws.extra[r] = total_extra;
+        // the row's byte sum and weighted sum, reduced mod 65521 for the Adler-32 combination
+        ws.row_s[r] = (s_ad[0][0] + s_ad[0][1] + s_ad[0][2] + s_ad[0][3]) % kAdlerMod;
+        ws.row_t[r] = (s_ad[1][0] + s_ad[1][1] + s_ad[1][2] + s_ad[1][3]) % kAdlerMod;
     }
 }
 
-// One workgroup: row bit offsets, the deflate end, Adler-32, every byte of the
-// PNG before the deflate data (signature, IHDR, IDAT length and type, zlib
-// header, block header) and after it up to the CRC (end-of-block code,
-// padding, Adler-32). The output is not cleared beforehand: the emit kernel
-// stores a row's interior words whole and ORs only the first and last word of
-// each row, which this kernel zeroes, with the words of the end-of-block code.
-__global__ void __launch_bounds__(1024) k_png_scan(int height, int stride, PngWs ws, uint8_t* png, uint32_t ihdr_crc,
-                                                   int width, int ch) {
-    __shared__ uint64_t s_part[1024];
-    __shared__ uint64_t s_a[1024], s_b[1024];
+// One workgroup of kScanThreads: the image's literal/length Huffman code, the
+// row bit lengths and offsets, Adler-32, and every byte of the PNG around the
+// row data: signature, IHDR, IDAT length and type, zlib header and the
+// dynamic block header before it; end-of-block code, padding and Adler-32
+// after it. The output is not cleared beforehand: the emit kernel stores a
+// row's interior words whole and ORs only the first and last word of each
+// row, which this kernel zeroes, with the words of the end-of-block code.
+//
+// The code: symbol counts summed over the rows' histograms (plus one end of
+// block), sorted by a bitonic sort, optimal lengths limited to kMaxCodeBits
+// by package-merge (one parallel merge per level), canonical codes.
+constexpr int kScanThreads = 1024;
+
+__global__ void __launch_bounds__(kScanThreads) k_huff_scan(int height, int stride, PngWs ws, uint8_t* png,
+                                                            uint32_t ihdr_crc, int width, int ch) {
+    __shared__ uint32_t s_freq[kNumLitLen];
+    __shared__ unsigned long long s_key[512];
+    __shared__ int s_npk[kMaxCodeBits + 1];        // packages per level
+    __shared__ int s_cnt[kMaxCodeBits + 1];        // leaves chosen per level
+    __shared__ uint16_t s_pos[kMaxCodeBits * kNumLitLen];   // leaf positions per level
+    __shared__ uint32_t s_len2[kHistStride / 2];   // code lengths of symbol pairs (low | high << 16)
+    __shared__ int s_wcnt[(kNumLitLen + 63) / 64][kMaxCodeBits + 1];
+    __shared__ int s_blc[kMaxCodeBits + 2];
+    __shared__ int s_next[kMaxCodeBits + 2];
+    __shared__ uint8_t s_len[kNumLitLen];
+    __shared__ uint32_t s_code[kNumLitLen];
+    __shared__ uint32_t s_hdr[40];                 // bits 320.. of the PNG: bytes 40-42 and the block header
+    __shared__ int s_m;
+    // the package lists of levels 1..kMaxCodeBits-1 and the merged list, then
+    // (once the code is built) the scan's arrays
+    constexpr int kBig = (kMaxCodeBits - 1) * kNumLitLen + 2 * kNumLitLen > 3 * kScanThreads
+                             ? (kMaxCodeBits - 1) * kNumLitLen + 2 * kNumLitLen
+                             : 3 * kScanThreads;
+    __shared__ unsigned long long s_big[kBig];
+    unsigned long long* s_pk = s_big;                                   // [level - 1][kNumLitLen]
+    unsigned long long* s_list = s_big + (kMaxCodeBits - 1) * kNumLitLen;   // 2 * kNumLitLen
+    unsigned long long* s_part = s_big;
+    unsigned long long* s_a = s_big + kScanThreads;
+    unsigned long long* s_b = s_big + 2 * kScanThreads;
     const int t = threadIdx.x, nt = blockDim.x;
+
+    // 1. symbol counts of the image
+    for (int i = t; i < kNumLitLen; i += nt) s_freq[i] = 0u;
+    for (int i = t; i < 40; i += nt) s_hdr[i] = 0u;
+    __syncthreads();
+    {
+        // thread t sums symbols 4q..4q+3 (q = t % 72) over the rows of group
+        // t / 72 (coalesced loads; a few atomics per symbol instead of one per row)
+        constexpr int kQuads = kHistStride / 4;
+        const int groups = nt / kQuads;
+        if (t < groups * kQuads) {
+            const int q = t % kQuads, g = t / kQuads;
+            uint32_t c[4] = {0, 0, 0, 0};
+            int r = g;
+            for (; r + 3 * groups < height; r += 4 * groups) {
+                uint2 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = ((const uint2*)(ws.hist + (size_t)(r + u * groups) * kHistStride))[q];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    c[0] += v[u].x & 0xFFFFu; c[1] += v[u].x >> 16;
+                    c[2] += v[u].y & 0xFFFFu; c[3] += v[u].y >> 16;
+                }
+            }
+            for (; r < height; r += groups) {
+                const uint2 v = ((const uint2*)(ws.hist + (size_t)r * kHistStride))[q];
+                c[0] += v.x & 0xFFFFu; c[1] += v.x >> 16;
+                c[2] += v.y & 0xFFFFu; c[3] += v.y >> 16;
+            }
+            for (int u = 0; u < 4; ++u)
+                if (4 * q + u < kNumLitLen && c[u]) atomicAdd(&s_freq[4 * q + u], c[u]);
+        }
+    }
+    __syncthreads();
+    if (t == 0) s_freq[256] = 1u;   // end of block
+    __syncthreads();
+
+    // 2. used symbols sorted by (count, symbol)
+    for (int i = t; i < 512; i += nt)
+        s_key[i] = (i < kNumLitLen && s_freq[i]) ? ((unsigned long long)s_freq[i] << 16 | (unsigned)i) : ~0ull;
+    __syncthreads();
+    for (int size = 2; size <= 512; size <<= 1)
+        for (int st = size >> 1; st > 0; st >>= 1) {
+            if (t < 512) {
+                const int j = t ^ st;
+                if (j > t) {
+                    const unsigned long long a = s_key[t], b = s_key[j];
+                    const bool up = (t & size) == 0;
+                    if ((a > b) == up) {
+                        s_key[t] = b;
+                        s_key[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    if (t < 512 && s_key[t] != ~0ull && (t == 511 || s_key[t + 1] == ~0ull)) s_m = t + 1;
+    __syncthreads();
+    const int m = s_m;   // >= 2: the end of block and every row's filter byte
+
+    // 3. optimal code lengths limited to kMaxCodeBits by package-merge: the
+    // list of level kMaxCodeBits is the leaves; the list of level l < that
+    // merges the leaves with the pairs ("packages") of the list of level
+    // l + 1, leaves first on equal weights. The first 2m - 2 items of level 1
+    // are chosen; a chosen package at level l chooses the first two items of
+    // level l + 1 per package. A leaf's code length is the number of levels
+    // at which it is chosen, and the leaves chosen at a level are a prefix of
+    // the sorted leaves, so only their count per level is needed.
+    auto leaf_w = [&](int i) { return s_key[i] >> 16; };
+    int src_len = m;   // the list of level l + 1 (the leaves at the deepest level)
+    for (int l = kMaxCodeBits - 1; l >= 1; --l) {
+        unsigned long long* pk = s_pk + (size_t)(l - 1) * kNumLitLen;
+        const int np = src_len / 2;
+        for (int j = t; j < np; j += nt)
+            pk[j] = l == kMaxCodeBits - 1 ? leaf_w(2 * j) + leaf_w(2 * j + 1) : s_list[2 * j] + s_list[2 * j + 1];
+        if (t == 0) s_npk[l] = np;
+        __syncthreads();
+        // merged list of level l
+        for (int i = t; i < m; i += nt) {
+            const unsigned long long w = leaf_w(i);
+            int lo = 0, hi = np;   // packages lighter than the leaf
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (pk[mid] < w) lo = mid + 1;
+                else hi = mid;
+            }
+            s_list[i + lo] = w;
+        }
+        for (int j = t; j < np; j += nt) {
+            const unsigned long long w = pk[j];
+            int lo = 0, hi = m;    // leaves not heavier than the package
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (leaf_w(mid) <= w) lo = mid + 1;
+                else hi = mid;
+            }
+            s_list[j + lo] = w;
+        }
+        src_len = m + np;
+        __syncthreads();
+    }
+    // position of leaf i in the merged list of level l: i + #(packages lighter than it)
+    if (t == 0) s_npk[kMaxCodeBits] = 0;
+    __syncthreads();
+    for (int k = t; k < kMaxCodeBits * m; k += nt) {
+        const int l = 1 + k / m, i = k % m;
+        const unsigned long long* pk = s_pk + (size_t)(l - 1) * kNumLitLen;
+        const unsigned long long w = leaf_w(i);
+        int a = 0, b = s_npk[l];
+        while (a < b) {
+            const int q = (a + b) >> 1;
+            if (pk[q] < w) a = q + 1;
+            else b = q;
+        }
+        s_pos[(l - 1) * kNumLitLen + i] = (uint16_t)(i + a);
+    }
+    __syncthreads();
+    if (t == 0) {
+        int sel = 2 * m - 2;
+        for (int l = 1; l <= kMaxCodeBits; ++l) {
+            // leaves among the first sel items (positions increase with i)
+            const uint16_t* pos = s_pos + (l - 1) * kNumLitLen;
+            int lo = 0, hi = m;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (pos[mid] < sel) lo = mid + 1;
+                else hi = mid;
+            }
+            s_cnt[l] = lo;
+            sel = 2 * (sel - lo);
+        }
+    }
+    for (int i = t; i < kNumLitLen; i += nt) s_len[i] = 0;
+    if (t < kMaxCodeBits + 2) s_blc[t] = 0;
+    __syncthreads();
+    if (t < m) {
+        int L = 0;
+        for (int l = 1; l <= kMaxCodeBits; ++l) L += t < s_cnt[l];
+        s_len[(int)(s_key[t] & 0xFFFFu)] = (uint8_t)L;
+        atomicAdd(&s_blc[L], 1);
+    }
+    __syncthreads();
+    if (t == 0) {
+        // canonical first codes per length
+        int code = 0;
+        s_blc[0] = 0;
+        for (int L = 1; L <= kMaxCodeBits; ++L) {
+            code = (code + s_blc[L - 1]) << 1;
+            s_next[L] = code;
+        }
+    }
+    __syncthreads();
+    // a symbol's code: its length's first code plus its rank among the
+    // symbols of that length (ballots within waves, counts across them)
+    constexpr int kSymWaves = (kNumLitLen + 63) / 64;
+    const int lane = t & 63, wave = t >> 6;
+    const int myL = t < kNumLitLen ? s_len[t] : 0;
+    int rank = 0;
+    if (wave < kSymWaves) {
+        for (int L = 1; L <= kMaxCodeBits; ++L) {
+            const unsigned long long b = __ballot(myL == L);
+            if (myL == L) rank = __popcll(b & ((1ull << lane) - 1ull));
+            if (lane == 0) s_wcnt[wave][L] = __popcll(b);
+        }
+    }
+    __syncthreads();
+    if (t < kNumLitLen) {
+        uint32_t v = 0;
+        if (myL) {
+            for (int q = 0; q < wave; ++q) rank += s_wcnt[q][myL];
+            v = bit_reverse((uint32_t)(s_next[myL] + rank), myL) | ((uint32_t)myL << 24);
+        }
+        s_code[t] = v;
+        ws.code[t] = v;
+    }
+    if (t < kHistStride / 2)
+        s_len2[t] = (2 * t < kNumLitLen ? s_len[2 * t] : 0u) | ((2 * t + 1 < kNumLitLen ? s_len[2 * t + 1] : 0u) << 16);
+    // 7. the block header, from bit 24 of the word that holds bytes 40-43
+    if (t == 0) {
+        s_hdr[0] = 'T' | (0x78u << 8) | (0x01u << 16);   // end of "IDAT", zlib header (deflate, 32K, fastest)
+        auto put = [&](int pos, uint32_t bits, int len) {
+            s_hdr[pos >> 5] |= bits << (pos & 31);
+            if ((pos & 31) + len > 32) s_hdr[(pos >> 5) + 1] |= bits >> (32 - (pos & 31));
+        };
+        put(24, 5u, 3);                              // BFINAL = 1, BTYPE = 2 (dynamic)
+        put(27, (uint32_t)(kNumLitLen - 257), 5);    // HLIT
+        put(32, 0u, 5);                              // HDIST: one distance code
+        put(37, 15u, 4);                             // HCLEN: all 19 code-length code lengths
+        const int order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+        for (int q = 0; q < 19; ++q) put(41 + 3 * q, order[q] <= 15 ? 4u : 0u, 3);
+        put(98 + 4 * kNumLitLen, bit_reverse(1u, 4), 4);   // the distance code: length 1
+    }
+    __syncthreads();
+    if (t < kNumLitLen) {
+        // code length l is sent as the 4-bit canonical code l (sixteen codes of length 4)
+        const int pos = 98 + 4 * t;
+        const uint32_t c = bit_reverse(s_len[t], 4);
+        atomicOr(&s_hdr[pos >> 5], c << (pos & 31));
+        if ((pos & 31) + 4 > 32) atomicOr(&s_hdr[(pos >> 5) + 1], c >> (32 - (pos & 31)));
+    }
+    __syncthreads();
+
+    // 8. row bit lengths, Adler-32 partials and the scan
     const int per = (height + nt - 1) / nt;
     const int r0 = min(height, t * per), r1 = min(height, r0 + per);
     const uint64_t n = (uint64_t)stride + 1, N = n * (uint64_t)height;
-    uint64_t sum = 0, A = 0, B = 0;
+    uint64_t sum = 0;
+    uint32_t A = 0, B = 0;
+    const uint32_t n_mod = (uint32_t)(n % kAdlerMod);
     for (int r = r0; r < r1; ++r) {
-        sum += ws.row_bits[r];
-        // Adler-32 of the filtered rows: A = 1 + sum d, B = N + sum_i (N - i + 1) d_i
-        const uint64_t o = (uint64_t)r * n;
-        const uint64_t S = ws.row_s[r] % kAdlerMod;
+        uint32_t bits = ws.extra[r];
+        const uint4* h = (const uint4*)(ws.hist + (size_t)r * kHistStride);
+        for (int q = 0; q < kHistStride / 8; ++q) {
+            const uint4 v = h[q];
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t l2 = s_len2[q * 4 + k];
+                bits += (w[k] & 0xFFFFu) * (l2 & 0xFFFFu) + (w[k] >> 16) * (l2 >> 16);
+            }
+        }
+        ws.row_bits[r] = bits;
+        sum += bits;
+        // Adler-32 of the filtered rows: A = 1 + sum d, B = N + sum_i (N - i + 1) d_i;
+        // row r starts at o = r n, and N - o - n = n (height - 1 - r)
+        const uint32_t S = (uint32_t)ws.row_s[r];   // already reduced mod 65521
+        const uint32_t coef = (uint32_t)(((uint64_t)n_mod * (uint32_t)((height - 1 - r) % kAdlerMod)) % kAdlerMod);
         A = (A + S) % kAdlerMod;
-        B = (B + ((N - o - n) % kAdlerMod) * S + ws.row_t[r] % kAdlerMod) % kAdlerMod;
+        B = (uint32_t)(((uint64_t)B + (uint64_t)coef * S + (uint32_t)ws.row_t[r]) % kAdlerMod);
     }
     s_part[t] = sum;
     s_a[t] = A;
@@ -372,12 +617,12 @@ __global__ void __launch_bounds__(1024) k_png_scan(int height, int stride, PngWs
         __syncthreads();
         s_part[t] += y;
         if (red) {
-            s_a[t] = (s_a[t] + ya) % kAdlerMod;
-            s_b[t] = (s_b[t] + yb) % kAdlerMod;
+            s_a[t] = (uint32_t)(s_a[t] + ya) % kAdlerMod;
+            s_b[t] = (uint32_t)(s_b[t] + yb) % kAdlerMod;
         }
         __syncthreads();
     }
-    const uint64_t start = kDeflateStartBit + 3;   // after BFINAL + BTYPE
+    const uint64_t start = kBlockStartBit + kHeaderBits;
     uint64_t off = start + (t > 0 ? s_part[t - 1] : 0);
     uint32_t* words = (uint32_t*)png;
     for (int r = r0; r < r1; ++r) {
@@ -388,15 +633,24 @@ __global__ void __launch_bounds__(1024) k_png_scan(int height, int stride, PngWs
         off += nb;
     }
     const uint64_t end = start + s_part[nt - 1];   // end of the row bits
-    // end-of-block: symbol 256 is seven zero bits; with the padding to the
-    // byte boundary it lies in the (at most two) words from the end bit on
-    const uint64_t dbytes_end = (end + 7 + 7) / 8;   // byte after the deflate data
+    const int eob_len = (int)(s_code[256] >> 24);
+    const uint64_t dbytes_end = (end + eob_len + 7) / 8;   // byte after the deflate data
     if (t == 0) {
-        words[end >> 5] = 0u;
-        words[(dbytes_end * 8 - 1) >> 5] = 0u;
+        // the words of the end-of-block code and the padding after it
+        for (uint64_t w = end >> 5; w <= (dbytes_end * 8 - 1) >> 5; ++w) words[w] = 0u;
     }
-    // the zeroed words complete before any thread writes bytes into them (one
+    // the zeroed words complete before any thread writes into them (one
     // workgroup: the barrier orders its global stores)
+    __syncthreads();
+    // bytes 40.. up to the first row: header words (the last one shares bits with row 0)
+    const int hdr_words = (24 + kHeaderBits + 31) / 32;
+    if (t < hdr_words) words[10 + t] = s_hdr[t];
+    __syncthreads();
+    if (t == 0) {
+        const uint32_t eob = s_code[256] & 0xFFFFFFu;
+        atomicOr(&words[end >> 5], eob << (end & 31));
+        if ((end & 31) + eob_len > 32) atomicOr(&words[(end >> 5) + 1], eob >> (32 - (end & 31)));
+    }
     __syncthreads();
     if (t == 0) {
         ws.totals[0] = end;
@@ -424,23 +678,21 @@ __global__ void __launch_bounds__(1024) k_png_scan(int height, int stride, PngWs
         h[16] = 8; h[17] = ch == 3 ? 2 : 6; h[18] = 0; h[19] = 0; h[20] = 0;
         h[21] = (uint8_t)(ihdr_crc >> 24); h[22] = (uint8_t)(ihdr_crc >> 16); h[23] = (uint8_t)(ihdr_crc >> 8);
         h[24] = (uint8_t)ihdr_crc;
-        // IDAT type
-        png[37] = 'I'; png[38] = 'D'; png[39] = 'A'; png[40] = 'T';
-        // zlib header: deflate, 32K window, no dictionary, fastest (0x78 0x01)
-        png[41] = 0x78; png[42] = 0x01;
-        // block header at bit 344: BFINAL = 1, BTYPE = 01 (fixed Huffman)
-        png[43] = 0x03;
+        // IDAT type (its last byte, 'T', went out with the header words)
+        png[37] = 'I'; png[38] = 'D'; png[39] = 'A';
     }
 }
 
-// Emit: one workgroup per row stages the filtered row in LDS, builds the
-// row's bits there, then writes them at the row's bit offset: interior words
-// whole, the first and last words (shared with the neighbouring rows or the
-// bytes around the deflate data, and zeroed by the scan) through atomic OR.
+// Emit: one workgroup per row stages the filtered row and the code table in
+// LDS, builds the row's bits there, then writes them at the row's bit offset:
+// interior words whole, the first and last words (shared with the
+// neighbouring rows or the bytes around the row data, and zeroed by the scan)
+// through atomic OR.
 __global__ void __launch_bounds__(kRowThreads) k_png_emit(int height, int stride, PngWs ws, uint8_t* png) {
     extern __shared__ uint32_t s_mem[];
     __shared__ uint32_t s_u[kRowWaves];
     __shared__ int s_i[kRowWaves];
+    __shared__ uint32_t s_code[kNumLitLen];
     const int r = blockIdx.x;
     if (r >= height) return;
     const int n = stride + 1;
@@ -452,36 +704,51 @@ __global__ void __launch_bounds__(kRowThreads) k_png_emit(int height, int stride
     const uint32_t nwords = (nbits + 31) / 32 + 1;
     for (int i = threadIdx.x; i < n; i += kRowThreads) s_row[i] = row[i];
     for (uint32_t i = threadIdx.x; i < nwords; i += kRowThreads) s_bits[i] = 0u;
+    for (int i = threadIdx.x; i < kNumLitLen; i += kRowThreads) s_code[i] = ws.code[i];
     __syncthreads();
     auto put = [&](uint32_t pos, uint32_t bits, int len) {
-        // len <= 32 (a match: at most 8 + 5 + 5 bits)
+        // len <= 32 (a match: at most 12 + 5 + 1 bits)
         const uint32_t w = pos >> 5, o = pos & 31;
         atomicOr(&s_bits[w], bits << o);
         if (o + len > 32) atomicOr(&s_bits[w + 1], bits >> (32 - o));
     };
-    parse_row<true>(s_row, n, s_u, s_i, [&](uint32_t pos, int v, int len) {
-        uint32_t b;
-        int l;
-        lit_code((uint32_t)v, b, l);
-        put(pos, b, l);
-        pos += l;
-        int m = len - 1;
-        while (m >= 258) {
-            match_code(258, b, l);
-            put(pos, b, l);
-            pos += l;
-            m -= 258;
-        }
-        if (m >= 3) {
-            match_code(m, b, l);
-            put(pos, b, l);
-        } else {
-            for (int k = 0; k < m; ++k) {
-                lit_code((uint32_t)v, b, l);
-                put(pos, b, l);
-                pos += l;
-            }
-        }
+    // bits of a run: the same symbols the histogram counted
+    auto run_len = [&](int v, int len) {
+        uint32_t b = 0;
+        run_symbols(
+            v, len, [&](int c, int count) { b += (s_code[c] >> 24) * (uint32_t)count; },
+            [&](int L) {
+                int sym, ne, base;
+                len_symbol(L, sym, ne, base);
+                b += (s_code[sym] >> 24) + (uint32_t)ne + 1u;
+            });
+        return b;
+    };
+    const RowRuns R = row_runs(s_row, n, s_i);
+    uint32_t mine = 0;
+    visit_runs(s_row, R, [&](int v, int len) { mine += run_len(v, len); });
+    uint32_t total;
+    uint32_t pos = block_excl_sum(mine, s_u, total);
+    visit_runs(s_row, R, [&](int v, int len) {
+        run_symbols(
+            v, len,
+            [&](int c, int count) {
+                const uint32_t code = s_code[c];
+                const int l = (int)(code >> 24);
+                for (int k = 0; k < count; ++k) {
+                    put(pos, code & 0xFFFFFFu, l);
+                    pos += l;
+                }
+            },
+            [&](int L) {
+                int sym, ne, base;
+                len_symbol(L, sym, ne, base);
+                const uint32_t code = s_code[sym];
+                const int l = (int)(code >> 24);
+                // length code, extra bits, then the distance code: one 0 bit
+                put(pos, (code & 0xFFFFFFu) | ((uint32_t)(L - base) << l), l + ne);
+                pos += l + ne + 1;
+            });
     });
     __syncthreads();
     const uint64_t off = ws.row_off[r];
@@ -615,14 +882,16 @@ uint32_t host_crc32(const uint8_t* p, size_t n) {
 }
 
 struct WsLayout {
-    size_t filt, row_bits, row_s, row_t, row_off, totals, crc_part, bytes;
+    size_t filt, row_bits, row_s, row_t, row_off, totals, crc_part, hist, extra, code, bytes;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 uint64_t png_bound(int w, int h, int ch) {
+    // no byte costs more than kMaxCodeBits (a literal; a match of 3 or more
+    // bytes costs at most kMaxCodeBits + 6)
     const uint64_t n = (uint64_t)w * ch + 1;
-    const uint64_t bits = 3 + (uint64_t)h * n * 9 + 7;
+    const uint64_t bits = (uint64_t)kHeaderBits + (uint64_t)h * n * kMaxCodeBits + kMaxCodeBits + 7;
     return kHeadBytes + 2 + (bits + 7) / 8 + 4 + 4 + 12 + 8;   // + slack for the last shared word
 }
 
@@ -636,6 +905,9 @@ WsLayout ws_layout(int w, int h, int ch, uint64_t bound) {
     L.row_off = o; o = align256(o + (size_t)h * 8);
     L.totals = o; o = align256(o + 8 * 8);
     L.crc_part = o; o = align256(o + (size_t)((bound + kCrcBlock - 1) / kCrcBlock + 1) * 4);
+    L.hist = o; o = align256(o + (size_t)h * kHistStride * 2);
+    L.extra = o; o = align256(o + (size_t)h * 4);
+    L.code = o; o = align256(o + (size_t)kNumLitLen * 4);
     L.bytes = o;
     return L;
 }
@@ -681,6 +953,9 @@ extern "C" mk_status mk_png_encode_async(const uint8_t* dev_pixels, int width, i
     ws.row_off = (uint64_t*)(w + L.row_off);
     ws.totals = (uint64_t*)(w + L.totals);
     ws.crc_part = (uint32_t*)(w + L.crc_part);
+    ws.hist = (uint16_t*)(w + L.hist);
+    ws.extra = (uint32_t*)(w + L.extra);
+    ws.code = (uint32_t*)(w + L.code);
     // IHDR CRC on the host (the header depends only on the dims)
     uint8_t ih[17] = {'I', 'H', 'D', 'R', (uint8_t)(width >> 24), (uint8_t)(width >> 16), (uint8_t)(width >> 8),
                       (uint8_t)width, (uint8_t)(height >> 24), (uint8_t)(height >> 16), (uint8_t)(height >> 8),
@@ -689,10 +964,11 @@ extern "C" mk_status mk_png_encode_async(const uint8_t* dev_pixels, int width, i
     const int stride = width * channels;
     hipStream_t s = (hipStream_t)stream;
     const size_t row_lds = ((size_t)stride + 1 + 3) / 4 * 4;
-    hipLaunchKernelGGL(k_png_filter_count, dim3(height), dim3(kRowThreads), row_lds, s, dev_pixels, width, height,
+    hipLaunchKernelGGL(k_png_filter_hist, dim3(height), dim3(kRowThreads), row_lds, s, dev_pixels, width, height,
                        channels, ws);
-    hipLaunchKernelGGL(k_png_scan, dim3(1), dim3(1024), 0, s, height, stride, ws, dev_png, ihdr_crc, width, channels);
-    const size_t emit_lds = row_lds + ((size_t)(stride + 1) * 9 / 32 + 2) * 4;
+    hipLaunchKernelGGL(k_huff_scan, dim3(1), dim3(kScanThreads), 0, s, height, stride, ws, dev_png, ihdr_crc, width,
+                       channels);
+    const size_t emit_lds = row_lds + ((size_t)(stride + 1) * kMaxCodeBits / 32 + 2) * 4;
     hipLaunchKernelGGL(k_png_emit, dim3(height), dim3(kRowThreads), emit_lds, s, height, stride, ws, dev_png);
     const uint64_t max_blocks = (bound + kCrcBlock - 1) / kCrcBlock;
     static const PowTable pw = make_pow_table();

@@ -46,3 +46,23 @@ def test_reader_decodes_fpng(shape):
     assert n > 0
     got, _ = read_png(out[:n].tobytes())
     assert np.array_equal(got, a)
+
+
+def test_block_reader_counts_literals_of_a_huffman_only_stream():
+    """The tests' small inflater (deflate_block_stats) against zlib: a
+    Huffman-only stream (no matches) of one dynamic block holds exactly the
+    data's bytes as literals, and its code is complete."""
+    import struct
+    import zlib
+
+    from tests.png_util import deflate_block_stats
+
+    rng = np.random.default_rng(4)
+    data = (np.abs(rng.normal(0, 12, 6000)).astype(np.uint8)).tobytes()
+    co = zlib.compressobj(9, zlib.DEFLATED, 15, 9, zlib.Z_HUFFMAN_ONLY)
+    z = co.compress(data) + co.flush()
+    png = b"\x89PNG\r\n\x1a\n" + struct.pack(">I", len(z)) + b"IDAT" + z + b"\0\0\0\0"
+    ll, dl, counts = deflate_block_stats(png)
+    hist = np.bincount(np.frombuffer(data, np.uint8), minlength=256)
+    assert counts[:256] == hist.tolist() and counts[256] == 1 and sum(counts[257:]) == 0
+    assert sum(2.0 ** -L for L in ll if L) == 1.0

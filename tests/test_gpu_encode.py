@@ -63,8 +63,8 @@ def test_random_and_edge_shapes(shape):
 
 @pytest.mark.parametrize("value", [0, 143, 144, 255])
 def test_constant_images_use_long_runs(value):
-    # runs of every length class: a 4097-byte row is one literal and copies
-    # of 258 plus a remainder; 143/144 straddle the 8/9-bit literal codes
+    # runs of every length class: a 4096-byte row is one literal and copies
+    # of 258 plus a remainder; a handful of symbols makes a very short code
     img = np.full((6, 1365, 3), value, np.uint8)
     png, _ = _check(img)
     assert len(png) < img.size // 20
@@ -151,3 +151,52 @@ def test_argument_errors():
     st = lib.mk_png_encode_async(buf.data_ptr(), 16, 16, 3, buf.data_ptr(), 10, buf.data_ptr(), 10,
                                  buf.data_ptr(), 0)
     assert st != 0 and b"capacity" in lib.mk_last_error(None)
+
+
+def _huffman_lengths(counts):
+    """Textbook Huffman code lengths (unlimited), by repeated merging."""
+    import heapq
+    import itertools
+
+    tick = itertools.count()
+    heap = [(c, next(tick), [s]) for s, c in enumerate(counts) if c > 0]
+    depth = {s: 0 for _, _, ss in heap for s in ss}
+    heapq.heapify(heap)
+    while len(heap) > 1:
+        c1, _, s1 = heapq.heappop(heap)
+        c2, _, s2 = heapq.heappop(heap)
+        for s in s1 + s2:
+            depth[s] += 1
+        heapq.heappush(heap, (c1 + c2, next(tick), s1 + s2))
+    return depth
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_huffman_code_is_complete_limited_and_optimal(seed):
+    from tests.png_util import deflate_block_stats
+
+    img = _vr_like(48, 64, seed=seed)
+    png, _ = _check(img)
+    ll, dl, counts = deflate_block_stats(png)
+    assert max(ll) <= 12 and dl == [1]
+    assert sum(2.0 ** -L for L in ll if L) == 1.0          # complete literal/length code
+    assert all((c > 0) == (L > 0) for c, L in zip(counts, ll) if c > 0)
+    cost = sum(c * L for c, L in zip(counts, ll))
+    opt = _huffman_lengths(counts)
+    if max(opt.values()) <= 12:                             # then the limit does not bind: Huffman's cost is the optimum
+        assert cost == sum(counts[s] * d for s, d in opt.items())
+    else:
+        assert cost >= sum(counts[s] * d for s, d in opt.items())
+
+
+def test_skewed_counts_hit_the_length_limit():
+    from tests.png_util import deflate_block_stats
+
+    # one byte value dominates, a few are rare: the unlimited Huffman code is deeper than 12
+    img = np.zeros((64, 64, 4), np.uint8)
+    rng = np.random.default_rng(9)
+    pts = rng.integers(0, 64, (400, 2))
+    img[pts[:, 0], pts[:, 1]] = rng.integers(1, 256, (400, 4), dtype=np.uint8)
+    png, _ = _check(img)
+    ll, _, counts = deflate_block_stats(png)
+    assert max(ll) <= 12 and sum(2.0 ** -L for L in ll if L) == 1.0
