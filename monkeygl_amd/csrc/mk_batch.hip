@@ -66,9 +66,11 @@ __global__ void __launch_bounds__(256) k_batch(const uint8_t* __restrict__ block
         extern __shared__ float4 smem[];
         mkr::Smem s;
         s.aw = smem;
-        s.lut = smem + 68;
+        s.lab = (int4*)(smem + mkr::kLabelRows);
+        s.lut = smem + 2 * mkr::kLabelRows;
         s.aobj = (float*)(s.lut + kBatchLutLds);
-        for (int i = threadIdx.x; i < 68; i += blockDim.x) s.aw[i] = d->aw[i];
+        for (int i = threadIdx.x; i < mkr::kLabelRows; i += blockDim.x) s.aw[i] = d->aw[i];
+        mkr::stage_labels(d->ra, s.lab);
         if constexpr (kLds)
             for (int i = threadIdx.x; i < d->ra.nlut_lds; i += blockDim.x) s.lut[i] = d->ra.lut[i];
         __syncthreads();

@@ -37,9 +37,9 @@ enum Variant : int {
 
 constexpr size_t lds_bytes(int v) {
     return v == V_VR_MASK_LDS || v == V_VR_MASK_GLB
-               ? 68 * sizeof(float4) + kBatchLutLds * sizeof(float4) + kBatchLabelSlots * 256 * sizeof(float)
-           : v == V_VR_LDS || v == V_VR_GLB ? 68 * sizeof(float4) + kBatchLutLds * sizeof(float4)
-           : v == V_MIP_MASK || v == V_MIP ? 68 * sizeof(float4)
+               ? mkr::kLabelLdsBytes + kBatchLutLds * sizeof(float4) + kBatchLabelSlots * 256 * sizeof(float)
+           : v == V_VR_LDS || v == V_VR_GLB ? mkr::kLabelLdsBytes + kBatchLutLds * sizeof(float4)
+           : v == V_MIP_MASK || v == V_MIP ? mkr::kLabelLdsBytes
                                             : 0;
 }
 

@@ -223,10 +223,10 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     A.nlut_lds = lds_lut ? total_lut : 0;
 
     int tw = 16;
-    size_t lds = 68 * sizeof(float4) + (size_t)A.nlut_lds * sizeof(float4) + (size_t)A.nslot * 256 * sizeof(float);
+    size_t lds = kLabelLdsBytes + (size_t)A.nlut_lds * sizeof(float4) + (size_t)A.nslot * 256 * sizeof(float);
     if (lds > kMaxLdsBytes) {
         tw = 8;
-        lds = 68 * sizeof(float4) + (size_t)A.nlut_lds * sizeof(float4) + (size_t)A.nslot * 64 * sizeof(float);
+        lds = kLabelLdsBytes + (size_t)A.nlut_lds * sizeof(float4) + (size_t)A.nslot * 64 * sizeof(float);
         if (lds > kMaxLdsBytes) return mk_fail(ctx, MK_ERR_ARG, "too many labels for the LDS budget");
     }
     const uint32_t tiles = (uint32_t)(((p->width + tw - 1) / tw) * ((p->height + tw - 1) / tw));

@@ -132,12 +132,10 @@ __device__ inline uint32_t macro_index(const RenderArgs& A, int cx, int cy, int 
     return ((uint32_t)(cz >> 5) * (uint32_t)A.nmy + (uint32_t)(cy >> 5)) * (uint32_t)A.nmx + (uint32_t)(cx >> 5);
 }
 
-// tex1D linear/clamp of label's LUT at normalized s.
+// tex1D linear/clamp at normalized s of the LUT of n entries at off.
 template <bool LDS>
-__device__ inline float4 lut_sample(const RenderArgs& A, const float4* s_lut, int label, float s) {
-    int n = A.tab.len[label];
+__device__ inline float4 lut_sample(const RenderArgs& A, const float4* s_lut, int off, int n, float s) {
     if (n <= 0) return make_float4(0.f, 0.f, 0.f, 0.f);
-    int off = A.tab.off[label];
     s = s >= -1.0f ? s : -1.0f;
     int i0, i1;
     float a;
@@ -330,20 +328,34 @@ __device__ inline bool tile_pixel(const RenderArgs& A, int& x, int& y) {
     return x < A.width && y < A.height;
 }
 
-// Shared-memory carve-up common to the kernels.
+// Shared-memory carve-up common to the kernels. Per-label values a ray
+// indexes by its sample's label live here rather than in the kernel
+// arguments, whose per-lane indexing would be a memory round trip each.
+constexpr int kLabelRows = 68;   // MK_NUM_LABELS rounded up
 struct Smem {
-    float4* aw;     // MK_NUM_LABELS x (alpha, ww, wl, -)
+    float4* aw;     // kLabelRows x (alpha, ww, wl, -)
+    int4* lab;      // kLabelRows x (LUT offset, LUT length, accumulator slot, -)
     float4* lut;    // nlut_lds entries
     float* aobj;    // nslot x blockDim
 };
+
+// LDS bytes of the carve-up ahead of the LUT entries.
+constexpr size_t kLabelLdsBytes = kLabelRows * (sizeof(float4) + sizeof(int4));
+
+__device__ inline void stage_labels(const RenderArgs& A, int4* lab) {
+    for (int i = threadIdx.x; i < MK_NUM_LABELS; i += blockDim.x)
+        lab[i] = make_int4(A.tab.off[i], A.tab.len[i], (int)A.slot_of[i], 0);
+}
 
 __device__ inline Smem stage_shared(const RenderArgs& A, const AwTable& T, bool stage_lut) {
     extern __shared__ float4 smem[];
     Smem s;
     s.aw = smem;
-    s.lut = smem + 68;
+    s.lab = (int4*)(smem + kLabelRows);
+    s.lut = smem + 2 * kLabelRows;
     s.aobj = (float*)(s.lut + A.nlut_lds);
     for (int i = threadIdx.x; i < MK_NUM_LABELS; i += blockDim.x) s.aw[i] = make_float4(T.alpha[i], T.ww[i], T.wl[i], 0.f);
+    stage_labels(A, s.lab);
     if (stage_lut)
         for (int i = threadIdx.x; i < A.nlut_lds; i += blockDim.x) s.lut[i] = A.lut[i];
     __syncthreads();
@@ -417,10 +429,11 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
                     touch_footprint(A, p[0], p[1], p[2]);
                 }
                 const float4 aw = s.aw[label];
+                const int4 lb = s.lab[label];
                 float t = sample_keep(A.v, p, tri);
                 t = (t - aw.z) / aw.y + 0.5f;
                 t = t > 1.0f ? 1.0f : t;
-                col = lut_sample<LDS_LUT>(A, s.lut, label, t);
+                col = lut_sample<LDS_LUT>(A, s.lut, lb.x, lb.y, t);
                 a = col.w;
             }
             // getNextStep (kernel.cu:84-125)
@@ -437,7 +450,7 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
             acc += step;
             if (a > 0.0005f) {
                 const float cap = s.aw[label].x;
-                float* slot = MASK ? &s.aobj[A.slot_of[label] * BS + tid] : &aobj0;
+                float* slot = MASK ? &s.aobj[s.lab[label].z * BS + tid] : &aobj0;
                 if (*slot < cap) {
                     // tracing (kernel.cu:49-81)
                     if (TOUCH) touch_gradient(A, p);

@@ -167,3 +167,45 @@ def test_footprint_matches_oracle(gpu_ctx, small, rt):
     # the measurement variant renders the same frame
     frac, dmax = frac_within(out.cpu().numpy().reshape(p.height, p.width, 3), img)
     assert frac >= MIN_FRAC, (frac, dmax)
+
+
+def test_skipping_is_exact_with_capped_label(gpu_ctx, small):
+    """Empty-cell skipping and the skip tables change nothing: the frame
+    equals, bit for bit, the measurement variant's, which samples every
+    lattice point. Label 0's LUT has alpha 0.05 everywhere and a cap of 0.3,
+    so rays reach the cap early and then march at L1/8 through label-0
+    tissue, between ellipsoids of labels with their own caps."""
+    import torch
+
+    vol, _ = small
+    mask = W.ellipsoid_mask((128, 128, 128), 3, seed=5)
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_mask(mask)
+    lut0 = np.zeros((100, 4), np.float32)
+    lut0[:, 0] = np.linspace(0.2, 1.0, 100)
+    lut0[:, 1] = 0.5
+    lut0[:, 2] = np.linspace(1.0, 0.1, 100)
+    lut0[:, 3] = 0.05
+    luts = {0: lut0}
+    labels = {0: P.LabelState(0.3, 2000.0, 0.0)}
+    for l in range(1, 4):
+        luts[l] = P.transfer_function(W.tf_points(20 + l))
+        labels[l] = P.LabelState([1.0, 0.5, 0.8][l - 1], 1600.0, -50.0 * l)
+    for l, a in luts.items():
+        gpu_ctx.set_lut(l, a)
+    s = torch.cuda.current_stream().cuda_stream
+    for ang in [(0, 0), (40, 10), (-75, 110)]:
+        p = _params(MK_RENDER_VR, ang, labels=labels)
+        got = gpu_ctx.render(p)
+        out = torch.empty(p.width * p.height * 3, dtype=torch.uint8, device="cuda")
+        tv = torch.zeros(vol.size, dtype=torch.uint8, device="cuda")
+        gpu_ctx.render_footprint(p, out, tv, None, stream=s)
+        torch.cuda.synchronize()
+        full = out.cpu().numpy().reshape(p.height, p.width, 3)
+        assert (got != 0).any()
+        assert np.array_equal(got, full), (ang, int((got != full).any(axis=-1).sum()))
+        if ang == (40, 10):
+            ref = O.render(p, vol, luts, mask_zyx=mask)
+            frac, dmax = frac_within(got, ref)
+            assert frac >= MIN_FRAC, (frac, dmax)
+    gpu_ctx.clear_mask()
