@@ -12,8 +12,9 @@ The JSON line carries:
   * value / ms_per_step: frames per second over K back-to-back frames;
   * roofline: algorithmic bytes per frame (2 B per distinct voxel the
     reference rule reads, incl. gradient taps, + 1 B per mask voxel read
-    + 3 B per output pixel) over the VR kernel's mean duration from HIP events
-    on the launch stream, against the 8 TB/s HBM peak;
+    + 3 B per output pixel) over the VR frame's mean duration from HIP events
+    on the launch stream (both passes: k_vr, then k_vr_tail for the rays it
+    queued), against the 8 TB/s HBM peak;
   * parity: a band of rows of one frame against the FP64 oracle;
   * cpu_baseline: the oracle's float instance (8-bit filter weights) on a
     band of rows of the same frame, converted to frames per second;
@@ -649,7 +650,7 @@ def main() -> int:
         "roofline": None if achieved is None else {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-            "bytes_per_frame": int(b_frame), "kernel": "k_vr", "achievable_copy_gbs": round(copy_gbs, 1),
+            "bytes_per_frame": int(b_frame), "kernel": "k_vr + k_vr_tail", "achievable_copy_gbs": round(copy_gbs, 1),
             "frac_of_copy": round(achieved / copy_gbs, 5)},
         "parity": parity,
         "cpu_baseline": cpu,

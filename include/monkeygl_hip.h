@@ -195,8 +195,18 @@ mk_status mk_set_transfer_func(mk_context* ctx, int label, const float* rgba, in
 mk_status mk_render(mk_context* ctx, const mk_render_params* p, uint8_t* host_rgb);
 
 /* Same frame into device memory (width*height*3 bytes) on the given stream;
- * returns after enqueueing. */
+ * returns after enqueueing. VR renders of one context share its queue of
+ * second-pass rays (see mk_set_vr_trip_budget): one enqueued on another
+ * stream than the previous one waits on the device for the previous one. */
 mk_status mk_render_async(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb, void* stream);
+
+/* VR in two passes (no reference counterpart): the first pass marches every
+ * ray for up to `trips` loop trips; the rays still marching then continue in
+ * a second pass that evaluates eight consecutive samples of a ray at once and
+ * applies the reference's loop (core/kernel.cu:133-264) to them in order. The
+ * frame is the same for every value; 0 renders in one pass. Default 160.
+ * Frames whose queue would exceed 2 GiB render in one pass. */
+mk_status mk_set_vr_trip_budget(mk_context* ctx, int trips);
 
 /* Measurement variant of mk_render_async (no reference counterpart): renders
  * the same frame without empty-space skipping and sets to 1 every voxel the

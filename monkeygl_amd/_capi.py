@@ -28,7 +28,7 @@ SYMBOLS = [
     "mk_context_stream", "mk_synchronize", "mk_copy_volume_data", "mk_copy_volume_data_device",
     "mk_copy_mask_data", "mk_copy_mask_data_device", "mk_clear_mask", "mk_merge_object_mask",
     "mk_merge_object_mask_device", "mk_get_mask_data", "mk_origin_slice_async", "mk_set_transfer_func",
-    "mk_render", "mk_render_async", "mk_render_footprint", "mk_render_plane", "mk_render_plane_async",
+    "mk_render", "mk_render_async", "mk_set_vr_trip_budget", "mk_render_footprint", "mk_render_plane", "mk_render_plane_async",
     "mk_render_cpr", "mk_render_cpr_async", "mk_marching_cubes", "mk_marching_cubes_async",
     "mk_tf_forward", "mk_tf_backward", "mk_set_orientation_mesh", "mk_orientation_workspace_bytes",
     "mk_merge_orientation_box_async", "mk_draw_polyline_async", "mk_png_max_bytes", "mk_png_workspace_bytes", "mk_png_encode_async",
@@ -123,6 +123,7 @@ def load() -> C.CDLL:
         "mk_render": (st, [vp, C.POINTER(RenderParams), vp]),
         "mk_render_async": (st, [vp, C.POINTER(RenderParams), vp, vp]),
         "mk_render_footprint": (st, [vp, C.POINTER(RenderParams), vp, vp, vp, vp]),
+        "mk_set_vr_trip_budget": (st, [vp, C.c_int]),
         "mk_render_plane": (st, [vp, C.POINTER(PlaneParams), vp]),
         "mk_render_plane_async": (st, [vp, C.POINTER(PlaneParams), vp, vp]),
         "mk_render_cpr": (st, [vp, i, i, vp, vp, i, i, vp]),
@@ -274,6 +275,10 @@ class Context:
 
     def render_async(self, p: RenderParams, dev_out, stream: int = 0) -> None:
         self._check(self.lib.mk_render_async(self.h, C.byref(p), _ptr(dev_out), stream))
+
+    def set_vr_trip_budget(self, trips: int) -> None:
+        """Loop trips of the first VR pass (0: one pass); the frame is the same for every value."""
+        self._check(self.lib.mk_set_vr_trip_budget(self.h, int(trips)))
 
     def render_footprint(self, p: RenderParams, dev_out, dev_touch_vol, dev_touch_mask=None, stream: int = 0) -> None:
         self._check(self.lib.mk_render_footprint(self.h, C.byref(p), _ptr(dev_out), _ptr(dev_touch_vol),

@@ -35,6 +35,10 @@ HIP_FLAGS = [
 ]
 # files whose arithmetic must match the host restatement bit for bit
 NO_CONTRACT = {"mk_plane.hip", "mk_mc.hip", "mk_batch_plane.hip", "mk_overlay.hip"}
+# files whose kernels must agree bit for bit with each other (one- and two-pass
+# VR, plain and batched): multiply-adds are fused only within one source
+# expression, never across expressions by the backend
+CONTRACT_ON = {"mk_render.hip", "mk_batch.hip"}
 
 HIP_SOURCES = ["mk_volume.hip", "mk_render.hip", "mk_plane.hip", "mk_mc.hip", "mk_fit.hip", "mk_batch.hip",
                "mk_batch_plane.hip", "mk_encode.hip", "mk_overlay.hip"]
@@ -55,6 +59,8 @@ def _hip_object(src: str) -> Path:
     flags = list(HIP_FLAGS)
     if src in NO_CONTRACT:
         flags.append("-ffp-contract=off")
+    elif src in CONTRACT_ON:
+        flags.append("-ffp-contract=on")
     _run([HIPCC, *flags, "-c", str(CSRC / src), "-o", str(out)])
     return out
 

@@ -35,6 +35,9 @@ struct MkLutTable {
     int len[MK_NUM_LABELS];
 };
 
+// Loop trips of the first VR pass before a ray moves to the second.
+constexpr int kVrTripBudget = 160;
+
 // Skip tables of one render type. occ/occ2: brick / macro cell (4x4x4 bricks)
 // may contribute (VR, Surface); mipb/mipb2: upper bound of the windowed value
 // (MIP). Valid for the (window/alpha palette, LUT version, volume version)
@@ -75,6 +78,16 @@ struct mk_context {
 
     // per-frame skip tables, one set per render type, cached on their inputs
     MkSkipTables skip[3];
+
+    // VR second pass: queue of the rays still marching after vr_budget loop
+    // trips of the first pass (0: one pass); renders on other streams are
+    // ordered after the last second pass through cont_done
+    int vr_budget = kVrTripBudget;
+    void* d_cont = nullptr;
+    size_t cont_bytes = 0;
+    hipEvent_t cont_done = nullptr;
+    hipStream_t cont_stream = nullptr;
+    bool cont_used = false;
     uint32_t vol_version = 0;
 
     // exact float step lattice a_{k+1} = a_k + L1 from a_0 = 0

@@ -31,6 +31,11 @@ namespace {
 
 constexpr int kMaxLdsBytes = 64 * 1024;
 constexpr int kLutLdsEntries = 1024;  // stage LUTs in LDS up to 16 KiB
+constexpr size_t kContHeader = 256;                  // queue counters
+constexpr size_t kContMaxBytes = size_t(2) << 30;    // larger frames render in one pass
+constexpr size_t kTailBlocks = 2048;                 // second-pass workgroups (8 per CU)
+constexpr float kLongSteps = 256.0f;                 // second-pass rays claimed first
+constexpr int kTailGroup = 8;                        // second-pass lanes per ray
 
 using namespace mkr;
 
@@ -40,6 +45,18 @@ __global__ void __launch_bounds__(TW * TW) k_vr(RenderArgs A, AwTable T) {
     int x, y;
     if (!tile_pixel<TW>(A, x, y)) return;
     vr_pixel<TW * TW, MASK, LDS_LUT, TOUCH>(A, s, x, y, threadIdx.x);
+}
+
+// Second VR pass over the rays the first pass queued: 256 / kTailGroup rays in
+// flight per workgroup (kTailGroup lanes per ray), claimed from the queue.
+template <bool MASK, bool LDS_LUT>
+__global__ void __launch_bounds__(256) k_vr_tail(RenderArgs A, AwTable T) {
+    Smem s = stage_shared(A, T, LDS_LUT);
+    vr_tail<MASK, LDS_LUT, kTailGroup>(A, s, s.aobj + (threadIdx.x / kTailGroup) * (MASK ? A.nslot : 0));
+}
+
+__global__ void k_zero_u32(uint32_t* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
 template <int TW, bool MASK, bool TOUCH>
@@ -234,6 +251,46 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     // the skip tables were (re)built on the context stream
     st = mk_order_after_context(ctx, s);
     if (st) return st;
+
+    // VR in two passes: rays still marching after the trip budget are queued
+    // and finished eight lanes per ray. The queue has a slot per pixel.
+    const bool touch = touch_v != nullptr;
+    const size_t npix = (size_t)p->width * (size_t)p->height;
+    const int cslots = mask ? std::max(A.nslot, 1) : 1;
+    const size_t cont_bytes = kContHeader + npix * (sizeof(float4) + 6 * sizeof(float) + (size_t)cslots * sizeof(float));
+    const bool two_pass = p->render_type == MK_RENDER_VR && !touch && ctx->vr_budget > 0 && cont_bytes <= kContMaxBytes;
+    if (two_pass) {
+        // a larger queue replaces the buffer: the last second pass must be done with it
+        if (ctx->cont_used && ctx->cont_bytes < cont_bytes) MK_HIP(ctx, hipEventSynchronize(ctx->cont_done));
+        st = mk_ensure_buffer(ctx, &ctx->d_cont, &ctx->cont_bytes, cont_bytes);
+        if (st) return st;
+        if (!ctx->cont_done) MK_HIP(ctx, hipEventCreateWithFlags(&ctx->cont_done, hipEventDisableTiming));
+        // renders share the queue: one on another stream waits for the last
+        if (ctx->cont_used && ctx->cont_stream != s) MK_HIP(ctx, hipStreamWaitEvent(s, ctx->cont_done, 0));
+        uint8_t* q = (uint8_t*)ctx->d_cont;
+        VrCont& C = A.cont;
+        C.counters = (uint32_t*)q;
+        q += kContHeader;
+        C.sum = (float4*)q;
+        q += npix * sizeof(float4);
+        C.pix = (int*)q;
+        q += npix * sizeof(int);
+        C.acc = (float*)q;
+        q += npix * sizeof(float);
+        C.step = (float*)q;
+        q += npix * sizeof(float);
+        C.apre = (float*)q;
+        q += npix * sizeof(float);
+        C.aacc = (float*)q;
+        q += npix * sizeof(float);
+        C.it = (int*)q;
+        q += npix * sizeof(int);
+        C.aobj = (float*)q;
+        C.cap = (int)npix;
+        C.budget = ctx->vr_budget;
+        C.long_steps = kLongSteps;
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, C.counters, 3);
+    }
     dim3 grid(tiles), block(tw * tw);
 #define MK_LAUNCH(K) hipLaunchKernelGGL(K, grid, block, lds, s, A, T)
 #define MK_VR(TOUCH)                                                                              \
@@ -243,9 +300,20 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     } else {                                                                                      \
         if (lds_lut) MK_LAUNCH((k_vr<8, true, true, TOUCH>)); else MK_LAUNCH((k_vr<8, true, false, TOUCH>)); \
     }
-    const bool touch = touch_v != nullptr;
     if (p->render_type == MK_RENDER_VR) {
         if (touch) { MK_VR(true) } else { MK_VR(false) }
+        if (two_pass) {
+            const size_t groups = 256 / kTailGroup;
+            const size_t lds2 = kLabelLdsBytes + (size_t)A.nlut_lds * sizeof(float4) + groups * (size_t)cslots * sizeof(float);
+            const dim3 grid2((uint32_t)std::min<size_t>((npix + groups - 1) / groups, kTailBlocks)), block2(256);
+#define MK_TAIL(K) hipLaunchKernelGGL(K, grid2, block2, lds2, s, A, T)
+            if (mask) { if (lds_lut) MK_TAIL((k_vr_tail<true, true>)); else MK_TAIL((k_vr_tail<true, false>)); }
+            else { if (lds_lut) MK_TAIL((k_vr_tail<false, true>)); else MK_TAIL((k_vr_tail<false, false>)); }
+#undef MK_TAIL
+            MK_HIP(ctx, hipEventRecord(ctx->cont_done, s));
+            ctx->cont_stream = s;
+            ctx->cont_used = true;
+        }
     } else if (p->render_type == MK_RENDER_MIP) {
         if (touch) { if (mask) MK_LAUNCH((k_mip<16, true, true>)); else MK_LAUNCH((k_mip<16, false, true>)); }
         else { if (mask) MK_LAUNCH((k_mip<16, true, false>)); else MK_LAUNCH((k_mip<16, false, false>)); }
@@ -310,6 +378,13 @@ mk_status mk_ensure_steps(mk_context* ctx, float l1) {
 extern "C" mk_status mk_render_async(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb, void* stream) {
     if (!ctx) return MK_ERR_ARG;
     return launch_render(ctx, p, dev_rgb, stream, nullptr, nullptr);
+}
+
+extern "C" mk_status mk_set_vr_trip_budget(mk_context* ctx, int trips) {
+    if (!ctx) return MK_ERR_ARG;
+    if (trips < 0) return mk_fail(ctx, MK_ERR_ARG, "trip budget %d is negative", trips);
+    ctx->vr_budget = trips;
+    return MK_OK;
 }
 
 extern "C" mk_status mk_render_footprint(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb,

@@ -9,6 +9,28 @@
 
 namespace mkr {
 
+// VR rays still marching after the first pass's trip budget, queued with the
+// state the reference's loop carries from one iteration to the next
+// (core/kernel.cu:133-264: position t, step, alpha of the previous sample,
+// accumulated alpha and colour, per-label accumulated alpha) plus the trip
+// count. Structure of arrays over the queue; cap = 0 disables the queue.
+struct VrCont {
+    uint32_t* counters;   // [0] rays queued from the front, [1] rays claimed by
+                          // the second pass, [2] rays queued from the back
+    int* pix;             // y * width + x
+    float* acc;
+    float* step;
+    float* apre;
+    float* aacc;
+    int* it;
+    float4* sum;
+    float* aobj;          // [slot * cap + ray]
+    int cap;
+    int budget;           // loop trips of the first pass
+    float long_steps;     // rays with at least this many L1 steps left to the
+                          // end of their interval queue from the front
+};
+
 struct RenderArgs {
     MkVolView v;
     const float4* lut;
@@ -39,6 +61,7 @@ struct RenderArgs {
     float l1;
     float exit_alpha;
     int max_iter;      // hard bound on loop trips per ray
+    VrCont cont;       // VR second pass (cap 0 in every other use)
 };
 
 struct AwTable {
@@ -46,6 +69,10 @@ struct AwTable {
     float ww[MK_NUM_LABELS];
     float wl[MK_NUM_LABELS];
 };
+
+// The reference's loop bound t < 1.732 (a double comparison of the float t):
+// 1.732f is the largest float below 1.732, so the bound is t <= 1.732f.
+constexpr float kAccEnd = 1.732f;
 
 __device__ inline void ray_pos(const RenderArgs& A, float u, float v, float acc, float p[3]) {
     float fy = (float)(((double)acc - 0.866) * A.scale);
@@ -378,6 +405,68 @@ __device__ inline void skip_lattice(float& acc, float l1, float a_skip, int& it,
     }
 }
 
+// Alpha of a sample taken at a refined step, as getNextStep corrects it
+// (core/kernel.cu:84-125): 1 - (1 - a)^(step / L1).
+__device__ inline float vr_step_alpha(float a, float step, float L1, float L4) {
+    if (a > 0.f && step != L1) a = 1.0f - __builtin_amdgcn_exp2f((step == L4 ? 0.25f : 0.125f) * __builtin_amdgcn_logf(1.0f - a));
+    return a;
+}
+
+// getNextStep's decision (core/kernel.cu:84-125) for a sample of corrected
+// alpha a at acc: true when it backs up (acc and step changed, the sample is
+// dropped); otherwise the sample is accepted, aPre becomes a and acc advances
+// by the (possibly changed) step.
+__device__ inline bool vr_step_back(float& acc, float& step, float& aPre, float a, float L1, float L4, float L8) {
+    if (acc > 0.f) {
+        if (fmaxf(a, aPre) > 0.001f) {
+            if (step == L1) { acc -= (L1 - L4); step = L4; return true; }
+            if (step == L4) { acc -= (L4 - L8); step = L8; return true; }
+        } else {
+            step = (step == L8) ? L4 : L1;
+        }
+    }
+    aPre = a;
+    acc += step;
+    return false;
+}
+
+// tracing's shading factor (core/kernel.cu:49-81) for normal N.
+__device__ inline float vr_shade(const RenderArgs& A, const float3 N) {
+    const float3 L = light_dir(A);
+    const float d = N.x * L.x + N.y * L.y + N.z * L.z;
+    float k = 0.35f;
+    if (d > 0.f) {
+        float d2 = d * d, d4 = d2 * d2;
+        k += d * 0.8f + 0.16f * (d4 * d4);
+    }
+    return k;
+}
+
+// Front-to-back compositing of an accepted sample (core/kernel.cu:225-240).
+__device__ inline void vr_composite(float4& sum, float& aAcc, float& slot, float a, float cr, float cg, float cb,
+                                    float k) {
+    const float w = (1.0f - aAcc) * a;
+    sum.x += w * (cr * k);
+    sum.y += w * (cg * k);
+    sum.z += w * (cb * k);
+    sum.w += w * (a * k);
+    slot += w;
+    aAcc += w;
+}
+
+// vr_composite with the products colour * k already formed (the same
+// roundings: each product is one multiply in either form).
+__device__ inline void vr_composite_k(float4& sum, float& aAcc, float& slot, float a, float ckr, float ckg, float ckb,
+                                      float cka) {
+    const float w = (1.0f - aAcc) * a;
+    sum.x += w * ckr;
+    sum.y += w * ckg;
+    sum.z += w * ckb;
+    sum.w += w * cka;
+    slot += w;
+    aAcc += w;
+}
+
 // d_render (core/kernel.cu:133-264) for pixel (x, y); tid indexes the
 // per-label accumulators of a block of BS threads. TOUCH: measurement variant
 // that skips nothing and marks every voxel the reference rule reads.
@@ -399,7 +488,11 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
         // moves to another brick
         uint32_t cur_brick = 0xFFFFFFFFu;
         int cur_macro_empty = 0, cur_occ = 1, cur_label = 0xFF;
-        for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
+        int trips = 0, it_queued = -1;
+        for (int it = 0; it < A.max_iter && acc <= kAccEnd && acc <= a_out; ++it) {
+            // a ray still marching after the budget is queued for the second
+            // pass (all lanes of a wave reach the budget on the same trip)
+            if (!TOUCH && A.cont.cap > 0 && trips++ == A.cont.budget) { it_queued = it; break; }
             float p[3];
             ray_pos(A, u, v, acc, p);
             int ix, iy, iz;
@@ -437,38 +530,16 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
                 a = col.w;
             }
             // getNextStep (kernel.cu:84-125)
-            if (a > 0.f && step != L1) a = 1.0f - __builtin_amdgcn_exp2f((step == L4 ? 0.25f : 0.125f) * __builtin_amdgcn_logf(1.0f - a));
-            if (acc > 0.f) {
-                if (fmaxf(a, aPre) > 0.001f) {
-                    if (step == L1) { acc -= (L1 - L4); step = L4; continue; }
-                    if (step == L4) { acc -= (L4 - L8); step = L8; continue; }
-                } else {
-                    step = (step == L8) ? L4 : L1;
-                }
-            }
-            aPre = a;
-            acc += step;
+            a = vr_step_alpha(a, step, L1, L4);
+            if (vr_step_back(acc, step, aPre, a, L1, L4, L8)) continue;
             if (a > 0.0005f) {
                 const float cap = s.aw[label].x;
                 float* slot = MASK ? &s.aobj[s.lab[label].z * BS + tid] : &aobj0;
                 if (*slot < cap) {
                     // tracing (kernel.cu:49-81)
                     if (TOUCH) touch_gradient(A, p);
-                    const float3 N = gradient_kept(A, tri);
-                    const float3 L = light_dir(A);
-                    const float d = N.x * L.x + N.y * L.y + N.z * L.z;
-                    float k = 0.35f;
-                    if (d > 0.f) {
-                        float d2 = d * d, d4 = d2 * d2;
-                        k += d * 0.8f + 0.16f * (d4 * d4);
-                    }
-                    const float w = (1.0f - aAcc) * a;
-                    sum.x += w * (col.x * k);
-                    sum.y += w * (col.y * k);
-                    sum.z += w * (col.z * k);
-                    sum.w += w * (a * k);
-                    *slot += w;
-                    aAcc += w;
+                    const float k = vr_shade(A, gradient_kept(A, tri));
+                    vr_composite(sum, aAcc, *slot, a, col.x, col.y, col.z, k);
                 }
             }
             if (aAcc > A.exit_alpha) break;
@@ -476,9 +547,245 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
             // it is accepted with alpha 0 and changes nothing but acc
             if (empty_shift && step == L1) skip_lattice(acc, L1, cell_exit(A, g, cx, cy, cz, empty_shift), it, A.max_iter);
         }
+        if (!TOUCH && it_queued >= 0) {
+            // one queue slot per lane, one atomic per wave and end; the queue
+            // holds every pixel of the frame, so it cannot overflow. Rays with
+            // a long way left fill it from the front, the others from the
+            // back, and the second pass takes the front first: the longest
+            // work starts first.
+            const VrCont& C = A.cont;
+            const bool lng = (a_out - acc) >= C.long_steps * L1;
+            const uint64_t m = __ballot(lng);
+            const uint64_t all = __ballot(1);
+            const uint64_t mine = lng ? m : (all & ~m);
+            const int lane = __lane_id();
+            const int leader = __ffsll((long long)mine) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&C.counters[lng ? 0 : 2], (uint32_t)__popcll(mine));
+            base = (uint32_t)__shfl((int)base, leader);
+            const uint32_t k = base + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull));
+            const uint32_t r = lng ? k : (uint32_t)C.cap - 1u - k;
+            C.pix[r] = y * A.width + x;
+            C.acc[r] = acc;
+            C.step[r] = step;
+            C.apre[r] = aPre;
+            C.aacc[r] = aAcc;
+            C.it[r] = it_queued;
+            C.sum[r] = sum;
+            if (MASK) {
+                for (int k = 0; k < A.nslot; ++k) C.aobj[(size_t)k * C.cap + r] = s.aobj[k * BS + tid];
+            } else {
+                C.aobj[r] = aobj0;
+            }
+            return;
+        }
     }
     if (sum.x == 0.f && sum.y == 0.f && sum.z == 0.f && sum.w == 0.f) sum = bg4(A);
     store_rgb(A, x, y, sum);
+}
+
+// Second pass over the queued rays: G lanes per ray. While the step stays the
+// same, the positions of a ray's next samples are known (the reference's float
+// accumulation acc + step + step ...), so lane j fetches and evaluates the
+// j-th of them (and its shading, if it may composite). Within such a run a
+// sample depends on the ones before it only through aPre (the alpha of the
+// last sample in the VOI before it), the step and the early exit; so each lane
+// finds its own aPre and the event its sample causes in the reference's loop
+// (core/kernel.cu:133-264: end of the ray, backing up, or a step change after
+// it), the first event ends the run, and the samples that may composite are
+// composited in order up to there with the same per-sample code as vr_pixel.
+// The frame is the same, bit for bit, as one pass's. gobj: this group's
+// per-label accumulators in LDS (MASK).
+template <bool MASK, bool LDS_LUT, int G>
+__device__ inline void vr_tail(const RenderArgs& A, const Smem& s, float* gobj) {
+    static_assert(G == 4 || G == 8 || G == 16 || G == 32 || G == 64, "group size");
+    const int lane = __lane_id();
+    const int j = lane & (G - 1), base = lane - j;
+    const uint64_t gbits = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    const float L1 = A.l1, L4 = L1 / 4.0f, L8 = L1 / 8.0f;
+    const VrCont& C = A.cont;
+    const uint32_t n_front = C.counters[0], total = n_front + C.counters[2];
+    bool have = false, fin = false;
+    int pix = 0;
+    float u = 0.f, v = 0.f, a_out = 0.f;
+    RayGeom g;
+    float acc = 0.f, step = L1, aPre = 0.f, aAcc = 0.f, aobj0 = 0.f;
+    float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+    int it = 0;
+    // this lane's brick cache
+    uint32_t cur_brick = 0xFFFFFFFFu;
+    int cur_macro_empty = 0, cur_occ = 1, cur_label = 0xFF;
+    for (;;) {
+        if (!have && !fin) {
+            uint32_t r = 0;
+            if (j == 0) r = atomicAdd(&C.counters[1], 1u);
+            r = (uint32_t)__shfl((int)r, base);
+            if (r >= total || total > (uint32_t)C.cap) {
+                fin = true;
+            } else {
+                if (r >= n_front) r = (uint32_t)C.cap - 1u - (r - n_front);
+                pix = C.pix[r];
+                const int x = pix % A.width, y = pix / A.width;
+                u = ((float)x - (float)A.width / 2.0f - A.xT) / (float)A.height;
+                v = ((float)y - (float)A.height / 2.0f - A.yT) / (float)A.height;
+                float a_in;
+                (void)ray_interval(A, u, v, g, a_in, a_out);
+                acc = C.acc[r];
+                step = C.step[r];
+                aPre = C.apre[r];
+                aAcc = C.aacc[r];
+                it = C.it[r];
+                sum = C.sum[r];
+                if (MASK) {
+                    for (int k = 0; k < A.nslot; ++k) gobj[k] = C.aobj[(size_t)k * C.cap + r];
+                } else {
+                    aobj0 = C.aobj[r];
+                }
+                have = true;
+            }
+        }
+        if (__ballot(have) == 0ull) break;
+        if (!have) continue;
+
+        // this lane's sample: the j-th position from acc at the current step
+        float accj = acc;
+        for (int q = 0; q < j; ++q) accj += step;
+        float p[3];
+        ray_pos(A, u, v, accj, p);
+        int ix, iy, iz;
+        const bool inv = in_voi(A, p, ix, iy, iz);
+        // for a sample that may composite: the colour-times-shade products,
+        // the label's cap and its accumulator slot. A sample in an empty cell
+        // reads alpha 0; the lattice points vr_pixel would skip after it are
+        // visited one by one, with the same result.
+        bool cand = false;
+        int slot_i = 0;
+        float a = 0.f, kr = 0.f, kg = 0.f, kb = 0.f, ka = 0.f, cap = 0.f;
+        if (inv) {
+            const int cx = mk_clampi(ix, 0, A.v.nx - 1), cy = mk_clampi(iy, 0, A.v.ny - 1), cz = mk_clampi(iz, 0, A.v.nz - 1);
+            const uint32_t b = mk_brick_index(A.v, cx, cy, cz);
+            if (b != cur_brick) {
+                cur_brick = b;
+                cur_macro_empty = !A.occ2[macro_index(A, cx, cy, cz)];
+                cur_occ = A.occ[b];
+                cur_label = MASK ? A.blabel[b] : 0;
+            }
+            if (!(cur_macro_empty || !cur_occ)) {
+                int label = 0;
+                if (MASK) label = cur_label == 0xFF ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label;
+                const float4 aw = s.aw[label];
+                const int4 lb = s.lab[label];
+                TriSample tri;
+                float t = sample_keep(A.v, p, tri);
+                t = (t - aw.z) / aw.y + 0.5f;
+                t = t > 1.0f ? 1.0f : t;
+                const float4 col = lut_sample<LDS_LUT>(A, s.lut, lb.x, lb.y, t);
+                a = vr_step_alpha(col.w, step, L1, L4);
+                // the label's accumulator only grows: one at its cap now
+                // stays there, and its samples never composite
+                const float held = MASK ? gobj[lb.z] : aobj0;
+                if (a > 0.0005f && held < aw.x) {
+                    const float k = vr_shade(A, gradient_kept(A, tri));
+                    kr = col.x * k;
+                    kg = col.y * k;
+                    kb = col.z * k;
+                    ka = a * k;
+                    cap = aw.x;
+                    slot_i = MASK ? lb.z : 0;
+                    cand = true;
+                }
+            }
+        }
+
+        // aPre at this sample: the alpha of the last sample in the VOI before
+        // it in this run, else the ray's
+        const uint64_t vb = (__ballot(inv) >> base) & gbits;
+        const uint64_t below = vb & ((1ull << j) - 1ull);
+        const int src = below ? base + 63 - __clzll((long long)below) : lane;
+        const float a_prev = __shfl(a, src);
+        const float apre_j = below ? a_prev : aPre;
+        // this sample's event in the reference's loop: 0 none, 1 the loop
+        // ends before it, 2 getNextStep backs up at it, 3 it is accepted and
+        // then the step changes
+        int ev = 0;
+        if (!(it + j < A.max_iter && accj <= kAccEnd && accj <= a_out)) {
+            ev = 1;
+        } else if (inv && accj > 0.f) {
+            const bool hi = fmaxf(a, apre_j) > 0.001f;
+            if (step == L8) ev = hi ? 0 : 3;
+            else if (step == L4) ev = hi ? 2 : 3;
+            else ev = hi ? 2 : 0;
+        }
+        const uint64_t eb = (__ballot(ev != 0) >> base) & gbits;
+        const int e = eb ? __ffsll((long long)eb) - 1 : G;
+        const int ev_e = __shfl(ev, base + (e < G ? e : G - 1));
+        const int kind = e < G ? ev_e : 0;
+
+        // composite, in order, the samples that may do so up to the event
+        const int lim = kind == 3 ? e + 1 : e;
+        uint64_t cb = (__ballot(cand) >> base) & gbits;
+        if (lim < 64) cb &= (1ull << lim) - 1ull;
+        bool done = kind == 1;
+        while (cb) {
+            const int q = __ffsll((long long)cb) - 1;
+            cb &= cb - 1ull;
+            const float c_q = __shfl(cap, base + q);
+            const float r_q = __shfl(kr, base + q);
+            const float g_q = __shfl(kg, base + q);
+            const float b_q = __shfl(kb, base + q);
+            const float w_q = __shfl(ka, base + q);
+            const float a_q = __shfl(a, base + q);
+            const int s_q = __shfl(slot_i, base + q);
+            float& slot = MASK ? gobj[s_q] : aobj0;
+            if (slot < c_q) vr_composite_k(sum, aAcc, slot, a_q, r_q, g_q, b_q, w_q);
+            if (aAcc > A.exit_alpha) {
+                done = true;
+                break;
+            }
+        }
+
+        if (!done) {
+            // the state after the run, as the reference's loop leaves it
+            const int last = kind == 0 ? G - 1 : e;
+            const float acc_l = __shfl(accj, base + last);
+            const float a_l = __shfl(a, base + last);
+            const float apre_l = __shfl(apre_j, base + last);
+            const uint64_t vbl = vb & (last >= 63 ? ~0ull : ((1ull << (last + 1)) - 1ull));
+            const int srcl = vbl ? base + 63 - __clzll((long long)vbl) : lane;
+            const float a_vl = __shfl(a, srcl);
+            if (kind == 0) {
+                // every sample accepted (out-of-VOI ones only advance acc)
+                acc = acc_l + step;
+                if (vbl) aPre = a_vl;
+                it += G;
+            } else if (kind == 2) {
+                // backing up at sample e, which is dropped
+                if (step == L1) {
+                    acc = acc_l - (L1 - L4);
+                    step = L4;
+                } else {
+                    acc = acc_l - (L4 - L8);
+                    step = L8;
+                }
+                aPre = apre_l;
+                it += e + 1;
+            } else {
+                // sample e accepted, then the step changes
+                step = (step == L8) ? L4 : L1;
+                aPre = a_l;
+                acc = acc_l + step;
+                it += e + 1;
+            }
+        }
+        if (done) {
+            if (j == 0) {
+                float4 o = sum;
+                if (o.x == 0.f && o.y == 0.f && o.z == 0.f && o.w == 0.f) o = bg4(A);
+                store_rgb(A, pix % A.width, pix / A.width, o);
+            }
+            have = false;
+        }
+    }
 }
 
 // d_renderMIP (core/kernel.cu:266-364) for pixel (x, y).
@@ -495,7 +802,7 @@ __device__ inline void mip_pixel(const RenderArgs& A, const Smem& s, int x, int 
         uint32_t cur_brick = 0xFFFFFFFFu;
         float cur_bm = INFINITY, cur_bb = INFINITY;
         int cur_label = 0xFF;
-        for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
+        for (int it = 0; it < A.max_iter && acc <= kAccEnd && acc <= a_out; ++it) {
             float p[3];
             ray_pos(A, u, v, acc, p);
             int ix, iy, iz;
@@ -549,7 +856,7 @@ __device__ inline void surface_pixel(const RenderArgs& A, int x, int y) {
         float acc = start_acc(A, a_in);
         uint32_t cur_brick = 0xFFFFFFFFu;
         int cur_macro_empty = 0, cur_occ = 1;
-        for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
+        for (int it = 0; it < A.max_iter && acc <= kAccEnd && acc <= a_out; ++it) {
             float p[3];
             ray_pos(A, u, v, acc, p);
             int ix, iy, iz;

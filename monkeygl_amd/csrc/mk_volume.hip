@@ -118,6 +118,8 @@ extern "C" void mk_context_destroy(mk_context* ctx) {
     mk_free_volume(ctx);
     mk_free(ctx->d_lut);
     mk_free(ctx->d_lutnz);
+    mk_free(ctx->d_cont);
+    if (ctx->cont_done) (void)hipEventDestroy(ctx->cont_done);
     mk_free(ctx->d_steps);
     mk_free(ctx->d_mc_table);
     if (ctx->d_mc_scratch) (void)hipFree(ctx->d_mc_scratch);

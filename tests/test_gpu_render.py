@@ -169,16 +169,10 @@ def test_footprint_matches_oracle(gpu_ctx, small, rt):
     assert frac >= MIN_FRAC, (frac, dmax)
 
 
-def test_skipping_is_exact_with_capped_label(gpu_ctx, small):
-    """Empty-cell skipping and the skip tables change nothing: the frame
-    equals, bit for bit, the measurement variant's, which samples every
-    lattice point. Label 0's LUT has alpha 0.05 everywhere and a cap of 0.3,
-    so rays reach the cap early and then march at L1/8 through label-0
-    tissue, between ellipsoids of labels with their own caps."""
-    import torch
-
-    vol, _ = small
-    mask = W.ellipsoid_mask((128, 128, 128), 3, seed=5)
+def _capped_scene(gpu_ctx, vol, n_labels=3, seed=5):
+    """Label 0 with alpha 0.05 everywhere and a cap of 0.3, plus ellipsoids of
+    labels with ordinary LUTs and their own caps."""
+    mask = W.ellipsoid_mask((128, 128, 128), n_labels, seed=seed)
     gpu_ctx.set_volume(vol)
     gpu_ctx.set_mask(mask)
     lut0 = np.zeros((100, 4), np.float32)
@@ -188,11 +182,63 @@ def test_skipping_is_exact_with_capped_label(gpu_ctx, small):
     lut0[:, 3] = 0.05
     luts = {0: lut0}
     labels = {0: P.LabelState(0.3, 2000.0, 0.0)}
-    for l in range(1, 4):
+    for l in range(1, n_labels + 1):
         luts[l] = P.transfer_function(W.tf_points(20 + l))
-        labels[l] = P.LabelState([1.0, 0.5, 0.8][l - 1], 1600.0, -50.0 * l)
+        labels[l] = P.LabelState([1.0, 0.5, 0.8, 0.9][(l - 1) % 4], 1600.0, -50.0 * l)
     for l, a in luts.items():
         gpu_ctx.set_lut(l, a)
+    return mask, luts, labels
+
+
+@pytest.mark.parametrize("masked", [True, False])
+def test_two_pass_vr_is_exact_for_every_budget(gpu_ctx, small, masked):
+    """Rays still marching after the first pass's trip budget finish in the
+    second pass, eight consecutive samples at a time; the frame must be the
+    same, bit for bit, for every budget (0 = one pass). Budgets 1 and 3 put
+    nearly all of the work in the second pass, with backtracking, step
+    changes, VOI exits, label caps and early exit inside its batches."""
+    vol, lut = small
+    if masked:
+        _, _, labels = _capped_scene(gpu_ctx, vol)
+    else:
+        gpu_ctx.set_volume(vol)
+        gpu_ctx.set_lut(0, lut)
+        labels = None
+    try:
+        for ang in [(0, 0), (40, 10), (-75, 110)]:
+            kw = dict(labels=labels) if labels else {}
+            p = _params(MK_RENDER_VR, ang, **kw)
+            if ang == (40, 10):
+                p.voi.left, p.voi.right = 20, 100   # rays leave and re-enter the VOI
+            gpu_ctx.set_vr_trip_budget(0)
+            one = gpu_ctx.render(p)
+            assert (one != 0).any()
+            for budget in (1, 3, 16, 128):
+                gpu_ctx.set_vr_trip_budget(budget)
+                two = gpu_ctx.render(p)
+                bad = int((one != two).any(axis=-1).sum())
+                assert bad == 0, (masked, ang, budget, bad)
+    finally:
+        gpu_ctx.set_vr_trip_budget(160)
+        if masked:
+            gpu_ctx.clear_mask()
+
+
+def test_vr_trip_budget_rejects_negative(gpu_ctx):
+    with pytest.raises(MkError):
+        gpu_ctx.set_vr_trip_budget(-1)
+
+
+def test_skipping_is_exact_with_capped_label(gpu_ctx, small):
+    """Empty-cell skipping and the skip tables change nothing: the frame
+    equals, bit for bit, the measurement variant's, which samples every
+    lattice point. Label 0's LUT has alpha 0.05 everywhere and a cap of 0.3,
+    so rays reach the cap early and then march at L1/8 through label-0
+    tissue, between ellipsoids of labels with their own caps."""
+    import torch
+
+    vol, _ = small
+    mask, luts, labels = _capped_scene(gpu_ctx, vol)
     s = torch.cuda.current_stream().cuda_stream
     for ang in [(0, 0), (40, 10), (-75, 110)]:
         p = _params(MK_RENDER_VR, ang, labels=labels)
