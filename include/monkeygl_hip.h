@@ -201,11 +201,13 @@ mk_status mk_render(mk_context* ctx, const mk_render_params* p, uint8_t* host_rg
 mk_status mk_render_async(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb, void* stream);
 
 /* VR in two passes (no reference counterpart): the first pass marches every
- * ray for up to `trips` loop trips; the rays still marching then continue in
- * a second pass that evaluates eight consecutive samples of a ray at once and
- * applies the reference's loop (core/kernel.cu:133-264) to them in order. The
- * frame is the same for every value; 0 renders in one pass. Default 160.
- * Frames whose queue would exceed 2 GiB render in one pass. */
+ * ray until it ends, until `trips` loop trips, or until 95% of the first
+ * pass's waves have finished (its last waves then hand off); the rays still
+ * marching continue in a second pass that evaluates eight consecutive samples
+ * of a ray at once and applies the reference's loop (core/kernel.cu:133-264)
+ * to them in order. The frame is the same for every value; 0 renders in one
+ * pass. Default 1024. Frames whose queue would exceed 2 GiB render in one
+ * pass. */
 mk_status mk_set_vr_trip_budget(mk_context* ctx, int trips);
 
 /* Measurement variant of mk_render_async (no reference counterpart): renders

@@ -35,8 +35,9 @@ struct MkLutTable {
     int len[MK_NUM_LABELS];
 };
 
-// Loop trips of the first VR pass before a ray moves to the second.
-constexpr int kVrTripBudget = 160;
+// Loop trips of the first VR pass after which a ray moves to the second (the
+// rays of its last waves move earlier).
+constexpr int kVrTripBudget = 1024;
 
 // Skip tables of one render type. occ/occ2: brick / macro cell (4x4x4 bricks)
 // may contribute (VR, Surface); mipb/mipb2: upper bound of the windowed value

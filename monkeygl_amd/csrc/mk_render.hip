@@ -36,6 +36,7 @@ constexpr size_t kContMaxBytes = size_t(2) << 30;    // larger frames render in 
 constexpr size_t kTailBlocks = 2048;                 // second-pass workgroups (8 per CU)
 constexpr float kLongSteps = 256.0f;                 // second-pass rays claimed first
 constexpr int kTailGroup = 8;                        // second-pass lanes per ray
+constexpr size_t kTailPercent = 95;                  // first-pass waves finished before the rest hand off
 
 using namespace mkr;
 
@@ -45,6 +46,12 @@ __global__ void __launch_bounds__(TW * TW) k_vr(RenderArgs A, AwTable T) {
     int x, y;
     if (!tile_pixel<TW>(A, x, y)) return;
     vr_pixel<TW * TW, MASK, LDS_LUT, TOUCH>(A, s, x, y, threadIdx.x);
+    // count the finished waves (rays queued or done): the first pass's last
+    // waves hand off once most have finished
+    if (!TOUCH && A.cont.cap > 0) {
+        const uint64_t m = __ballot(1);
+        if (__lane_id() == __ffsll((long long)m) - 1) atomicAdd(&A.cont.counters[3], 1u);
+    }
 }
 
 // Second VR pass over the rays the first pass queued: 256 / kTailGroup rays in
@@ -289,7 +296,8 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
         C.cap = (int)npix;
         C.budget = ctx->vr_budget;
         C.long_steps = kLongSteps;
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, C.counters, 3);
+        C.tail_waves = (uint32_t)((size_t)tiles * (size_t)(tw * tw / 64) * kTailPercent / 100);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, C.counters, 4);
     }
     dim3 grid(tiles), block(tw * tw);
 #define MK_LAUNCH(K) hipLaunchKernelGGL(K, grid, block, lds, s, A, T)

@@ -16,7 +16,8 @@ namespace mkr {
 // count. Structure of arrays over the queue; cap = 0 disables the queue.
 struct VrCont {
     uint32_t* counters;   // [0] rays queued from the front, [1] rays claimed by
-                          // the second pass, [2] rays queued from the back
+                          // the second pass, [2] rays queued from the back,
+                          // [3] first-pass waves finished
     int* pix;             // y * width + x
     float* acc;
     float* step;
@@ -27,6 +28,7 @@ struct VrCont {
     float* aobj;          // [slot * cap + ray]
     int cap;
     int budget;           // loop trips of the first pass
+    uint32_t tail_waves;  // first-pass waves finished after which running ones hand off
     float long_steps;     // rays with at least this many L1 steps left to the
                           // end of their interval queue from the front
 };
@@ -490,9 +492,19 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
         int cur_macro_empty = 0, cur_occ = 1, cur_label = 0xFF;
         int trips = 0, it_queued = -1;
         for (int it = 0; it < A.max_iter && acc <= kAccEnd && acc <= a_out; ++it) {
-            // a ray still marching after the budget is queued for the second
-            // pass (all lanes of a wave reach the budget on the same trip)
-            if (!TOUCH && A.cont.cap > 0 && trips++ == A.cont.budget) { it_queued = it; break; }
+            // a ray still marching after the budget, or when most of the
+            // first pass's waves have finished (this wave is in its tail), is
+            // queued for the second pass; all lanes of a wave take either
+            // exit on the same trip
+            if (!TOUCH && A.cont.cap > 0) {
+                if (trips == A.cont.budget) { it_queued = it; break; }
+                if ((trips & 15) == 15 &&
+                    __hip_atomic_load(&A.cont.counters[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= A.cont.tail_waves) {
+                    it_queued = it;
+                    break;
+                }
+                ++trips;
+            }
             float p[3];
             ray_pos(A, u, v, acc, p);
             int ix, iy, iz;

@@ -219,7 +219,7 @@ def test_two_pass_vr_is_exact_for_every_budget(gpu_ctx, small, masked):
                 bad = int((one != two).any(axis=-1).sum())
                 assert bad == 0, (masked, ang, budget, bad)
     finally:
-        gpu_ctx.set_vr_trip_budget(160)
+        gpu_ctx.set_vr_trip_budget(1024)
         if masked:
             gpu_ctx.clear_mask()
 
