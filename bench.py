@@ -321,7 +321,106 @@ def overlay_section(ctx, size: int, with_cpu: bool):
     return out
 
 
-def serve_section(args, body_ctx, spec, vol, mask, luts, labels):
+def configs_section(scene2, body_ctx, spec, vr_trip_budget=None) -> dict:
+    """One timing per remaining workload of the configurations, each on its own
+    plain path with inputs resident: the 128^3 blob set (256^2 VR with the
+    4096-entry LUT, half-voxel step and exit at 0.99; 256^2 MIP), the
+    cardiac-sized set (1024^2 VR and MIP over 12 views about z; axial, sagittal
+    and coronal MPR at 512^2, nearest and trilinear), and on the body set the
+    stretched and straightened CPR at 512 px width along the 200-point
+    centreline and marching cubes at iso 200 on a 256^3 sub-block. Mean time
+    per item from HIP events over repeated back-to-back launches."""
+    import torch
+
+    from monkeygl_amd import cpr as G
+    from monkeygl_amd import params as P
+    from monkeygl_amd import workloads as W
+    from monkeygl_amd._capi import (MK_MPR_AVERAGE, MK_RENDER_MIP, MK_RENDER_VR, MK_SAMPLE_LINEAR,
+                                    MK_SAMPLE_NEAREST, Context)
+
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+
+    def timed(launch, n_items, reps):
+        for i in range(n_items):
+            launch(i)
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            for i in range(n_items):
+                launch(i)
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / (reps * n_items)
+
+    def views(sv, labels, rt, size, n=12, **kw):
+        out = []
+        for k in range(n):
+            info = P.RenderInfo()
+            info.set_vr_size(size, size)
+            info.rotate(360.0 * k / n, 0.0)
+            out.append(P.render_params(sv.spec.dims, sv.spec.spacing, info, labels, rt, width=size, height=size, **kw))
+        return out
+
+    res = {}
+    small, cardiac = scene2[0], scene2[1]
+    frame = torch.empty(1024 * 1024 * 3, dtype=torch.uint8, device="cuda")
+
+    cs = Context(0)
+    cc = Context(0)
+    if vr_trip_budget is not None:
+        cs.set_vr_trip_budget(vr_trip_budget)
+        cc.set_vr_trip_budget(vr_trip_budget)
+    cs.set_volume(small.vol)
+    pts = W.tf_points(0)
+    cs.set_lut(0, P.transfer_function({int(round(k * 4095 / 99)): v for k, v in pts.items()}, n=4096))
+    lab = {0: P.LabelState(1.0, 2000.0, 0.0)}
+    for rt, name, kw in ((MK_RENDER_VR, "blobs128_vr_256", dict(step_scale=0.5, exit_alpha=0.99)),
+                         (MK_RENDER_MIP, "blobs128_mip_256", {})):
+        vs = views(small, lab, rt, 256, **kw)
+        ms = timed(lambda i: cs.render_async(vs[i], frame, stream=sh), len(vs), 8)
+        res[name] = {"ms": round(ms, 4), "frames_per_s": round(1e3 / ms, 1)}
+
+    cc.set_volume(cardiac.vol)
+    cc.set_lut(0, P.transfer_function(W.tf_points(1)))
+    lab = {0: P.LabelState(1.0, 900.0, 250.0)}
+    for rt, name in ((MK_RENDER_VR, "cardiac_vr_1024"), (MK_RENDER_MIP, "cardiac_mip_1024")):
+        vs = views(cardiac, lab, rt, 1024)
+        ms = timed(lambda i: cc.render_async(vs[i], frame, stream=sh), len(vs), 2)
+        res[name] = {"ms": round(ms, 4), "frames_per_s": round(1e3 / ms, 1)}
+    img = torch.empty(512 * 512, dtype=torch.int16, device="cuda")
+    for sampling, name in ((MK_SAMPLE_NEAREST, "cardiac_mpr_512_nearest"), (MK_SAMPLE_LINEAR, "cardiac_mpr_512_linear")):
+        qs = [P.plane_params(cardiac.spec.dims, cardiac.spec.spacing, plane, mpr_type=MK_MPR_AVERAGE,
+                             sampling=sampling, size=(512, 512))
+              for plane in (P.PLANE_AXIAL, P.PLANE_SAGITTAL, P.PLANE_CORONAL)]
+        ms = timed(lambda i: cc.render_plane_async(qs[i], img, stream=sh), len(qs), 20)
+        res[name] = {"ms": round(ms, 4), "images_per_s": round(1e3 / ms, 1)}
+
+    info = G.CPRInfo()
+    info.set_spacing(spec.spacing)
+    info.set_cpr_line_voxel(W.centerline(spec.dims, 200, seed=4))
+    for which, name in ((G.PLANE_STRETCHED_CPR, "body_cpr_stretched_512"),
+                        (G.PLANE_STRAIGHTENED_CPR, "body_cpr_straightened_512")):
+        pts_, dirs_, _, h = info.get_cpr_info(which)
+        dp = torch.from_numpy(np.ascontiguousarray(pts_, dtype=np.float64).reshape(-1)).cuda()
+        dd = torch.from_numpy(np.ascontiguousarray(dirs_, dtype=np.float64).reshape(-1)).cuda()
+        out = torch.empty(h * 512, dtype=torch.int16, device="cuda")
+        ms = timed(lambda i: body_ctx.render_cpr_async(512, h, dp, dd, False, MK_SAMPLE_LINEAR, out, stream=sh), 1, 20)
+        res[name] = {"ms": round(ms, 4), "rows": int(h), "images_per_s": round(1e3 / ms, 1)}
+
+    box = (128, 128, 651, 384, 384, 907)
+    n_tri = len(body_ctx.marching_cubes(box, 200.0))
+    tri = torch.empty(max(n_tri, 1) * 9, dtype=torch.float32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ms = timed(lambda i: body_ctx.marching_cubes_async(box, 200.0, tri, max(n_tri, 1), cnt, stream=sh), 1, 10)
+    res["body_mc_256_iso200"] = {"ms": round(ms, 4), "triangles": n_tri, "box": list(box)}
+    cs.close()
+    cc.close()
+    return res
+
+
+def serve_section(args, body_ctx, spec, vol, mask, luts, labels, scene2=None):
     """Served vs sequential plain rendering of one seeded request stream."""
     import dataclasses
 
@@ -330,11 +429,13 @@ def serve_section(args, body_ctx, spec, vol, mask, luts, labels):
     from monkeygl_amd.serve import RenderServer, latency_stats, run_sequential
 
     t = time.time()
-    scene = W.serving_scene([(W.SMALL, 0), (W.CARDIAC, 0)])
+    scene = list(scene2) if scene2 is not None else W.serving_scene([(W.SMALL, 0), (W.CARDIAC, 0)])
     scene.append(W.ServedVolume(spec, vol, mask, luts, labels, W.centerline(spec.dims, 200, seed=4)))
     ctxs = []
     for sv in scene[:2]:
         c = Context(0)
+        if args.vr_trip_budget is not None:
+            c.set_vr_trip_budget(args.vr_trip_budget)
         c.set_volume(sv.vol)
         for l, lut in sv.luts.items():
             c.set_lut(l, lut)
@@ -436,6 +537,11 @@ def main() -> int:
     ap.add_argument("--serve-qps", type=float, default=500.0)
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the timings of the other configurations' workloads")
+    ap.add_argument("--vr-trip-budget", type=int, default=None,
+                    help="first-pass trip bound of plain VR frames for every context (0: one pass; default: the "
+                         "library's); the frames are the same for every value")
     ap.add_argument("--traffic", type=str, default=str(ROOT / "profiles" / "traffic_vr_body1024.json"),
                     help="measured HBM bytes per frame from rocprof counters (see the file), if present")
     args = ap.parse_args()
@@ -455,6 +561,8 @@ def main() -> int:
 
     spec, vol, mask, luts, labels = build_scene()
     ctx = Context(0)
+    if args.vr_trip_budget is not None:
+        ctx.set_vr_trip_budget(args.vr_trip_budget)
     t = time.time()
     ctx.set_volume(vol)
     ctx.set_mask(mask)
@@ -626,7 +734,16 @@ def main() -> int:
     series = series_section(ctx, spec)
     mask_ingest = mask_ingest_section(spec, vol, mask)
     overlay = overlay_section(ctx, W_, not args.no_cpu)
-    served = None if args.no_serve else serve_section(args, ctx, spec, vol, mask, luts, labels)
+    scene2 = None
+    if not (args.no_serve and args.no_configs):
+        from monkeygl_amd import workloads as W
+        t = time.time()
+        scene2 = W.serving_scene([(W.SMALL, 0), (W.CARDIAC, 0)])
+        log(f"blob and cardiac sets generated in {time.time() - t:.1f}s")
+    configs = None if args.no_configs else configs_section(scene2, ctx, spec, args.vr_trip_budget)
+    if configs:
+        log(f"configs: {configs}")
+    served = None if args.no_serve else serve_section(args, ctx, spec, vol, mask, luts, labels, scene2)
 
     line = {
         "metric": "VR fps 512\u00b2\u00d71559 i16\u21921024\u00b2 plain vs served p50/p99 @\u22641/255 err; HBM GB/s",
@@ -658,6 +775,7 @@ def main() -> int:
         "series": series,
         "mask_ingest": mask_ingest,
         "orientation_figure": overlay,
+        "configs": configs,
         "served": served,
     }
     print(json.dumps(line), flush=True)
