@@ -277,7 +277,7 @@ class Context:
         self._check(self.lib.mk_render_async(self.h, C.byref(p), _ptr(dev_out), stream))
 
     def set_vr_trip_budget(self, trips: int) -> None:
-        """Loop trips of the first VR pass (0: one pass); the frame is the same for every value."""
+        """Loop trips of the first VR pass (0: one pass, -1: chosen per frame); the frame is the same for every value."""
         self._check(self.lib.mk_set_vr_trip_budget(self.h, int(trips)))
 
     def render_footprint(self, p: RenderParams, dev_out, dev_touch_vol, dev_touch_mask=None, stream: int = 0) -> None:

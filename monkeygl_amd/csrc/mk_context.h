@@ -36,8 +36,13 @@ struct MkLutTable {
 };
 
 // Loop trips of the first VR pass after which a ray moves to the second (the
-// rays of its last waves move earlier).
-constexpr int kVrTripBudget = 1024;
+// rays of its last waves move earlier), chosen per frame unless set: a frame
+// whose waves all fit on the device at once (few rays, idle SIMDs) hands off
+// early, a larger one late.
+constexpr int kVrTripBudgetAuto = -1;
+constexpr int kVrTripBudgetSmall = 64;
+constexpr int kVrTripBudgetLarge = 1024;
+constexpr int kVrResidentWavesPerCu = 12;   // 4 SIMDs x 3 waves of the first pass
 
 // Skip tables of one render type. occ/occ2: brick / macro cell (4x4x4 bricks)
 // may contribute (VR, Surface); mipb/mipb2: upper bound of the windowed value
@@ -83,7 +88,8 @@ struct mk_context {
     // VR second pass: queue of the rays still marching after vr_budget loop
     // trips of the first pass (0: one pass); renders on other streams are
     // ordered after the last second pass through cont_done
-    int vr_budget = kVrTripBudget;
+    int vr_budget = kVrTripBudgetAuto;
+    int n_cu = 0;   // compute units of the device (queried once)
     void* d_cont = nullptr;
     size_t cont_bytes = 0;
     hipEvent_t cont_done = nullptr;

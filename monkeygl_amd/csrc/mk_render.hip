@@ -265,7 +265,13 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     const size_t npix = (size_t)p->width * (size_t)p->height;
     const int cslots = mask ? std::max(A.nslot, 1) : 1;
     const size_t cont_bytes = kContHeader + npix * (sizeof(float4) + 6 * sizeof(float) + (size_t)cslots * sizeof(float));
-    const bool two_pass = p->render_type == MK_RENDER_VR && !touch && ctx->vr_budget > 0 && cont_bytes <= kContMaxBytes;
+    int budget = ctx->vr_budget;
+    if (budget == kVrTripBudgetAuto) {
+        if (ctx->n_cu <= 0) MK_HIP(ctx, hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        const size_t waves = (size_t)tiles * (size_t)(tw * tw / 64);
+        budget = waves <= (size_t)ctx->n_cu * kVrResidentWavesPerCu ? kVrTripBudgetSmall : kVrTripBudgetLarge;
+    }
+    const bool two_pass = p->render_type == MK_RENDER_VR && !touch && budget > 0 && cont_bytes <= kContMaxBytes;
     if (two_pass) {
         // a larger queue replaces the buffer: the last second pass must be done with it
         if (ctx->cont_used && ctx->cont_bytes < cont_bytes) MK_HIP(ctx, hipEventSynchronize(ctx->cont_done));
@@ -294,7 +300,7 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
         q += npix * sizeof(int);
         C.aobj = (float*)q;
         C.cap = (int)npix;
-        C.budget = ctx->vr_budget;
+        C.budget = budget;
         C.long_steps = kLongSteps;
         C.tail_waves = (uint32_t)((size_t)tiles * (size_t)(tw * tw / 64) * kTailPercent / 100);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, C.counters, 4);
@@ -390,7 +396,7 @@ extern "C" mk_status mk_render_async(mk_context* ctx, const mk_render_params* p,
 
 extern "C" mk_status mk_set_vr_trip_budget(mk_context* ctx, int trips) {
     if (!ctx) return MK_ERR_ARG;
-    if (trips < 0) return mk_fail(ctx, MK_ERR_ARG, "trip budget %d is negative", trips);
+    if (trips < kVrTripBudgetAuto) return mk_fail(ctx, MK_ERR_ARG, "trip budget %d: give trips >= 0, or -1 for automatic", trips);
     ctx->vr_budget = trips;
     return MK_OK;
 }

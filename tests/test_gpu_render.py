@@ -194,7 +194,8 @@ def _capped_scene(gpu_ctx, vol, n_labels=3, seed=5):
 def test_two_pass_vr_is_exact_for_every_budget(gpu_ctx, small, masked):
     """Rays still marching after the first pass's trip budget finish in the
     second pass, eight consecutive samples at a time; the frame must be the
-    same, bit for bit, for every budget (0 = one pass). Budgets 1 and 3 put
+    same, bit for bit, for every budget (0 = one pass, -1 = chosen per
+    frame). Budgets 1 and 3 put
     nearly all of the work in the second pass, with backtracking, step
     changes, VOI exits, label caps and early exit inside its batches."""
     vol, lut = small
@@ -213,20 +214,21 @@ def test_two_pass_vr_is_exact_for_every_budget(gpu_ctx, small, masked):
             gpu_ctx.set_vr_trip_budget(0)
             one = gpu_ctx.render(p)
             assert (one != 0).any()
-            for budget in (1, 3, 16, 128):
+            for budget in (1, 3, 16, 128, -1):
                 gpu_ctx.set_vr_trip_budget(budget)
                 two = gpu_ctx.render(p)
                 bad = int((one != two).any(axis=-1).sum())
                 assert bad == 0, (masked, ang, budget, bad)
     finally:
-        gpu_ctx.set_vr_trip_budget(1024)
+        gpu_ctx.set_vr_trip_budget(-1)
         if masked:
             gpu_ctx.clear_mask()
 
 
-def test_vr_trip_budget_rejects_negative(gpu_ctx):
+def test_vr_trip_budget_rejects_values_below_auto(gpu_ctx):
     with pytest.raises(MkError):
-        gpu_ctx.set_vr_trip_budget(-1)
+        gpu_ctx.set_vr_trip_budget(-2)
+    gpu_ctx.set_vr_trip_budget(-1)
 
 
 def test_skipping_is_exact_with_capped_label(gpu_ctx, small):
