@@ -16,7 +16,14 @@
 //     machine still runs with alpha 0, so results are unchanged;
 //   * LUTs and the per-label window/alpha palette staged in LDS, per-label
 //     opacity accumulators in LDS instead of a 65-entry private array;
-//   * 16x16 pixel tiles of four 8x8 waves, mapped XCD-aware.
+//   * 16x16 pixel tiles of four 8x8 waves, mapped XCD-aware;
+//   * VR in two passes: k_vr marches every ray up to a trip budget (chosen
+//     per frame) or until most of its waves have finished, and queues the
+//     rays still marching with the state the reference's loop carries;
+//     k_vr_tail finishes them eight lanes per ray, each lane evaluating one of
+//     the ray's next samples at the current step and wavefront ballots finding
+//     where the step machine backs up, changes step or exits. The frame is
+//     the same, bit for bit, as a single pass's.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
