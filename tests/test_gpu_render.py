@@ -225,6 +225,52 @@ def test_two_pass_vr_is_exact_for_every_budget(gpu_ctx, small, masked):
             gpu_ctx.clear_mask()
 
 
+def test_two_pass_vr_on_two_streams(gpu_ctx, small):
+    """VR renders of one context share its second-pass queue; frames enqueued
+    alternately on two streams (the queue growing between them) must each
+    equal the frame rendered alone in one pass."""
+    import torch
+
+    vol, lut = small
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_lut(0, lut)
+    ps = [_params(MK_RENDER_VR, ang, size=sz) for ang, sz in
+          [((0, 0), 256), ((40, 10), 384), ((-75, 110), 256), ((20, 35), 512), ((10, 20), 320)]]
+    try:
+        gpu_ctx.set_vr_trip_budget(0)
+        want = [gpu_ctx.render(p) for p in ps]
+        gpu_ctx.set_vr_trip_budget(3)
+        streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+        outs = [torch.empty(p.width * p.height * 3, dtype=torch.uint8, device="cuda") for p in ps]
+        for k, (p, o) in enumerate(zip(ps, outs)):
+            gpu_ctx.render_async(p, o, stream=streams[k % 2].cuda_stream)
+        torch.cuda.synchronize()
+        for k, (p, o, w) in enumerate(zip(ps, outs, want)):
+            got = o.cpu().numpy().reshape(p.height, p.width, 3)
+            assert np.array_equal(got, w), (k, int((got != w).any(axis=-1).sum()))
+    finally:
+        gpu_ctx.set_vr_trip_budget(-1)
+
+
+def test_two_pass_vr_queue_grows_and_is_reused(gpu_ctx, small):
+    """The queue has a slot per pixel and grows with the frame: a small frame,
+    a larger one, then the small one again, all equal to one pass."""
+    vol, lut = small
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_lut(0, lut)
+    try:
+        for sz in (64, 300, 1024, 64, 300):
+            p = _params(MK_RENDER_VR, (25, 15), size=sz)
+            gpu_ctx.set_vr_trip_budget(0)
+            one = gpu_ctx.render(p)
+            for budget in (1, -1):
+                gpu_ctx.set_vr_trip_budget(budget)
+                two = gpu_ctx.render(p)
+                assert np.array_equal(one, two), (sz, budget)
+    finally:
+        gpu_ctx.set_vr_trip_budget(-1)
+
+
 def test_vr_trip_budget_rejects_values_below_auto(gpu_ctx):
     with pytest.raises(MkError):
         gpu_ctx.set_vr_trip_budget(-2)
