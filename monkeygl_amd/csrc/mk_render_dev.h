@@ -209,25 +209,49 @@ struct TriSample {
     float c[2][2][2];   // [z][y][x], raw voxels (-32768 read as -32767)
 };
 
-__device__ inline float sample_keep(const MkVolView& v, const float p[3], TriSample& T) {
-    grad_axis_base(p[0], v.nx, T.ix, T.ax);
-    grad_axis_base(p[1], v.ny, T.iy, T.ay);
-    grad_axis_base(p[2], v.nz, T.iz, T.az);
-    const int x0 = mk_clampi(T.ix, 0, v.nx - 1), x1 = mk_clampi(T.ix + 1, 0, v.nx - 1);
-    const int y0 = mk_clampi(T.iy, 0, v.ny - 1), y1 = mk_clampi(T.iy + 1, 0, v.ny - 1);
-    const int z0 = mk_clampi(T.iz, 0, v.nz - 1), z1 = mk_clampi(T.iz + 1, 0, v.nz - 1);
+// The eight corner voxels of a sample, loaded but not yet used: a caller can
+// issue other loads before the first use, so that they share a round trip.
+struct TriFetch {
+    int ix, iy, iz;
+    float ax, ay, az;
+    int r[8];   // raw voxels, [z][y][x] order
+};
+
+__device__ inline void sample_fetch(const MkVolView& v, const float p[3], TriFetch& F) {
+    grad_axis_base(p[0], v.nx, F.ix, F.ax);
+    grad_axis_base(p[1], v.ny, F.iy, F.ay);
+    grad_axis_base(p[2], v.nz, F.iz, F.az);
+    const int x0 = mk_clampi(F.ix, 0, v.nx - 1), x1 = mk_clampi(F.ix + 1, 0, v.nx - 1);
+    const int y0 = mk_clampi(F.iy, 0, v.ny - 1), y1 = mk_clampi(F.iy + 1, 0, v.ny - 1);
+    const int z0 = mk_clampi(F.iz, 0, v.nz - 1), z1 = mk_clampi(F.iz + 1, 0, v.nz - 1);
     const uint32_t o = mk_voxel_offset(v, x0, y0, z0);
     const uint32_t dx = x1 == x0 ? 0u : ((x0 & 7) == 7 ? 512u - 7u : 1u);
     const uint32_t dy = y1 == y0 ? 0u : ((y0 & 7) == 7 ? (uint32_t)v.nbx * 512u - 56u : 8u);
     const uint32_t dz = z1 == z0 ? 0u : ((z0 & 7) == 7 ? (uint32_t)v.nbx * (uint32_t)v.nby * 512u - 448u : 64u);
-    T.c[0][0][0] = mk_rawo(v, o);
-    T.c[0][0][1] = mk_rawo(v, o + dx);
-    T.c[0][1][0] = mk_rawo(v, o + dy);
-    T.c[0][1][1] = mk_rawo(v, o + dy + dx);
-    T.c[1][0][0] = mk_rawo(v, o + dz);
-    T.c[1][0][1] = mk_rawo(v, o + dz + dx);
-    T.c[1][1][0] = mk_rawo(v, o + dz + dy);
-    T.c[1][1][1] = mk_rawo(v, o + dz + dy + dx);
+    F.r[0] = v.vox[o];
+    F.r[1] = v.vox[o + dx];
+    F.r[2] = v.vox[o + dy];
+    F.r[3] = v.vox[o + dy + dx];
+    F.r[4] = v.vox[o + dz];
+    F.r[5] = v.vox[o + dz + dx];
+    F.r[6] = v.vox[o + dz + dy];
+    F.r[7] = v.vox[o + dz + dy + dx];
+}
+
+// The trilinear value of a fetched sample, keeping what the gradient reuses
+// in T (same arithmetic as mk_sample_hu).
+__device__ inline float sample_finish(const TriFetch& F, TriSample& T) {
+    T.ix = F.ix;
+    T.iy = F.iy;
+    T.iz = F.iz;
+    T.ax = F.ax;
+    T.ay = F.ay;
+    T.az = F.az;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int q = F.r[k];
+        T.c[k >> 2][(k >> 1) & 1][k & 1] = (float)(q < -32767 ? -32767 : q);
+    }
     const float c00 = T.c[0][0][0] + T.ax * (T.c[0][0][1] - T.c[0][0][0]);
     const float c10 = T.c[0][1][0] + T.ax * (T.c[0][1][1] - T.c[0][1][0]);
     const float c01 = T.c[1][0][0] + T.ax * (T.c[1][0][1] - T.c[1][0][0]);
@@ -236,6 +260,12 @@ __device__ inline float sample_keep(const MkVolView& v, const float p[3], TriSam
     const float c1 = c01 + T.ay * (c11 - c01);
     const float c = c0 + T.az * (c1 - c0);
     return c * (32768.0f / 32767.0f);
+}
+
+__device__ inline float sample_keep(const MkVolView& v, const float p[3], TriSample& T) {
+    TriFetch F;
+    sample_fetch(v, p, F);
+    return sample_finish(F, T);
 }
 
 // The gradient from a kept sample: 24 more voxels around its centre block.
@@ -800,7 +830,144 @@ __device__ inline void vr_tail(const RenderArgs& A, const Smem& s, float* gobj) 
     }
 }
 
-// d_renderMIP (core/kernel.cu:266-364) for pixel (x, y).
+// MIP's march over one ray, one lattice point at a time (d_renderMIP,
+// core/kernel.cu:266-364): a cell whose bound cannot exceed the running
+// maximum leaves it unchanged and is skipped.
+template <bool MASK, bool TOUCH>
+__device__ inline void mip_march_single(const RenderArgs& A, const Smem& s, float u, float v, const RayGeom& g,
+                                        float acc, float a_out, float& aAcc) {
+    const float L1 = A.l1;
+    uint32_t cur_brick = 0xFFFFFFFFu;
+    float cur_bm = INFINITY, cur_bb = INFINITY;
+    int cur_label = 0xFF;
+    for (int it = 0; it < A.max_iter && acc <= kAccEnd && acc <= a_out; ++it) {
+        float p[3];
+        ray_pos(A, u, v, acc, p);
+        int ix, iy, iz;
+        if (in_voi(A, p, ix, iy, iz)) {
+            const int cx = mk_clampi(ix, 0, A.v.nx - 1), cy = mk_clampi(iy, 0, A.v.ny - 1), cz = mk_clampi(iz, 0, A.v.nz - 1);
+            int skip_shift = 0;
+            if (!TOUCH) {
+                const uint32_t b = mk_brick_index(A.v, cx, cy, cz);
+                if (b != cur_brick) {
+                    cur_brick = b;
+                    cur_bm = A.mipb2[macro_index(A, cx, cy, cz)];
+                    cur_bb = A.mipb[b];
+                    cur_label = MASK ? A.blabel[b] : 0;
+                }
+                skip_shift = cur_bm <= aAcc ? 5 : (cur_bb <= aAcc ? 3 : 0);
+            }
+            if (!skip_shift) {
+                int label = MASK ? ((TOUCH || cur_label == 0xFF) ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label) : 0;
+                if (TOUCH && MASK) touch_mask(A, cx, cy, cz);
+                const float4 aw = s.aw[label];
+                if (aw.x > 0.f) {
+                    if (TOUCH) touch_footprint(A, p[0], p[1], p[2]);
+                    float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
+                    t = (t - aw.z) / aw.y + 0.5f;
+                    if (aAcc < t) aAcc = t;
+                }
+            }
+            acc += L1;
+            if (skip_shift) skip_lattice(acc, L1, cell_exit(A, g, cx, cy, cz, skip_shift), it, A.max_iter);
+        } else {
+            acc += L1;
+        }
+    }
+}
+
+// The same march kMipGroup lattice points at a time. The result is the
+// maximum of the samples, which does not depend on the order they are taken
+// in: the group's bricks are looked up first, then all its fetches are issued,
+// then they are folded into the maximum. Deciding a skip against the maximum
+// from before the group only skips less, so the result is the same as one
+// point at a time.
+template <bool MASK, bool TOUCH>
+__device__ inline void mip_march_grouped(const RenderArgs& A, const Smem& s, float u, float v, const RayGeom& g,
+                                         float acc, float a_out, float& aAcc) {
+    constexpr int kMipGroup = 4;
+    const float L1 = A.l1;
+    uint32_t cur_brick = 0xFFFFFFFFu;
+    float cur_bm = INFINITY, cur_bb = INFINITY;
+    int cur_label = 0xFF;
+    int it = 0;
+    while (it < A.max_iter && acc <= kAccEnd && acc <= a_out) {
+        // 1: the group's points, their bricks' skip entries and labels
+        float pp[kMipGroup][3];
+        int lab[kMipGroup];
+        bool want[kMipGroup];
+        int n = 0, last_shift = 0, lx = 0, ly = 0, lz = 0;
+        float acc_j = acc;
+#pragma unroll
+        for (int j = 0; j < kMipGroup; ++j) {
+            want[j] = false;
+            lab[j] = 0;
+            // the loop's bound, point by point: a contiguous run
+            if (n == j && it + j < A.max_iter && acc_j <= kAccEnd && acc_j <= a_out) {
+                n = j + 1;
+                last_shift = 0;
+                ray_pos(A, u, v, acc_j, pp[j]);
+                int ix, iy, iz;
+                if (in_voi(A, pp[j], ix, iy, iz)) {
+                    const int cx = mk_clampi(ix, 0, A.v.nx - 1), cy = mk_clampi(iy, 0, A.v.ny - 1),
+                              cz = mk_clampi(iz, 0, A.v.nz - 1);
+                    int skip_shift = 0;
+                    if (!TOUCH) {
+                        const uint32_t b = mk_brick_index(A.v, cx, cy, cz);
+                        if (b != cur_brick) {
+                            cur_brick = b;
+                            cur_bm = A.mipb2[macro_index(A, cx, cy, cz)];
+                            cur_bb = A.mipb[b];
+                            cur_label = MASK ? A.blabel[b] : 0;
+                        }
+                        skip_shift = cur_bm <= aAcc ? 5 : (cur_bb <= aAcc ? 3 : 0);
+                    }
+                    if (!skip_shift) {
+                        lab[j] = MASK ? ((TOUCH || cur_label == 0xFF) ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label) : 0;
+                        if (TOUCH && MASK) touch_mask(A, cx, cy, cz);
+                        want[j] = true;
+                    }
+                    last_shift = skip_shift;
+                    lx = cx;
+                    ly = cy;
+                    lz = cz;
+                }
+                acc_j += L1;
+            }
+        }
+        // 2: every wanted sample's corners, all in flight together
+        TriFetch fr[kMipGroup];
+#pragma unroll
+        for (int j = 0; j < kMipGroup; ++j) {
+            if (want[j] && s.aw[lab[j]].x > 0.f) {
+                if (TOUCH) touch_footprint(A, pp[j][0], pp[j][1], pp[j][2]);
+                sample_fetch(A.v, pp[j], fr[j]);
+            } else {
+                want[j] = false;
+            }
+        }
+        // 3: the maximum
+#pragma unroll
+        for (int j = 0; j < kMipGroup; ++j) {
+            if (want[j]) {
+                const float4 aw = s.aw[lab[j]];
+                TriSample tri;
+                float t = sample_finish(fr[j], tri);
+                t = (t - aw.z) / aw.y + 0.5f;
+                if (aAcc < t) aAcc = t;
+            }
+        }
+        acc = acc_j;
+        it += n;
+        // the last point's cell, skipped as one point at a time would
+        if (last_shift) skip_lattice(acc, L1, cell_exit(A, g, lx, ly, lz, last_shift), it, A.max_iter);
+    }
+}
+
+// d_renderMIP (core/kernel.cu:266-364) for pixel (x, y). With a mask the
+// march takes four lattice points at a time (measured faster); without one a
+// group's fetches did not stay in flight together, and one point at a time is
+// faster.
 template <bool MASK, bool TOUCH>
 __device__ inline void mip_pixel(const RenderArgs& A, const Smem& s, int x, int y) {
     const float u = ((float)x - (float)A.width / 2.0f - A.xT) / (float)A.height;
@@ -809,47 +976,8 @@ __device__ inline void mip_pixel(const RenderArgs& A, const Smem& s, int x, int 
     float a_in, a_out;
     RayGeom g;
     if (ray_interval(A, u, v, g, a_in, a_out)) {
-        const float L1 = A.l1;
-        float acc = start_acc(A, a_in);
-        uint32_t cur_brick = 0xFFFFFFFFu;
-        float cur_bm = INFINITY, cur_bb = INFINITY;
-        int cur_label = 0xFF;
-        for (int it = 0; it < A.max_iter && acc <= kAccEnd && acc <= a_out; ++it) {
-            float p[3];
-            ray_pos(A, u, v, acc, p);
-            int ix, iy, iz;
-            if (in_voi(A, p, ix, iy, iz)) {
-                const int cx = mk_clampi(ix, 0, A.v.nx - 1), cy = mk_clampi(iy, 0, A.v.ny - 1),
-                          cz = mk_clampi(iz, 0, A.v.nz - 1);
-                // a cell whose bound cannot exceed the running maximum leaves it unchanged
-                int skip_shift = 0;
-                if (!TOUCH) {
-                    const uint32_t b = mk_brick_index(A.v, cx, cy, cz);
-                    if (b != cur_brick) {
-                        cur_brick = b;
-                        cur_bm = A.mipb2[macro_index(A, cx, cy, cz)];
-                        cur_bb = A.mipb[b];
-                        cur_label = MASK ? A.blabel[b] : 0;
-                    }
-                    skip_shift = cur_bm <= aAcc ? 5 : (cur_bb <= aAcc ? 3 : 0);
-                }
-                if (!skip_shift) {
-                    int label = MASK ? ((TOUCH || cur_label == 0xFF) ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label) : 0;
-                    if (TOUCH && MASK) touch_mask(A, cx, cy, cz);
-                    const float4 aw = s.aw[label];
-                    if (aw.x > 0.f) {
-                        if (TOUCH) touch_footprint(A, p[0], p[1], p[2]);
-                        float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
-                        t = (t - aw.z) / aw.y + 0.5f;
-                        if (aAcc < t) aAcc = t;
-                    }
-                }
-                acc += L1;
-                if (skip_shift) skip_lattice(acc, L1, cell_exit(A, g, cx, cy, cz, skip_shift), it, A.max_iter);
-            } else {
-                acc += L1;
-            }
-        }
+        if constexpr (MASK) mip_march_grouped<MASK, TOUCH>(A, s, u, v, g, start_acc(A, a_in), a_out, aAcc);
+        else mip_march_single<MASK, TOUCH>(A, s, u, v, g, start_acc(A, a_in), a_out, aAcc);
     }
     float4 c = aAcc <= 0.f ? bg4(A) : make_float4(aAcc, aAcc, aAcc, aAcc);
     store_rgb(A, x, y, c);
