@@ -24,6 +24,7 @@
 #include <cstring>
 
 #include "mk_context.h"
+#include "mk_render_dev.h"
 
 namespace {
 
@@ -68,16 +69,14 @@ __device__ inline bool fit_voi(const FitArgs& A, const float p[3]) {
 }
 
 // Shading factor 0.35 + (d > 0 ? 0.8 d + 0.16 d^8 : 0) of the normalized
-// one-voxel central-difference gradient against the headlight.
-__device__ inline float fit_shade(const FitArgs& A, const float p[3], float3 L) {
+// one-voxel central-difference gradient against the headlight. The six
+// samples at p +- 1/N per axis share p's weights, so the differences are the
+// interpolations (with p's weights) of voxel differences two apart: the
+// sample's own eight corners plus 24 more voxels, loaded together; the common
+// 32768/32767 scale cancels in the normalization.
+__device__ inline float fit_shade(const FitArgs& A, const mkr::TriSample& tri, float3 L) {
     float N[3];
-#pragma unroll 1
-    for (int i = 0; i < 3; ++i) {
-        float q0[3] = {p[0], p[1], p[2]}, q1[3] = {p[0], p[1], p[2]};
-        q0[i] += A.sv[i];
-        q1[i] -= A.sv[i];
-        N[i] = mk_sample_hu(A.v, q0[0], q0[1], q0[2]) - mk_sample_hu(A.v, q1[0], q1[1], q1[2]);
-    }
+    mkr::grad_diff_kept(A.v, tri, N);
     if (A.invert_z) N[2] = -N[2];
     float r = rsqrtf(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
     float d = (N[0] * L.x + N[1] * L.y + N[2] * L.z) * r;
@@ -97,9 +96,10 @@ struct Sample {
     bool grad_ok; // the clamp is inactive: 0 <= s*alpha < 1 (one-sided at 0)
 };
 
-__device__ inline Sample fit_sample(const FitArgs& A, const float4* s_lut, const float p[3]) {
+// tri keeps the trilinear sample's corners for fit_shade.
+__device__ inline Sample fit_sample(const FitArgs& A, const float4* s_lut, const float p[3], mkr::TriSample& tri) {
     Sample S;
-    float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
+    float t = mkr::sample_keep(A.v, p, tri);
     t = (t - A.wl) / A.ww + 0.5f;
     t = t > 1.0f ? 1.0f : (t >= -1.0f ? t : -1.0f);
     mk_split(t, A.n, S.i0, S.i1, S.f);
@@ -176,9 +176,10 @@ __global__ void __launch_bounds__(256) k_fit_forward(FitArgs A, float* __restric
             fit_pos(A, u, v, acc, p);
             acc += A.l1;
             if (!fit_voi(A, p)) continue;
-            const Sample S = fit_sample(A, s_lut, p);
+            mkr::TriSample tri;
+            const Sample S = fit_sample(A, s_lut, p, tri);
             if (!(S.a > 0.0f)) continue;
-            const float k = fit_shade(A, p, L);
+            const float k = fit_shade(A, tri, L);
             const double w = T * (double)S.a * (double)k;
             C[0] += w * S.c.x;
             C[1] += w * S.c.y;
@@ -217,9 +218,10 @@ __global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __
                 fit_pos(A, u, v, acc, p);
                 acc += A.l1;
                 if (!fit_voi(A, p)) continue;
-                const Sample S = fit_sample(A, s_lut, p);
+                mkr::TriSample tri;
+                const Sample S = fit_sample(A, s_lut, p, tri);
                 if (!(S.a > 0.0f)) continue;
-                const double w = T * (double)S.a * (double)fit_shade(A, p, L);
+                const double w = T * (double)S.a * (double)fit_shade(A, tri, L);
                 C[0] += w * S.c.x;
                 C[1] += w * S.c.y;
                 C[2] += w * S.c.z;
@@ -235,10 +237,11 @@ __global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __
                 fit_pos(A, u, v, acc, p);
                 acc += A.l1;
                 if (!fit_voi(A, p)) continue;
-                const Sample S = fit_sample(A, s_lut, p);
+                mkr::TriSample tri;
+                const Sample S = fit_sample(A, s_lut, p, tri);
                 // samples with zero alpha still move the image when their LUT
                 // entries grow: da > 0 from 0 is a smooth onset
-                const double k = (double)fit_shade(A, p, L);
+                const double k = (double)fit_shade(A, tri, L);
                 const double a = S.a;
                 const double w = T * a * k;
                 const double c[3] = {S.c.x, S.c.y, S.c.z};

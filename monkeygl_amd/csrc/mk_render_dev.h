@@ -268,9 +268,9 @@ __device__ inline float sample_keep(const MkVolView& v, const float p[3], TriSam
     return sample_finish(F, T);
 }
 
-// The gradient from a kept sample: 24 more voxels around its centre block.
-__device__ inline float3 gradient_kept(const RenderArgs& A, const TriSample& T) {
-    const MkVolView& v = A.v;
+// The central differences at a kept sample, unnormalized and without the
+// common 32768/32767 scale: 24 more voxels around its centre block.
+__device__ inline void grad_diff_kept(const MkVolView& v, const TriSample& T, float N[3]) {
     const float ax = T.ax, ay = T.ay, az = T.az;
     const uint32_t sy = (uint32_t)v.nbx * 512u, sz = (uint32_t)v.nbx * (uint32_t)v.nby * 512u;
     uint32_t OX[4], OY[4], OZ[4];
@@ -313,7 +313,6 @@ __device__ inline float3 gradient_kept(const RenderArgs& A, const TriSample& T) 
             const float loz = c[1][k][j] - ez[k][j][0], hiz = ez[k][j][1] - c[0][k][j];
             dz[k][j] = loz + az * (hiz - loz);
         }
-    float N[3];
     {   // dx[k][j]: j over y, k over z
         const float a0 = dx[0][0] + ay * (dx[0][1] - dx[0][0]);
         const float a1 = dx[1][0] + ay * (dx[1][1] - dx[1][0]);
@@ -329,6 +328,12 @@ __device__ inline float3 gradient_kept(const RenderArgs& A, const TriSample& T) 
         const float a1 = dz[1][0] + ax * (dz[1][1] - dz[1][0]);
         N[2] = a0 + ay * (a1 - a0);
     }
+}
+
+// The gradient from a kept sample (normalized, z flipped with the volume).
+__device__ inline float3 gradient_kept(const RenderArgs& A, const TriSample& T) {
+    float N[3];
+    grad_diff_kept(A.v, T, N);
     if (A.invert_z) N[2] = -N[2];
     float r = rsqrtf(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
     return make_float3(N[0] * r, N[1] * r, N[2] * r);
