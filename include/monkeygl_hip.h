@@ -259,18 +259,21 @@ mk_status mk_marching_cubes_async(mk_context* ctx, const int* box, float iso, mk
  * output. dev_lut holds n RGBA float entries (2 <= n <= 4096) in device
  * memory: the LUT is a parameter, not the context's label LUT. */
 
-/* Forward: width*height*3 floats (RGB) into dev_rgb. */
+/* Forward: width*height*3 floats (RGB) into dev_rgb and, unless dev_rgb64 is
+ * NULL, the same colours as width*height*3 doubles into dev_rgb64 (what the
+ * adjoint takes to skip a march). */
 mk_status mk_tf_forward(mk_context* ctx, const mk_render_params* p, const float* dev_lut, int n,
-                        float opacity_scale, float* dev_rgb, void* stream);
+                        float opacity_scale, float* dev_rgb, double* dev_rgb64, void* stream);
 
 /* Adjoint: given dL/dRGB (width*height*3 floats), ADDS dL/dLUT (n*4 doubles)
  * and dL/d(opacity_scale) (1 double) into the caller's zeroed or running
- * accumulators. Each ray is re-marched (first pass: its colour; second pass:
- * the gradients, un-compositing the rest of the ray with a (1 - alpha)
- * division); nothing per sample is stored. */
+ * accumulators. Each ray is re-marched once for the gradients, un-compositing
+ * the rest of the ray with a (1 - alpha) division from its final colour:
+ * dev_rgb64 as mk_tf_forward wrote it for the same view, LUT and scale, or
+ * NULL, when a first march recomputes it. Nothing per sample is stored. */
 mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, const float* dev_lut, int n,
-                         float opacity_scale, const float* dev_grad_rgb, double* dev_grad_lut,
-                         double* dev_grad_scale, void* stream);
+                         float opacity_scale, const float* dev_grad_rgb, const double* dev_rgb64,
+                         double* dev_grad_lut, double* dev_grad_scale, void* stream);
 
 /* ---- VR overlay: the orientation figure --------------------------------- */
 

@@ -130,8 +130,8 @@ def load() -> C.CDLL:
         "mk_render_cpr_async": (st, [vp, i, i, vp, vp, i, i, vp, vp]),
         "mk_marching_cubes": (st, [vp, vp, C.c_float, vp, C.c_uint64, C.POINTER(C.c_uint64)]),
         "mk_marching_cubes_async": (st, [vp, vp, C.c_float, vp, C.c_uint64, vp, vp]),
-        "mk_tf_forward": (st, [vp, C.POINTER(RenderParams), vp, i, C.c_float, vp, vp]),
-        "mk_tf_backward": (st, [vp, C.POINTER(RenderParams), vp, i, C.c_float, vp, vp, vp, vp]),
+        "mk_tf_forward": (st, [vp, C.POINTER(RenderParams), vp, i, C.c_float, vp, vp, vp]),
+        "mk_tf_backward": (st, [vp, C.POINTER(RenderParams), vp, i, C.c_float, vp, vp, vp, vp, vp]),
         "mk_png_max_bytes": (C.c_uint64, [i, i, i]),
         "mk_png_workspace_bytes": (C.c_uint64, [i, i, i]),
         "mk_png_encode_async": (st, [vp, i, i, i, vp, C.c_uint64, vp, C.c_uint64, vp, vp]),
@@ -322,13 +322,17 @@ class Context:
         self._check(self.lib.mk_marching_cubes_async(self.h, _ptr(b), iso, _ptr(dev_out), capacity, _ptr(dev_count),
                                                      stream))
 
-    def tf_forward(self, p: RenderParams, dev_lut, n: int, scale: float, dev_rgb, stream: int = 0) -> None:
-        self._check(self.lib.mk_tf_forward(self.h, C.byref(p), _ptr(dev_lut), n, scale, _ptr(dev_rgb), stream))
+    def tf_forward(self, p: RenderParams, dev_lut, n: int, scale: float, dev_rgb, stream: int = 0,
+                   dev_rgb64=None) -> None:
+        """dev_rgb64 (optional): the colours in float64 as well, for tf_backward."""
+        self._check(self.lib.mk_tf_forward(self.h, C.byref(p), _ptr(dev_lut), n, scale, _ptr(dev_rgb),
+                                           _ptr(dev_rgb64), stream))
 
     def tf_backward(self, p: RenderParams, dev_lut, n: int, scale: float, dev_grad_rgb, dev_grad_lut, dev_grad_scale,
-                    stream: int = 0) -> None:
+                    stream: int = 0, dev_rgb64=None) -> None:
+        """dev_rgb64 (optional): tf_forward's float64 colours of this view; without them each ray is marched twice."""
         self._check(self.lib.mk_tf_backward(self.h, C.byref(p), _ptr(dev_lut), n, scale, _ptr(dev_grad_rgb),
-                                            _ptr(dev_grad_lut), _ptr(dev_grad_scale), stream))
+                                            _ptr(dev_rgb64), _ptr(dev_grad_lut), _ptr(dev_grad_scale), stream))
 
 
 class Batcher:

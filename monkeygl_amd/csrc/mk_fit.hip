@@ -14,10 +14,11 @@
 //   dC/dc_i.rgb = T_i a_i k_i,
 //   dC/da_i     = T_i k_i c_i.rgb - (C - C_{<=i}) / (1 - a_i),
 // where C_{<=i} is the colour composited up to and including sample i. The
-// adjoint re-marches each ray: a first pass gives C, a second pass
-// un-composites the remainder with that (1 - a) division and scatters the
-// LUT and scale gradients into per-workgroup LDS (double), then into HBM with
-// double atomics. No sample is stored.
+// adjoint re-marches each ray once, un-compositing the remainder with that
+// (1 - a) division, and scatters the LUT and scale gradients into
+// per-workgroup LDS (double), then into HBM with double atomics. C comes from
+// the forward pass, which can keep it in double (24 bytes per pixel); without
+// it the adjoint first re-marches the ray for C. No sample is stored.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -159,8 +160,9 @@ __device__ inline void fit_tile(const FitArgs& A, int& x, int& y) {
     y = (int)(t / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
 }
 
-// Forward: float RGB per pixel (width*height*3).
-__global__ void __launch_bounds__(256) k_fit_forward(FitArgs A, float* __restrict__ out) {
+// Forward: float RGB per pixel (width*height*3), and the same colours in
+// double into out64 unless it is null.
+__global__ void __launch_bounds__(256) k_fit_forward(FitArgs A, float* __restrict__ out, double* __restrict__ out64) {
     extern __shared__ float4 s_lut[];
     for (int i = threadIdx.x; i < A.n; i += blockDim.x) s_lut[i] = A.lut[i];
     __syncthreads();
@@ -191,10 +193,18 @@ __global__ void __launch_bounds__(256) k_fit_forward(FitArgs A, float* __restric
     o[0] = (float)C[0];
     o[1] = (float)C[1];
     o[2] = (float)C[2];
+    if (out64) {
+        double* o64 = out64 + 3 * ((size_t)y * A.width + x);
+        o64[0] = C[0];
+        o64[1] = C[1];
+        o64[2] = C[2];
+    }
 }
 
 // Adjoint: given g = dL/dC per pixel, accumulate dL/dLUT (n x 4) and dL/ds.
+// color64 holds the forward pass's double colours, or is null.
 __global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __restrict__ grad_img,
+                                                      const double* __restrict__ color64,
                                                       double* __restrict__ grad_lut, double* __restrict__ grad_scale) {
     extern __shared__ float4 smem[];
     float4* s_lut = smem;
@@ -210,10 +220,16 @@ __global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __
         const double g[3] = {gp[0], gp[1], gp[2]};
         const float3 L = fit_light(A);
         if (g[0] != 0.0 || g[1] != 0.0 || g[2] != 0.0) {
-            // pass 1: the final colour
+            // the final colour: kept by the forward pass, else a first march
             double C[3] = {0.0, 0.0, 0.0}, T = 1.0;
             float acc = acc0;
-            for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
+            if (color64) {
+                const double* cp = color64 + 3 * ((size_t)y * A.width + x);
+                C[0] = cp[0];
+                C[1] = cp[1];
+                C[2] = cp[2];
+            }
+            for (int it = 0; !color64 && it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
                 float p[3];
                 fit_pos(A, u, v, acc, p);
                 acc += A.l1;
@@ -227,7 +243,7 @@ __global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __
                 C[2] += w * S.c.z;
                 T *= 1.0 - (double)S.a;
             }
-            // pass 2: gradients, un-compositing the remainder of the ray
+            // gradients, un-compositing the remainder of the ray
             double Cp[3] = {0.0, 0.0, 0.0};
             T = 1.0;
             acc = acc0;
@@ -318,7 +334,7 @@ mk_status fit_prepare(mk_context* ctx, const mk_render_params* p, const float* d
 }  // namespace
 
 extern "C" mk_status mk_tf_forward(mk_context* ctx, const mk_render_params* p, const float* dev_lut, int n,
-                                   float opacity_scale, float* dev_rgb, void* stream) {
+                                   float opacity_scale, float* dev_rgb, double* dev_rgb64, void* stream) {
     if (!ctx) return MK_ERR_ARG;
     MK_HIP(ctx, hipSetDevice(ctx->device));
     FitArgs A;
@@ -329,14 +345,14 @@ extern "C" mk_status mk_tf_forward(mk_context* ctx, const mk_render_params* p, c
     st = mk_order_after_context(ctx, s);
     if (st) return st;
     const uint32_t tiles = (uint32_t)(((p->width + 15) / 16) * ((p->height + 15) / 16));
-    hipLaunchKernelGGL(k_fit_forward, dim3(tiles), dim3(256), (size_t)n * sizeof(float4), s, A, dev_rgb);
+    hipLaunchKernelGGL(k_fit_forward, dim3(tiles), dim3(256), (size_t)n * sizeof(float4), s, A, dev_rgb, dev_rgb64);
     MK_HIP(ctx, hipGetLastError());
     return MK_OK;
 }
 
 extern "C" mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, const float* dev_lut, int n,
-                                    float opacity_scale, const float* dev_grad_rgb, double* dev_grad_lut,
-                                    double* dev_grad_scale, void* stream) {
+                                    float opacity_scale, const float* dev_grad_rgb, const double* dev_rgb64,
+                                    double* dev_grad_lut, double* dev_grad_scale, void* stream) {
     if (!ctx) return MK_ERR_ARG;
     MK_HIP(ctx, hipSetDevice(ctx->device));
     FitArgs A;
@@ -349,7 +365,8 @@ extern "C" mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, 
     const uint32_t tiles = (uint32_t)(((p->width + 15) / 16) * ((p->height + 15) / 16));
     const size_t lds = (size_t)n * sizeof(float4) + (4 * (size_t)n + 1) * sizeof(double);
     if (lds > 64 * 1024) return mk_fail(ctx, MK_ERR_ARG, "LUT of %d entries exceeds the LDS budget of the adjoint", n);
-    hipLaunchKernelGGL(k_fit_backward, dim3(tiles), dim3(256), lds, s, A, dev_grad_rgb, dev_grad_lut, dev_grad_scale);
+    hipLaunchKernelGGL(k_fit_backward, dim3(tiles), dim3(256), lds, s, A, dev_grad_rgb, dev_rgb64, dev_grad_lut,
+                       dev_grad_scale);
     MK_HIP(ctx, hipGetLastError());
     return MK_OK;
 }

@@ -3,8 +3,10 @@
 ``render_tf(ctx, views, lut, scale)`` renders the fitting rule of
 ``mk_tf_forward`` (see include/monkeygl_hip.h) for a list of views and is
 differentiable with respect to the LUT (n x 4, float32, on the GPU) and the
-opacity scale (a 0-d tensor). The backward pass calls ``mk_tf_backward``,
-which re-marches every ray; gradients accumulate in float64 on the device.
+opacity scale (a 0-d tensor). When a gradient is wanted the forward pass also
+keeps the colours in float64 (24 bytes per pixel), and the backward pass
+calls ``mk_tf_backward`` with them, which re-marches every ray once;
+gradients accumulate in float64 on the device.
 """
 from __future__ import annotations
 
@@ -24,11 +26,14 @@ class _TFRender(torch.autograd.Function):
         if any(p.height != h or p.width != w for p in views):
             raise ValueError("all views must have the same image size")
         out = torch.empty((len(views), h, w, 3), dtype=torch.float32, device=lut.device)
+        keep = ctx_.needs_input_grad[0] or ctx_.needs_input_grad[1]
+        rgb64 = torch.empty((len(views), h, w, 3), dtype=torch.float64, device=lut.device) if keep else None
         stream = torch.cuda.current_stream(lut.device).cuda_stream
         for i, p in enumerate(views):
-            mk.tf_forward(p, lut_c, lut_c.shape[0], s, out[i], stream=stream)
+            mk.tf_forward(p, lut_c, lut_c.shape[0], s, out[i], stream=stream,
+                          dev_rgb64=None if rgb64 is None else rgb64[i])
         ctx_.save_for_backward(lut_c)
-        ctx_.mk, ctx_.views, ctx_.s, ctx_.scale_like = mk, views, s, scale
+        ctx_.mk, ctx_.views, ctx_.s, ctx_.scale_like, ctx_.rgb64 = mk, views, s, scale, rgb64
         return out
 
     @staticmethod
@@ -40,7 +45,7 @@ class _TFRender(torch.autograd.Function):
         g_s = torch.zeros(1, dtype=torch.float64, device=lut.device)
         stream = torch.cuda.current_stream(lut.device).cuda_stream
         for i, p in enumerate(ctx_.views):
-            ctx_.mk.tf_backward(p, lut, n, ctx_.s, g[i], g_lut, g_s, stream=stream)
+            ctx_.mk.tf_backward(p, lut, n, ctx_.s, g[i], g_lut, g_s, stream=stream, dev_rgb64=ctx_.rgb64[i])
         sl = ctx_.scale_like
         return (g_lut.view(n, 4).to(torch.float32), g_s.view(()).to(sl.dtype).to(sl.device), None, None)
 

@@ -132,6 +132,38 @@ def test_tf_gradient_matches_fp64_finite_differences(gpu_ctx):
 
 
 @pytest.mark.gpu
+def test_tf_backward_from_kept_colours_equals_remarch(gpu_ctx):
+    """The adjoint given the forward pass's float64 colours equals the adjoint
+    that re-marches each ray for them (up to the order of the double atomic
+    sums), and those colours round to the float image."""
+    import torch
+
+    vol = W.make_volume(SPEC)
+    gpu_ctx.set_volume(vol)
+    lut = torch.tensor(_lut(4), device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    for p in _views(angles=((0.0, 0.0), (35.0, 20.0), (-60.0, 75.0))):
+        rgb = torch.empty((p.height, p.width, 3), dtype=torch.float32, device="cuda")
+        rgb64 = torch.full((p.height, p.width, 3), float("nan"), dtype=torch.float64, device="cuda")
+        gpu_ctx.tf_forward(p, lut, N_LUT, 1.4, rgb, stream=stream, dev_rgb64=rgb64)
+        gen = torch.Generator(device="cuda").manual_seed(5)
+        g = torch.randn((p.height, p.width, 3), generator=gen, device="cuda")
+        out = []
+        for kept in (rgb64, None):
+            g_lut = torch.zeros(N_LUT * 4, dtype=torch.float64, device="cuda")
+            g_s = torch.zeros(1, dtype=torch.float64, device="cuda")
+            gpu_ctx.tf_backward(p, lut, N_LUT, 1.4, g, g_lut, g_s, stream=stream, dev_rgb64=kept)
+            out.append((g_lut, g_s))
+        torch.cuda.synchronize()
+        assert torch.equal(rgb64.to(torch.float32), rgb)
+        (a, sa), (b, sb) = out
+        tol = 1e-10 * float(b.abs().max())
+        assert float(b.abs().max()) > 0.0
+        assert float((a - b).abs().max()) <= tol, float((a - b).abs().max())
+        assert abs(float(sa) - float(sb)) <= 1e-10 * abs(float(sb)), (float(sa), float(sb))
+
+
+@pytest.mark.gpu
 def test_short_fit_reduces_loss(gpu_ctx):
     import torch
 
