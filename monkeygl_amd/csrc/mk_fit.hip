@@ -75,9 +75,9 @@ __device__ inline bool fit_voi(const FitArgs& A, const float p[3]) {
 // interpolations (with p's weights) of voxel differences two apart: the
 // sample's own eight corners plus 24 more voxels, loaded together; the common
 // 32768/32767 scale cancels in the normalization.
-__device__ inline float fit_shade(const FitArgs& A, const mkr::TriSample& tri, float3 L) {
+__device__ inline float fit_shade(const FitArgs& A, const mkr::TriSample& tri, const mkr::GradTaps& taps, float3 L) {
     float N[3];
-    mkr::grad_diff_kept(A.v, tri, N);
+    mkr::grad_diff_finish(tri, taps, N);
     if (A.invert_z) N[2] = -N[2];
     float r = rsqrtf(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
     float d = (N[0] * L.x + N[1] * L.y + N[2] * L.z) * r;
@@ -97,10 +97,10 @@ struct Sample {
     bool grad_ok; // the clamp is inactive: 0 <= s*alpha < 1 (one-sided at 0)
 };
 
-// tri keeps the trilinear sample's corners for fit_shade.
-__device__ inline Sample fit_sample(const FitArgs& A, const float4* s_lut, const float p[3], mkr::TriSample& tri) {
+// A sample from its fetched corners; tri keeps them for fit_shade.
+__device__ inline Sample fit_sample(const FitArgs& A, const float4* s_lut, const mkr::TriFetch& F, mkr::TriSample& tri) {
     Sample S;
-    float t = mkr::sample_keep(A.v, p, tri);
+    float t = mkr::sample_finish(F, tri);
     t = (t - A.wl) / A.ww + 0.5f;
     t = t > 1.0f ? 1.0f : (t >= -1.0f ? t : -1.0f);
     mk_split(t, A.n, S.i0, S.i1, S.f);
@@ -160,6 +160,58 @@ __device__ inline void fit_tile(const FitArgs& A, int& x, int& y) {
     y = (int)(t / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
 }
 
+// Lattice points marched per group: the group's positions do not depend on
+// its samples (fixed step), so all their corner and gradient loads are issued
+// before the first is used and share one round trip. One point already issues
+// its 8 corners and 24 gradient taps together; groups of 2, 3 or 4 measured
+// the same (the loads are bound by the L1 cache's access rate: 64-lane
+// gathers touch ~34 cache lines each, and 99.8% hit) and need more VGPRs.
+constexpr int kFitGroup = 1;
+
+// Visits the ray's in-VOI lattice points in order: visit(S, k) gets the
+// sample and its shading factor and returns false to end the march. The
+// points are those of the loop
+//   for (it = 0; it < max_iter && (double)acc < 1.732 && acc <= a_out; ++it)
+//     { p = pos(acc); acc += L1; if in VOI: visit }
+// taken kFitGroup at a time; loads of points past the end read clamped
+// in-volume voxels and are not used.
+template <typename Visit>
+__device__ inline void fit_march(const FitArgs& A, const float4* s_lut, float u, float v, float acc, float a_out,
+                                 float3 L, Visit&& visit) {
+    int it = 0;
+    bool more = true;
+    while (more) {
+        float P[kFitGroup][3];
+        bool use[kFitGroup];
+#pragma unroll
+        for (int j = 0; j < kFitGroup; ++j) {
+            const bool ok = more && it < A.max_iter && (double)acc < 1.732 && acc <= a_out;
+            fit_pos(A, u, v, acc, P[j]);
+            use[j] = ok && fit_voi(A, P[j]);
+            acc = ok ? acc + A.l1 : acc;
+            it += ok ? 1 : 0;
+            more = ok;
+        }
+        mkr::TriFetch F[kFitGroup];
+        mkr::GradTaps G[kFitGroup];
+#pragma unroll
+        for (int j = 0; j < kFitGroup; ++j) {
+            mkr::sample_fetch(A.v, P[j], F[j]);
+            mkr::grad_taps_fetch(A.v, F[j].ix, F[j].iy, F[j].iz, G[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kFitGroup; ++j) {
+            if (!use[j]) continue;
+            mkr::TriSample tri;
+            const Sample S = fit_sample(A, s_lut, F[j], tri);
+            if (!visit(S, fit_shade(A, tri, G[j], L))) {
+                more = false;
+                break;
+            }
+        }
+    }
+}
+
 // Forward: float RGB per pixel (width*height*3), and the same colours in
 // double into out64 unless it is null.
 __global__ void __launch_bounds__(256) k_fit_forward(FitArgs A, float* __restrict__ out, double* __restrict__ out64) {
@@ -172,22 +224,16 @@ __global__ void __launch_bounds__(256) k_fit_forward(FitArgs A, float* __restric
     double C[3] = {0.0, 0.0, 0.0}, T = 1.0;
     float u, v, acc, a_out;
     if (fit_ray(A, x, y, u, v, acc, a_out)) {
-        const float3 L = fit_light(A);
-        for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
-            float p[3];
-            fit_pos(A, u, v, acc, p);
-            acc += A.l1;
-            if (!fit_voi(A, p)) continue;
-            mkr::TriSample tri;
-            const Sample S = fit_sample(A, s_lut, p, tri);
-            if (!(S.a > 0.0f)) continue;
-            const float k = fit_shade(A, tri, L);
-            const double w = T * (double)S.a * (double)k;
-            C[0] += w * S.c.x;
-            C[1] += w * S.c.y;
-            C[2] += w * S.c.z;
-            T *= 1.0 - (double)S.a;
-        }
+        fit_march(A, s_lut, u, v, acc, a_out, fit_light(A), [&](const Sample& S, float k) {
+            if (S.a > 0.0f) {
+                const double w = T * (double)S.a * (double)k;
+                C[0] += w * S.c.x;
+                C[1] += w * S.c.y;
+                C[2] += w * S.c.z;
+                T *= 1.0 - (double)S.a;
+            }
+            return true;
+        });
     }
     float* o = out + 3 * ((size_t)y * A.width + x);
     o[0] = (float)C[0];
@@ -222,42 +268,31 @@ __global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __
         if (g[0] != 0.0 || g[1] != 0.0 || g[2] != 0.0) {
             // the final colour: kept by the forward pass, else a first march
             double C[3] = {0.0, 0.0, 0.0}, T = 1.0;
-            float acc = acc0;
             if (color64) {
                 const double* cp = color64 + 3 * ((size_t)y * A.width + x);
                 C[0] = cp[0];
                 C[1] = cp[1];
                 C[2] = cp[2];
-            }
-            for (int it = 0; !color64 && it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
-                float p[3];
-                fit_pos(A, u, v, acc, p);
-                acc += A.l1;
-                if (!fit_voi(A, p)) continue;
-                mkr::TriSample tri;
-                const Sample S = fit_sample(A, s_lut, p, tri);
-                if (!(S.a > 0.0f)) continue;
-                const double w = T * (double)S.a * (double)fit_shade(A, tri, L);
-                C[0] += w * S.c.x;
-                C[1] += w * S.c.y;
-                C[2] += w * S.c.z;
-                T *= 1.0 - (double)S.a;
+            } else {
+                fit_march(A, s_lut, u, v, acc0, a_out, L, [&](const Sample& S, float k) {
+                    if (S.a > 0.0f) {
+                        const double w = T * (double)S.a * (double)k;
+                        C[0] += w * S.c.x;
+                        C[1] += w * S.c.y;
+                        C[2] += w * S.c.z;
+                        T *= 1.0 - (double)S.a;
+                    }
+                    return true;
+                });
             }
             // gradients, un-compositing the remainder of the ray
             double Cp[3] = {0.0, 0.0, 0.0};
             T = 1.0;
-            acc = acc0;
             double gs = 0.0;
-            for (int it = 0; it < A.max_iter && (double)acc < 1.732 && acc <= a_out; ++it) {
-                float p[3];
-                fit_pos(A, u, v, acc, p);
-                acc += A.l1;
-                if (!fit_voi(A, p)) continue;
-                mkr::TriSample tri;
-                const Sample S = fit_sample(A, s_lut, p, tri);
+            fit_march(A, s_lut, u, v, acc0, a_out, L, [&](const Sample& S, float kf) {
                 // samples with zero alpha still move the image when their LUT
                 // entries grow: da > 0 from 0 is a smooth onset
-                const double k = (double)fit_shade(A, tri, L);
+                const double k = (double)kf;
                 const double a = S.a;
                 const double w = T * a * k;
                 const double c[3] = {S.c.x, S.c.y, S.c.z};
@@ -282,8 +317,8 @@ __global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __
                     atomicAdd(&s_g[4 * S.i0 + ch], (1.0 - f) * d4[ch]);
                     atomicAdd(&s_g[4 * S.i1 + ch], f * d4[ch]);
                 }
-                if (T == 0.0) break;
-            }
+                return T != 0.0;
+            });
             atomicAdd(&s_g[4 * A.n], gs);
         }
     }

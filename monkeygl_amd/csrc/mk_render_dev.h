@@ -268,35 +268,46 @@ __device__ inline float sample_keep(const MkVolView& v, const float p[3], TriSam
     return sample_finish(F, T);
 }
 
-// The central differences at a kept sample, unnormalized and without the
-// common 32768/32767 scale: 24 more voxels around its centre block.
-__device__ inline void grad_diff_kept(const MkVolView& v, const TriSample& T, float N[3]) {
-    const float ax = T.ax, ay = T.ay, az = T.az;
+// The 24 voxels around a sample's centre block that its central differences
+// read (the outer layers at index 0 and 3 of each axis), loaded but not yet
+// used, so that a caller can put them in the same round trip as other loads.
+// Their addresses depend on the sample's position only. Voxels as floats
+// (-32768 read as -32767), [k][j][end] per axis as grad_diff_finish reads them.
+struct GradTaps {
+    float x[2][2][2], y[2][2][2], z[2][2][2];
+};
+
+__device__ inline void grad_taps_fetch(const MkVolView& v, int ix, int iy, int iz, GradTaps& G) {
     const uint32_t sy = (uint32_t)v.nbx * 512u, sz = (uint32_t)v.nbx * (uint32_t)v.nby * 512u;
     uint32_t OX[4], OY[4], OZ[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int x = mk_clampi(T.ix - 1 + k, 0, v.nx - 1);
-        const int y = mk_clampi(T.iy - 1 + k, 0, v.ny - 1);
-        const int z = mk_clampi(T.iz - 1 + k, 0, v.nz - 1);
+        const int x = mk_clampi(ix - 1 + k, 0, v.nx - 1);
+        const int y = mk_clampi(iy - 1 + k, 0, v.ny - 1);
+        const int z = mk_clampi(iz - 1 + k, 0, v.nz - 1);
         OX[k] = (uint32_t)(x >> 3) * 512u + (uint32_t)(x & 7);
         OY[k] = (uint32_t)(y >> 3) * sy + (uint32_t)(y & 7) * 8u;
         OZ[k] = (uint32_t)(z >> 3) * sz + (uint32_t)(z & 7) * 64u;
     }
 #define MK_V(a, b, c) mk_rawo(v, OX[a] + OY[b] + OZ[c])
-    float ex[2][2][2], ey[2][2][2], ez[2][2][2];   // the outer layers at index 0 and 3
 #pragma unroll
     for (int k = 0; k < 2; ++k)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            ex[k][j][0] = MK_V(0, 1 + j, 1 + k);
-            ex[k][j][1] = MK_V(3, 1 + j, 1 + k);
-            ey[k][j][0] = MK_V(1 + j, 0, 1 + k);
-            ey[k][j][1] = MK_V(1 + j, 3, 1 + k);
-            ez[k][j][0] = MK_V(1 + j, 1 + k, 0);
-            ez[k][j][1] = MK_V(1 + j, 1 + k, 3);
+            G.x[k][j][0] = MK_V(0, 1 + j, 1 + k);
+            G.x[k][j][1] = MK_V(3, 1 + j, 1 + k);
+            G.y[k][j][0] = MK_V(1 + j, 0, 1 + k);
+            G.y[k][j][1] = MK_V(1 + j, 3, 1 + k);
+            G.z[k][j][0] = MK_V(1 + j, 1 + k, 0);
+            G.z[k][j][1] = MK_V(1 + j, 1 + k, 3);
         }
 #undef MK_V
+}
+
+// The central differences at a kept sample from its fetched taps,
+// unnormalized and without the common 32768/32767 scale.
+__device__ inline void grad_diff_finish(const TriSample& T, const GradTaps& G, float N[3]) {
+    const float ax = T.ax, ay = T.ay, az = T.az;
     const float(&c)[2][2][2] = T.c;
     // x: lerp(V2 - V0, V3 - V1, ax) per (y, z), then bilinear in (ay, az)
     float dx[2][2], dy[2][2], dz[2][2];
@@ -304,13 +315,13 @@ __device__ inline void grad_diff_kept(const MkVolView& v, const TriSample& T, fl
     for (int k = 0; k < 2; ++k)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const float lo = c[k][j][1] - ex[k][j][0], hi = ex[k][j][1] - c[k][j][0];
+            const float lo = c[k][j][1] - G.x[k][j][0], hi = G.x[k][j][1] - c[k][j][0];
             dx[k][j] = lo + ax * (hi - lo);
-            // y: index j of ey runs over x (1 + j), k over z (1 + k)
-            const float loy = c[k][1][j] - ey[k][j][0], hiy = ey[k][j][1] - c[k][0][j];
+            // y: index j of the y taps runs over x (1 + j), k over z (1 + k)
+            const float loy = c[k][1][j] - G.y[k][j][0], hiy = G.y[k][j][1] - c[k][0][j];
             dy[k][j] = loy + ay * (hiy - loy);
-            // z: ez[k][j] is at x = 1 + j, y = 1 + k
-            const float loz = c[1][k][j] - ez[k][j][0], hiz = ez[k][j][1] - c[0][k][j];
+            // z: the z taps [k][j] are at x = 1 + j, y = 1 + k
+            const float loz = c[1][k][j] - G.z[k][j][0], hiz = G.z[k][j][1] - c[0][k][j];
             dz[k][j] = loz + az * (hiz - loz);
         }
     {   // dx[k][j]: j over y, k over z
@@ -328,6 +339,14 @@ __device__ inline void grad_diff_kept(const MkVolView& v, const TriSample& T, fl
         const float a1 = dz[1][0] + ax * (dz[1][1] - dz[1][0]);
         N[2] = a0 + ay * (a1 - a0);
     }
+}
+
+// The central differences at a kept sample, unnormalized and without the
+// common 32768/32767 scale: 24 more voxels around its centre block.
+__device__ inline void grad_diff_kept(const MkVolView& v, const TriSample& T, float N[3]) {
+    GradTaps G;
+    grad_taps_fetch(v, T.ix, T.iy, T.iz, G);
+    grad_diff_finish(T, G, N);
 }
 
 // The gradient from a kept sample (normalized, z flipped with the volume).
