@@ -29,8 +29,37 @@ __host__ __device__ inline uint32_t mk_brick_index(const MkVolView& v, int x, in
            (uint32_t)(x >> MK_BRICK_SHIFT);
 }
 
+// Voxel order inside a volume brick: its 2x2x2 blocks of 4x4x4 voxels (128
+// bytes, one cache line each) in x, y, z order, x fastest inside a block. The
+// 64 samples of a wave lie on a small patch of a plane at any tilt; blocks
+// give that patch about half the cache lines that 8x8 z-slices of the brick
+// would. Bits of (x, y, z) & 7 in the in-brick offset: x0 x1 y0 y1 z0 z1 x2 y2 z2.
+__host__ __device__ inline uint32_t mk_inbrick(int x, int y, int z) {
+    return ((uint32_t)(z & 4) << 6) | ((uint32_t)(y & 4) << 5) | ((uint32_t)(x & 4) << 4) | ((uint32_t)(z & 3) << 4) |
+           ((uint32_t)(y & 3) << 2) | (uint32_t)(x & 3);
+}
+
 __host__ __device__ inline uint32_t mk_voxel_offset(const MkVolView& v, int x, int y, int z) {
+    return (mk_brick_index(v, x, y, z) << 9) | mk_inbrick(x, y, z);
+}
+
+// The label mask keeps x-rows inside its bricks (8 bytes per row, z slowest):
+// its label kernels handle a brick as 64 rows.
+__host__ __device__ inline uint32_t mk_mask_offset(const MkVolView& v, int x, int y, int z) {
     return (mk_brick_index(v, x, y, z) << 9) | ((uint32_t)(z & 7) << 6) | ((uint32_t)(y & 7) << 3) | (uint32_t)(x & 7);
+}
+
+// mk_voxel_offset as a sum of one term per axis, for samplers that combine
+// several indices per axis.
+__device__ inline uint32_t mk_term_x(int x) {
+    return ((uint32_t)(x >> 3) << 9) | ((uint32_t)(x & 4) << 4) | (uint32_t)(x & 3);
+}
+__device__ inline uint32_t mk_term_y(const MkVolView& v, int y) {
+    return (uint32_t)(y >> 3) * ((uint32_t)v.nbx << 9) + (((uint32_t)(y & 4) << 5) | ((uint32_t)(y & 3) << 2));
+}
+__device__ inline uint32_t mk_term_z(const MkVolView& v, int z) {
+    return (uint32_t)(z >> 3) * (((uint32_t)v.nbx * (uint32_t)v.nby) << 9) +
+           (((uint32_t)(z & 4) << 6) | ((uint32_t)(z & 3) << 4));
 }
 
 __device__ inline int mk_clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -64,23 +93,21 @@ __device__ inline float mk_rawo(const MkVolView& v, uint32_t off) {
 }
 
 // Trilinear sample in HU' units (32768 * normalized-float read). The eight
-// corner offsets come from one brick-offset computation plus per-axis deltas:
-// +1/+8/+64 inside a brick, a jump to the next brick across its face, 0 where
-// the index clamps at the volume edge.
+// corner offsets are sums of two terms per axis (mk_term_*); an index that
+// clamps at the volume edge repeats its neighbour's term.
 __device__ inline float mk_sample_hu(const MkVolView& v, float fx, float fy, float fz) {
     int x0, x1, y0, y1, z0, z1;
     float ax, ay, az;
     mk_split(fx, v.nx, x0, x1, ax);
     mk_split(fy, v.ny, y0, y1, ay);
     mk_split(fz, v.nz, z0, z1, az);
-    const uint32_t o = mk_voxel_offset(v, x0, y0, z0);
-    const uint32_t dx = x1 == x0 ? 0u : ((x0 & 7) == 7 ? 512u - 7u : 1u);
-    const uint32_t dy = y1 == y0 ? 0u : ((y0 & 7) == 7 ? (uint32_t)v.nbx * 512u - 56u : 8u);
-    const uint32_t dz = z1 == z0 ? 0u : ((z0 & 7) == 7 ? (uint32_t)v.nbx * (uint32_t)v.nby * 512u - 448u : 64u);
-    float c000 = mk_rawo(v, o), c100 = mk_rawo(v, o + dx);
-    float c010 = mk_rawo(v, o + dy), c110 = mk_rawo(v, o + dy + dx);
-    float c001 = mk_rawo(v, o + dz), c101 = mk_rawo(v, o + dz + dx);
-    float c011 = mk_rawo(v, o + dz + dy), c111 = mk_rawo(v, o + dz + dy + dx);
+    const uint32_t X0 = mk_term_x(x0), X1 = mk_term_x(x1);
+    const uint32_t Y0 = mk_term_y(v, y0), Y1 = mk_term_y(v, y1);
+    const uint32_t Z0 = mk_term_z(v, z0), Z1 = mk_term_z(v, z1);
+    float c000 = mk_rawo(v, X0 + Y0 + Z0), c100 = mk_rawo(v, X1 + Y0 + Z0);
+    float c010 = mk_rawo(v, X0 + Y1 + Z0), c110 = mk_rawo(v, X1 + Y1 + Z0);
+    float c001 = mk_rawo(v, X0 + Y0 + Z1), c101 = mk_rawo(v, X1 + Y0 + Z1);
+    float c011 = mk_rawo(v, X0 + Y1 + Z1), c111 = mk_rawo(v, X1 + Y1 + Z1);
     float c00 = c000 + ax * (c100 - c000);
     float c10 = c010 + ax * (c110 - c010);
     float c01 = c001 + ax * (c101 - c001);

@@ -42,11 +42,31 @@ __device__ inline void cube_coords(const McArgs& A, uint64_t q, int& i, int& j, 
     k = A.z0 + (int)(r / (uint64_t)A.cy);
 }
 
-__device__ inline int cube_case(const McArgs& A, int i, int j, int k, float f[8]) {
+// The corner values and case index of cube (i, j, k) (valid: the lane holds a
+// cube). Lanes hold consecutive cubes, so a cube's four corners at x = i + 1
+// are the next lane's corners at x = i: a shuffle brings them unless the next
+// lane is outside the wave or starts another row. Every lane of the wave
+// calls this.
+__device__ inline int cube_case(const McArgs& A, bool valid, int i, int j, int k, float f[8]) {
+    float lo[2][2], hi[2][2];   // corners at x = i and x = i + 1, [dz][dy]
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+            lo[dz][dy] = valid ? (float)A.v.vox[mk_voxel_offset(A.v, i, j + dy, k + dz)] : 0.0f;
+    const bool from_next = valid && (threadIdx.x & 63) != 63 && i + 1 < A.x0 + A.cx;
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            const float nxt = __shfl_down(lo[dz][dy], 1);
+            hi[dz][dy] = from_next ? nxt : (valid ? (float)A.v.vox[mk_voxel_offset(A.v, i + 1, j + dy, k + dz)] : 0.0f);
+        }
     int c = 0;
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
-        f[n] = (float)A.v.vox[mk_voxel_offset(A.v, i + kMkMcCorner[n][0], j + kMkMcCorner[n][1], k + kMkMcCorner[n][2])];
+        const int dy = kMkMcCorner[n][1], dz = kMkMcCorner[n][2];
+        f[n] = kMkMcCorner[n][0] ? hi[dz][dy] : lo[dz][dy];
         c |= (f[n] < A.iso ? 1 : 0) << n;
     }
     return c;
@@ -72,13 +92,12 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, ui
 __global__ void __launch_bounds__(kMcBlock) k_mc_count(McArgs A, uint32_t* __restrict__ block_sum) {
     __shared__ uint32_t s_wave[4];
     const uint64_t q = (uint64_t)blockIdx.x * kMcBlock + threadIdx.x;
-    uint32_t n = 0;
-    if (q < A.ncubes) {
-        int i, j, k;
-        cube_coords(A, q, i, j, k);
-        float f[8];
-        n = A.cnt[cube_case(A, i, j, k, f)];
-    }
+    const bool valid = q < A.ncubes;
+    int i = 0, j = 0, k = 0;
+    if (valid) cube_coords(A, q, i, j, k);
+    float f[8];
+    const int c = cube_case(A, valid, i, j, k, f);
+    const uint32_t n = valid ? A.cnt[c] : 0u;
     uint32_t total;
     (void)block_exclusive_scan(n, s_wave, total);
     if (threadIdx.x == 0) block_sum[blockIdx.x] = total;
@@ -112,14 +131,12 @@ __global__ void __launch_bounds__(kMcBlock) k_mc_emit(McArgs A, const uint64_t* 
                                                       float* __restrict__ out, uint64_t capacity) {
     __shared__ uint32_t s_wave[4];
     const uint64_t q = (uint64_t)blockIdx.x * kMcBlock + threadIdx.x;
+    const bool valid = q < A.ncubes;
     float f[8];
-    int c = 0, i = 0, j = 0, k = 0;
-    uint32_t n = 0;
-    if (q < A.ncubes) {
-        cube_coords(A, q, i, j, k);
-        c = cube_case(A, i, j, k, f);
-        n = A.cnt[c];
-    }
+    int i = 0, j = 0, k = 0;
+    if (valid) cube_coords(A, q, i, j, k);
+    const int c = cube_case(A, valid, i, j, k, f);
+    const uint32_t n = valid ? A.cnt[c] : 0u;
     uint32_t total;
     const uint32_t rank = block_exclusive_scan(n, s_wave, total);
     uint64_t t = block_off[blockIdx.x] + rank;

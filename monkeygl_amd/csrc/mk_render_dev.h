@@ -221,21 +221,17 @@ __device__ inline void sample_fetch(const MkVolView& v, const float p[3], TriFet
     grad_axis_base(p[0], v.nx, F.ix, F.ax);
     grad_axis_base(p[1], v.ny, F.iy, F.ay);
     grad_axis_base(p[2], v.nz, F.iz, F.az);
-    const int x0 = mk_clampi(F.ix, 0, v.nx - 1), x1 = mk_clampi(F.ix + 1, 0, v.nx - 1);
-    const int y0 = mk_clampi(F.iy, 0, v.ny - 1), y1 = mk_clampi(F.iy + 1, 0, v.ny - 1);
-    const int z0 = mk_clampi(F.iz, 0, v.nz - 1), z1 = mk_clampi(F.iz + 1, 0, v.nz - 1);
-    const uint32_t o = mk_voxel_offset(v, x0, y0, z0);
-    const uint32_t dx = x1 == x0 ? 0u : ((x0 & 7) == 7 ? 512u - 7u : 1u);
-    const uint32_t dy = y1 == y0 ? 0u : ((y0 & 7) == 7 ? (uint32_t)v.nbx * 512u - 56u : 8u);
-    const uint32_t dz = z1 == z0 ? 0u : ((z0 & 7) == 7 ? (uint32_t)v.nbx * (uint32_t)v.nby * 512u - 448u : 64u);
-    F.r[0] = v.vox[o];
-    F.r[1] = v.vox[o + dx];
-    F.r[2] = v.vox[o + dy];
-    F.r[3] = v.vox[o + dy + dx];
-    F.r[4] = v.vox[o + dz];
-    F.r[5] = v.vox[o + dz + dx];
-    F.r[6] = v.vox[o + dz + dy];
-    F.r[7] = v.vox[o + dz + dy + dx];
+    const uint32_t X0 = mk_term_x(mk_clampi(F.ix, 0, v.nx - 1)), X1 = mk_term_x(mk_clampi(F.ix + 1, 0, v.nx - 1));
+    const uint32_t Y0 = mk_term_y(v, mk_clampi(F.iy, 0, v.ny - 1)), Y1 = mk_term_y(v, mk_clampi(F.iy + 1, 0, v.ny - 1));
+    const uint32_t Z0 = mk_term_z(v, mk_clampi(F.iz, 0, v.nz - 1)), Z1 = mk_term_z(v, mk_clampi(F.iz + 1, 0, v.nz - 1));
+    F.r[0] = v.vox[X0 + Y0 + Z0];
+    F.r[1] = v.vox[X1 + Y0 + Z0];
+    F.r[2] = v.vox[X0 + Y1 + Z0];
+    F.r[3] = v.vox[X1 + Y1 + Z0];
+    F.r[4] = v.vox[X0 + Y0 + Z1];
+    F.r[5] = v.vox[X1 + Y0 + Z1];
+    F.r[6] = v.vox[X0 + Y1 + Z1];
+    F.r[7] = v.vox[X1 + Y1 + Z1];
 }
 
 // The trilinear value of a fetched sample, keeping what the gradient reuses
@@ -278,16 +274,12 @@ struct GradTaps {
 };
 
 __device__ inline void grad_taps_fetch(const MkVolView& v, int ix, int iy, int iz, GradTaps& G) {
-    const uint32_t sy = (uint32_t)v.nbx * 512u, sz = (uint32_t)v.nbx * (uint32_t)v.nby * 512u;
     uint32_t OX[4], OY[4], OZ[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int x = mk_clampi(ix - 1 + k, 0, v.nx - 1);
-        const int y = mk_clampi(iy - 1 + k, 0, v.ny - 1);
-        const int z = mk_clampi(iz - 1 + k, 0, v.nz - 1);
-        OX[k] = (uint32_t)(x >> 3) * 512u + (uint32_t)(x & 7);
-        OY[k] = (uint32_t)(y >> 3) * sy + (uint32_t)(y & 7) * 8u;
-        OZ[k] = (uint32_t)(z >> 3) * sz + (uint32_t)(z & 7) * 64u;
+        OX[k] = mk_term_x(mk_clampi(ix - 1 + k, 0, v.nx - 1));
+        OY[k] = mk_term_y(v, mk_clampi(iy - 1 + k, 0, v.ny - 1));
+        OZ[k] = mk_term_z(v, mk_clampi(iz - 1 + k, 0, v.nz - 1));
     }
 #define MK_V(a, b, c) mk_rawo(v, OX[a] + OY[b] + OZ[c])
 #pragma unroll
@@ -582,7 +574,7 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
             int label = 0;
             TriSample tri;
             if (!empty_shift) {
-                if (MASK) label = (TOUCH || cur_label == 0xFF) ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label;
+                if (MASK) label = (TOUCH || cur_label == 0xFF) ? A.v.mask[mk_mask_offset(A.v, cx, cy, cz)] : cur_label;
                 if (TOUCH) {
                     if (MASK) touch_mask(A, cx, cy, cz);
                     touch_footprint(A, p[0], p[1], p[2]);
@@ -738,7 +730,7 @@ __device__ inline void vr_tail(const RenderArgs& A, const Smem& s, float* gobj) 
             }
             if (!(cur_macro_empty || !cur_occ)) {
                 int label = 0;
-                if (MASK) label = cur_label == 0xFF ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label;
+                if (MASK) label = cur_label == 0xFF ? A.v.mask[mk_mask_offset(A.v, cx, cy, cz)] : cur_label;
                 const float4 aw = s.aw[label];
                 const int4 lb = s.lab[label];
                 TriSample tri;
@@ -882,7 +874,7 @@ __device__ inline void mip_march_single(const RenderArgs& A, const Smem& s, floa
                 skip_shift = cur_bm <= aAcc ? 5 : (cur_bb <= aAcc ? 3 : 0);
             }
             if (!skip_shift) {
-                int label = MASK ? ((TOUCH || cur_label == 0xFF) ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label) : 0;
+                int label = MASK ? ((TOUCH || cur_label == 0xFF) ? A.v.mask[mk_mask_offset(A.v, cx, cy, cz)] : cur_label) : 0;
                 if (TOUCH && MASK) touch_mask(A, cx, cy, cz);
                 const float4 aw = s.aw[label];
                 if (aw.x > 0.f) {
@@ -947,7 +939,7 @@ __device__ inline void mip_march_grouped(const RenderArgs& A, const Smem& s, flo
                         skip_shift = cur_bm <= aAcc ? 5 : (cur_bb <= aAcc ? 3 : 0);
                     }
                     if (!skip_shift) {
-                        lab[j] = MASK ? ((TOUCH || cur_label == 0xFF) ? A.v.mask[mk_voxel_offset(A.v, cx, cy, cz)] : cur_label) : 0;
+                        lab[j] = MASK ? ((TOUCH || cur_label == 0xFF) ? A.v.mask[mk_mask_offset(A.v, cx, cy, cz)] : cur_label) : 0;
                         if (TOUCH && MASK) touch_mask(A, cx, cy, cz);
                         want[j] = true;
                     }

@@ -159,16 +159,21 @@ extern "C" mk_status mk_synchronize(mk_context* ctx) {
 
 // ---- bricking ------------------------------------------------------------
 
-// One workgroup of 512 threads per brick: thread o writes voxel o of brick b.
-// Voxels past the volume edge are zero (the samplers clamp indices, so they
-// are never read).
-template <typename E>
+// One workgroup of 512 threads per brick: thread o writes voxel o of brick b,
+// in the volume's order (mk_inbrick: 4x4x4 blocks) or, for the label mask
+// (ROWS), in x-rows. Voxels past the volume edge are zero (the samplers clamp
+// indices, so they are never read).
+template <typename E, bool ROWS>
 __global__ void __launch_bounds__(512) k_brickify(const E* __restrict__ lin, E* __restrict__ out, int nx, int ny, int nz,
                                                   int nbx, int nby) {
     uint32_t b = blockIdx.x;
     uint32_t o = threadIdx.x;
     int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-    int x = bx * 8 + (o & 7), y = by * 8 + ((o >> 3) & 7), z = bz * 8 + (o >> 6);
+    // inverse of mk_inbrick: offset bits x0 x1 y0 y1 z0 z1 x2 y2 z2
+    const int ix = ROWS ? (int)(o & 7) : (int)((o & 3) | ((o >> 4) & 4));
+    const int iy = ROWS ? (int)((o >> 3) & 7) : (int)(((o >> 2) & 3) | ((o >> 5) & 4));
+    const int iz = ROWS ? (int)(o >> 6) : (int)(((o >> 4) & 3) | ((o >> 6) & 4));
+    int x = bx * 8 + ix, y = by * 8 + iy, z = bz * 8 + iz;
     E v = 0;
     if (x < nx && y < ny && z < nz) v = lin[((size_t)z * ny + y) * nx + x];
     out[((size_t)b << 9) | o] = v;
@@ -325,9 +330,8 @@ __global__ void __launch_bounds__(256) k_origin_slice(MkVolView v, int z, int16_
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint32_t)v.nx * (uint32_t)v.ny) return;
     const int x = (int)(i % (uint32_t)v.nx), y = (int)(i / (uint32_t)v.nx);
-    const uint32_t o = mk_voxel_offset(v, x, y, z);
-    int16_t val = v.vox[o];
-    if (v.mask && v.mask[o] == 0) val = -2048;
+    int16_t val = v.vox[mk_voxel_offset(v, x, y, z)];
+    if (v.mask && v.mask[mk_mask_offset(v, x, y, z)] == 0) val = -2048;
     out[i] = val;
 }
 
@@ -377,7 +381,7 @@ static mk_status mk_volume_from_device(mk_context* ctx, const int16_t* d_lin, in
         MK_HIP(ctx, hipMalloc((void**)&t.occ2, nm * sizeof(uint8_t)));
         MK_HIP(ctx, hipMalloc((void**)&t.mipb2, nm * sizeof(float)));
     }
-    hipLaunchKernelGGL(k_brickify<int16_t>, dim3((uint32_t)nb), dim3(512), 0, ctx->stream, d_lin, ctx->d_vox, nx, ny, nz,
+    hipLaunchKernelGGL((k_brickify<int16_t, false>), dim3((uint32_t)nb), dim3(512), 0, ctx->stream, d_lin, ctx->d_vox, nx, ny, nz,
                        ctx->nbx, ctx->nby);
     MK_HIP(ctx, hipGetLastError());
     MkVolView v = ctx->view();
@@ -436,7 +440,7 @@ static mk_status mk_mask_from_device(mk_context* ctx, const uint8_t* d_lin, int 
     size_t nb = ctx->nbricks();
     uint8_t* d_mask = nullptr;
     MK_HIP(ctx, hipMalloc((void**)&d_mask, nb * 512));
-    hipLaunchKernelGGL(k_brickify<uint8_t>, dim3((uint32_t)nb), dim3(512), 0, ctx->stream, d_lin, d_mask, nx, ny, nz,
+    hipLaunchKernelGGL((k_brickify<uint8_t, true>), dim3((uint32_t)nb), dim3(512), 0, ctx->stream, d_lin, d_mask, nx, ny, nz,
                        ctx->nbx, ctx->nby);
     MK_HIP(ctx, hipGetLastError());
     unsigned long long* d_flags = nullptr;
