@@ -91,70 +91,115 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, ui
 
 __global__ void __launch_bounds__(kMcBlock) k_mc_count(McArgs A, uint32_t* __restrict__ block_sum) {
     __shared__ uint32_t s_wave[4];
+    __shared__ uint8_t s_cnt[256];
+    static_assert(kMcBlock == 256, "one thread stages each case's count");
+    s_cnt[threadIdx.x] = A.cnt[threadIdx.x];
     const uint64_t q = (uint64_t)blockIdx.x * kMcBlock + threadIdx.x;
     const bool valid = q < A.ncubes;
     int i = 0, j = 0, k = 0;
     if (valid) cube_coords(A, q, i, j, k);
     float f[8];
     const int c = cube_case(A, valid, i, j, k, f);
-    const uint32_t n = valid ? A.cnt[c] : 0u;
+    __syncthreads();
+    const uint32_t n = valid ? s_cnt[c] : 0u;
     uint32_t total;
     (void)block_exclusive_scan(n, s_wave, total);
     if (threadIdx.x == 0) block_sum[blockIdx.x] = total;
 }
 
-// Exclusive scan of the block counts by one workgroup; writes the total.
+// Exclusive scan of the block counts by one workgroup of 1024 threads; writes
+// the total. Thread t owns a contiguous run of the counts: it sums its run,
+// the run sums are scanned across the workgroup, and the thread writes its
+// run's offsets from its run's start.
 __global__ void __launch_bounds__(1024) k_mc_scan(const uint32_t* __restrict__ block_sum, uint64_t* __restrict__ block_off,
                                                   uint32_t nblocks, unsigned long long* __restrict__ total) {
-    __shared__ unsigned long long s[1024];
-    unsigned long long carry = 0;
-    for (uint32_t base = 0; base < nblocks; base += 1024) {
-        const uint32_t idx = base + threadIdx.x;
-        unsigned long long v = idx < nblocks ? block_sum[idx] : 0ull;
-        s[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            unsigned long long y = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0ull;
-            __syncthreads();
-            s[threadIdx.x] += y;
-            __syncthreads();
-        }
-        if (idx < nblocks) block_off[idx] = carry + s[threadIdx.x] - v;
-        const unsigned long long chunk = s[1023];
-        __syncthreads();
-        carry += chunk;
+    __shared__ unsigned long long s_wave[16];
+    const uint32_t per = (nblocks + 1023) / 1024;
+    const uint32_t lo = min(nblocks, threadIdx.x * per), hi = min(nblocks, lo + per);
+    unsigned long long run = 0;
+    // unrolled so that a batch of independent loads shares one round trip
+#pragma unroll 16
+    for (uint32_t b = lo; b < hi; ++b) run += block_sum[b];
+    // inclusive scan of the run sums: within each wave, then over the waves
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long x = run;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long y = __shfl_up(x, off);
+        if (lane >= off) x += y;
     }
-    if (threadIdx.x == 0) *total = carry;
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    unsigned long long before = 0;
+    for (int w = 0; w < wave; ++w) before += s_wave[w];
+    unsigned long long acc = before + x - run;
+#pragma unroll 16
+    for (uint32_t b = lo; b < hi; ++b) {
+        block_off[b] = acc;
+        acc += block_sum[b];
+    }
+    if (threadIdx.x == 1023) *total = before + x;
 }
 
+// Edge e of the cube as used by the emit pass: its corner with the smaller
+// coordinate along the edge (lo), the other (hi), the edge's axis, and lo's
+// offsets, packed as lo | hi << 3 | axis << 6 | dx << 8 | dy << 9 | dz << 10.
+__device__ inline uint32_t mc_edge_info(int e) {
+    int a = kMkMcEdge[e][0], b = kMkMcEdge[e][1];
+    const bool swap = kMkMcCorner[a][0] + kMkMcCorner[a][1] + kMkMcCorner[a][2] >
+                      kMkMcCorner[b][0] + kMkMcCorner[b][1] + kMkMcCorner[b][2];
+    const int lo = swap ? b : a, hi = swap ? a : b;
+    const int axis = kMkMcCorner[lo][0] != kMkMcCorner[hi][0] ? 0 : (kMkMcCorner[lo][1] != kMkMcCorner[hi][1] ? 1 : 2);
+    return (uint32_t)lo | (uint32_t)hi << 3 | (uint32_t)axis << 6 | (uint32_t)kMkMcCorner[lo][0] << 8 |
+           (uint32_t)kMkMcCorner[lo][1] << 9 | (uint32_t)kMkMcCorner[lo][2] << 10;
+}
+
+// f[n] for a lane-varying corner index n, by selects (no indexed registers).
+__device__ inline float mc_corner(const float f[8], uint32_t n) {
+    const float a = (n & 1) ? f[1] : f[0], b = (n & 1) ? f[3] : f[2];
+    const float c = (n & 1) ? f[5] : f[4], d = (n & 1) ? f[7] : f[6];
+    const float ab = (n & 2) ? b : a, cd = (n & 2) ? d : c;
+    return (n & 4) ? cd : ab;
+}
+
+// The triangle table (edge indices per case), the triangle counts and the
+// edge table are staged in LDS, so that a vertex costs one LDS byte and one
+// LDS word instead of a chain of dependent global loads.
 __global__ void __launch_bounds__(kMcBlock) k_mc_emit(McArgs A, const uint64_t* __restrict__ block_off,
                                                       float* __restrict__ out, uint64_t capacity) {
     __shared__ uint32_t s_wave[4];
+    __shared__ uint4 s_tri[256];      // [case] 16 edge indices
+    __shared__ uint8_t s_cnt[256];
+    __shared__ uint32_t s_edge[12];
+    static_assert(kMcBlock == 256, "one thread stages each case's row and count");
+    s_tri[threadIdx.x] = ((const uint4*)A.tri)[threadIdx.x];
+    s_cnt[threadIdx.x] = A.cnt[threadIdx.x];
+    if (threadIdx.x < 12) s_edge[threadIdx.x] = mc_edge_info((int)threadIdx.x);
     const uint64_t q = (uint64_t)blockIdx.x * kMcBlock + threadIdx.x;
     const bool valid = q < A.ncubes;
     float f[8];
     int i = 0, j = 0, k = 0;
     if (valid) cube_coords(A, q, i, j, k);
     const int c = cube_case(A, valid, i, j, k, f);
-    const uint32_t n = valid ? A.cnt[c] : 0u;
+    __syncthreads();
+    const uint32_t n = valid ? s_cnt[c] : 0u;
     uint32_t total;
     const uint32_t rank = block_exclusive_scan(n, s_wave, total);
     uint64_t t = block_off[blockIdx.x] + rank;
-    const int8_t* row = A.tri + 16 * c;
+    const int8_t* row = (const int8_t*)&s_tri[c];
     for (uint32_t m = 0; m < n; ++m, ++t) {
         if (t >= capacity) break;
         float* o = out + 9 * t;
         for (int vtx = 0; vtx < 3; ++vtx) {
-            const int e = row[3 * m + vtx];
-            int a = kMkMcEdge[e][0], b = kMkMcEdge[e][1];
-            // order the corners along the edge axis
-            const bool swap = kMkMcCorner[a][0] + kMkMcCorner[a][1] + kMkMcCorner[a][2] >
-                              kMkMcCorner[b][0] + kMkMcCorner[b][1] + kMkMcCorner[b][2];
-            const int lo = swap ? b : a, hi = swap ? a : b;
-            const float tt = (A.iso - f[lo]) / (f[hi] - f[lo]);
-            float p[3] = {(float)(i + kMkMcCorner[lo][0]), (float)(j + kMkMcCorner[lo][1]), (float)(k + kMkMcCorner[lo][2])};
-            const int axis = kMkMcCorner[lo][0] != kMkMcCorner[hi][0] ? 0 : (kMkMcCorner[lo][1] != kMkMcCorner[hi][1] ? 1 : 2);
-            p[axis] = p[axis] + tt;
+            const uint32_t info = s_edge[row[3 * m + vtx]];
+            const float flo = mc_corner(f, info & 7u), fhi = mc_corner(f, (info >> 3) & 7u);
+            const float tt = (A.iso - flo) / (fhi - flo);
+            float p[3] = {(float)(i + (int)((info >> 8) & 1u)), (float)(j + (int)((info >> 9) & 1u)),
+                          (float)(k + (int)((info >> 10) & 1u))};
+            const uint32_t axis = (info >> 6) & 3u;
+            p[0] = axis == 0 ? p[0] + tt : p[0];
+            p[1] = axis == 1 ? p[1] + tt : p[1];
+            p[2] = axis == 2 ? p[2] + tt : p[2];
             o[3 * vtx + 0] = p[0];
             o[3 * vtx + 1] = p[1];
             o[3 * vtx + 2] = p[2];
