@@ -289,6 +289,19 @@ __global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __
             double Cp[3] = {0.0, 0.0, 0.0};
             T = 1.0;
             double gs = 0.0;
+            // LUT gradients of consecutive samples that interpolate the same two
+            // entries are summed in registers and added to LDS when the pair changes
+            int p0 = -1, p1 = -1;
+            double q0[4] = {0.0, 0.0, 0.0, 0.0}, q1[4] = {0.0, 0.0, 0.0, 0.0};
+            auto flush = [&]() {
+                if (p0 < 0) return;
+                for (int ch = 0; ch < 4; ++ch) {
+                    if (q0[ch] != 0.0) atomicAdd(&s_g[4 * p0 + ch], q0[ch]);
+                    if (q1[ch] != 0.0) atomicAdd(&s_g[4 * p1 + ch], q1[ch]);
+                    q0[ch] = 0.0;
+                    q1[ch] = 0.0;
+                }
+            };
             fit_march(A, s_lut, u, v, acc0, a_out, L, [&](const Sample& S, float kf) {
                 // samples with zero alpha still move the image when their LUT
                 // entries grow: da > 0 from 0 is a smooth onset
@@ -312,13 +325,18 @@ __global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __
                 gs += dw * (double)S.c.w;
                 const double f = S.f;
                 const double d4[4] = {drgb[0], drgb[1], drgb[2], dcw};
+                if (S.i0 != p0 || S.i1 != p1) {
+                    flush();
+                    p0 = S.i0;
+                    p1 = S.i1;
+                }
                 for (int ch = 0; ch < 4; ++ch) {
-                    if (d4[ch] == 0.0) continue;
-                    atomicAdd(&s_g[4 * S.i0 + ch], (1.0 - f) * d4[ch]);
-                    atomicAdd(&s_g[4 * S.i1 + ch], f * d4[ch]);
+                    q0[ch] += (1.0 - f) * d4[ch];
+                    q1[ch] += f * d4[ch];
                 }
                 return T != 0.0;
             });
+            flush();
             atomicAdd(&s_g[4 * A.n], gs);
         }
     }
