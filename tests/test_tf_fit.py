@@ -164,6 +164,46 @@ def test_tf_backward_from_kept_colours_equals_remarch(gpu_ctx):
 
 
 @pytest.mark.gpu
+def test_tf_ragged_image_forward_and_adjoint(gpu_ctx):
+    """A 75 x 53 view (tiles cut at both edges, width and height unequal) with
+    a pan and zoom: the forward against FP64, and the adjoint from kept
+    colours equal to the re-marching adjoint, with every gradient finite."""
+    import torch
+
+    vol = W.make_volume(SPEC)
+    gpu_ctx.set_volume(vol)
+    info = P.RenderInfo()
+    info.set_vr_size(75, 53)
+    info.rotate(-20.0, 50.0)
+    info.zoom(1.3)
+    info.pan(4.0, -3.0)
+    p = P.render_params(SPEC.dims, SPEC.spacing, info, {0: P.LabelState(1.0, 2200.0, 0.0)}, width=75, height=53)
+    lut = _lut(6)
+    dl = torch.tensor(lut, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    rgb = torch.empty((53, 75, 3), dtype=torch.float32, device="cuda")
+    rgb64 = torch.empty((53, 75, 3), dtype=torch.float64, device="cuda")
+    gpu_ctx.tf_forward(p, dl, N_LUT, 1.3, rgb, stream=stream, dev_rgb64=rgb64)
+    g = torch.randn((53, 75, 3), generator=torch.Generator(device="cuda").manual_seed(9), device="cuda")
+    grads = []
+    for kept in (rgb64, None):
+        g_lut = torch.zeros(N_LUT * 4, dtype=torch.float64, device="cuda")
+        g_s = torch.zeros(1, dtype=torch.float64, device="cuda")
+        gpu_ctx.tf_backward(p, dl, N_LUT, 1.3, g, g_lut, g_s, stream=stream, dev_rgb64=kept)
+        grads.append((g_lut, g_s))
+    torch.cuda.synchronize()
+    ref = O.fit_render(p, vol, lut.astype(np.float64), 1.3)
+    err = np.abs(rgb.cpu().numpy().astype(np.float64) - ref)
+    assert ref.max() > 0.0
+    assert err.max() < 2e-4 * max(1.0, ref.max()), err.max()
+    (a, sa), (b, sb) = grads
+    assert bool(torch.isfinite(a).all()) and bool(torch.isfinite(sa).all())
+    assert float(b.abs().max()) > 0.0
+    assert float((a - b).abs().max()) <= 1e-10 * float(b.abs().max())
+    assert abs(float(sa) - float(sb)) <= 1e-10 * abs(float(sb))
+
+
+@pytest.mark.gpu
 def test_short_fit_reduces_loss(gpu_ctx):
     import torch
 
