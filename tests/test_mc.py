@@ -99,7 +99,9 @@ def test_gpu_marching_cubes_bit_exact(gpu_ctx, iso):
 
     vol = W.make_volume(W.SMALL)
     gpu_ctx.set_volume(vol)
-    for box in [(0, 0, 0, 128, 128, 128), (5, 17, 3, 90, 101, 77)]:
+    # rows of 127, 84, 1 and 30 cubes: a wave holds parts of several rows, and
+    # the last box ends at the volume's far x and z faces
+    for box in [(0, 0, 0, 128, 128, 128), (5, 17, 3, 90, 101, 77), (60, 0, 0, 62, 128, 128), (97, 30, 64, 128, 70, 128)]:
         got = gpu_ctx.marching_cubes(box, iso)
         ref = O.marching_cubes(vol, box, iso)
         assert got.shape == ref.shape and ref.shape[0] > 0
