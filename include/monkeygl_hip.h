@@ -20,7 +20,13 @@
  * calls run on the context's own stream. Every call that takes a `stream`
  * argument enqueues its kernels on that hipStream_t (NULL = HIP's default
  * stream, as everywhere in HIP) after the context's pending preparation work
- * (skip tables, uploads), and returns without waiting. Different contexts are
+ * (skip tables, uploads), and returns without waiting. A later call that
+ * rewrites or frees context data which earlier work on a caller's stream
+ * still reads (LUTs, per-render-type skip tables, the step lattice, the
+ * volume, the mask, the orientation figure) waits for that work on the device
+ * first, so frames may be enqueued back to back with different palettes,
+ * LUTs or step sizes. Marching-cubes calls of one context share its scratch
+ * and run one after another on the device. Different contexts are
  * independent.
  */
 #ifndef MONKEYGL_HIP_H

@@ -103,11 +103,29 @@ struct mk_context {
     float steps_l1 = -1.0f;
     int steps_n = 0;
 
-    // marching cubes: expanded triangle table, per-block counts/offsets, total
-    uint8_t* d_mc_table = nullptr;
+    // marching cubes: per-block counts/offsets and the total (the triangle
+    // table is per device, set up at context creation); calls on different
+    // streams share this scratch, so a call waits on the device for the last
+    // one (mc_done)
     void* d_mc_scratch = nullptr;
     size_t mc_scratch_cap = 0;
     void* d_mc_total = nullptr;
+    hipEvent_t mc_done = nullptr;
+    hipStream_t mc_stream = nullptr;
+    bool mc_used = false;
+
+    // Work enqueued on streams other than the context's that reads the
+    // context's device data (volume, mask, statistics, LUTs, skip tables, step
+    // lattice, orientation figure): one event per stream, recorded after each
+    // such launch. Before the context rewrites or frees any of that data, its
+    // own stream waits for every one of them (mk_wait_readers).
+    struct Reader {
+        hipStream_t stream;
+        hipEvent_t done;
+    };
+    std::vector<Reader> readers;
+    // orders a caller's stream after the context stream (mk_order_after_context)
+    hipEvent_t order_ev = nullptr;
 
     // orientation figure of the VR overlay: normalised facets (9 floats each)
     float* d_orient = nullptr;
@@ -138,11 +156,20 @@ mk_status mk_upload_luts(mk_context* ctx);
 // Makes stream s wait for the work queued so far on the context's own stream
 // (skip tables, lattice and table uploads); no-op when s is that stream.
 mk_status mk_order_after_context(mk_context* ctx, hipStream_t s);
+// Records that work just enqueued on stream s reads the context's device
+// data; no-op when s is the context's stream (its own work is in order).
+mk_status mk_note_reader(mk_context* ctx, hipStream_t s);
+// Makes the context's stream wait for all work recorded by mk_note_reader, so
+// that what it enqueues next may rewrite that data; a synchronize of the
+// context stream after this call also waits for those readers.
+mk_status mk_wait_readers(mk_context* ctx);
 // builds (on the context stream) the skip tables of render_type for the given
 // palette unless the cached ones match; the LUTs must be uploaded
 mk_status mk_ensure_skip_tables(mk_context* ctx, int render_type, const mk_alpha_wwwl* aw);
 // uploads the float lattice a_{k+1} = a_k + l1 (a_0 = 0) up to past 1.732
 mk_status mk_ensure_steps(mk_context* ctx, float l1);
+// once per device, with the device current: the marching-cubes triangle table
+mk_status mk_mc_device_init(int device);
 
 #define MK_HIP(ctx, call)                                          \
     do {                                                           \

@@ -400,7 +400,7 @@ extern "C" mk_status mk_tf_forward(mk_context* ctx, const mk_render_params* p, c
     const uint32_t tiles = (uint32_t)(((p->width + 15) / 16) * ((p->height + 15) / 16));
     hipLaunchKernelGGL(k_fit_forward, dim3(tiles), dim3(256), (size_t)n * sizeof(float4), s, A, dev_rgb, dev_rgb64);
     MK_HIP(ctx, hipGetLastError());
-    return MK_OK;
+    return mk_note_reader(ctx, s);
 }
 
 extern "C" mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, const float* dev_lut, int n,
@@ -421,5 +421,5 @@ extern "C" mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, 
     hipLaunchKernelGGL(k_fit_backward, dim3(tiles), dim3(256), lds, s, A, dev_grad_rgb, dev_rgb64, dev_grad_lut,
                        dev_grad_scale);
     MK_HIP(ctx, hipGetLastError());
-    return MK_OK;
+    return mk_note_reader(ctx, s);
 }

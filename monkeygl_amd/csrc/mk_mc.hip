@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 
 #include "mk_context.h"
 #include "mk_mc_table.h"
@@ -24,6 +25,18 @@
 namespace {
 
 constexpr int kMcBlock = 256;
+
+// The expanded triangle table, one copy per device shared by every context
+// ([256][16] edge indices, then [256] counts), filled once per device by
+// mk_mc_device_init (called from mk_context_create).
+constexpr size_t kMcTableBytes = 256 * 16 + 256;
+constexpr int kMcMaxDevices = 64;
+
+__device__ __align__(256) uint8_t mc_table[kMcTableBytes];
+
+// Device address of the table per device (from hipGetSymbolAddress).
+std::mutex g_mc_mu;
+uint8_t* g_mc_table_ptr[kMcMaxDevices] = {};
 
 struct McArgs {
     MkVolView v;
@@ -112,7 +125,8 @@ __global__ void __launch_bounds__(kMcBlock) k_mc_count(McArgs A, uint32_t* __res
 // the run sums are scanned across the workgroup, and the thread writes its
 // run's offsets from its run's start.
 __global__ void __launch_bounds__(1024) k_mc_scan(const uint32_t* __restrict__ block_sum, uint64_t* __restrict__ block_off,
-                                                  uint32_t nblocks, unsigned long long* __restrict__ total) {
+                                                  uint32_t nblocks, unsigned long long* __restrict__ total,
+                                                  unsigned long long* __restrict__ total2) {
     __shared__ unsigned long long s_wave[16];
     const uint32_t per = (nblocks + 1023) / 1024;
     const uint32_t lo = min(nblocks, threadIdx.x * per), hi = min(nblocks, lo + per);
@@ -138,7 +152,10 @@ __global__ void __launch_bounds__(1024) k_mc_scan(const uint32_t* __restrict__ b
         block_off[b] = acc;
         acc += block_sum[b];
     }
-    if (threadIdx.x == 1023) *total = before + x;
+    if (threadIdx.x == 1023) {
+        *total = before + x;
+        if (total2) *total2 = before + x;
+    }
 }
 
 // Edge e of the cube as used by the emit pass: its corner with the smaller
@@ -207,20 +224,6 @@ __global__ void __launch_bounds__(kMcBlock) k_mc_emit(McArgs A, const uint64_t* 
     }
 }
 
-mk_status ensure_table(mk_context* ctx) {
-    if (ctx->d_mc_table) return MK_OK;
-    int8_t tri[256][16];
-    uint8_t cnt[256];
-    mk_mc_expand(tri, cnt);
-    std::vector<uint8_t> buf(256 * 16 + 256);
-    memcpy(buf.data(), tri, 256 * 16);
-    memcpy(buf.data() + 256 * 16, cnt, 256);
-    MK_HIP(ctx, hipMalloc((void**)&ctx->d_mc_table, buf.size()));
-    MK_HIP(ctx, hipMemcpyAsync(ctx->d_mc_table, buf.data(), buf.size(), hipMemcpyHostToDevice, ctx->stream));
-    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return MK_OK;
-}
-
 mk_status prepare(mk_context* ctx, const int* box, float iso, McArgs& A, uint32_t& nblocks) {
     if (!box) return mk_fail(ctx, MK_ERR_ARG, "null box");
     if (!ctx->d_vox) return mk_fail(ctx, MK_ERR_STATE, "marching cubes before any volume was uploaded");
@@ -230,41 +233,79 @@ mk_status prepare(mk_context* ctx, const int* box, float iso, McArgs& A, uint32_
         z1 - z0 < 2)
         return mk_fail(ctx, MK_ERR_ARG, "bad box [%d,%d,%d)-[%d,%d,%d) for volume %dx%dx%d", x0, y0, z0, x1, y1, z1,
                        ctx->nx, ctx->ny, ctx->nz);
-    mk_status st = ensure_table(ctx);
-    if (st) return st;
+    uint8_t* table = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mc_mu);
+        if (ctx->device >= 0 && ctx->device < kMcMaxDevices) table = g_mc_table_ptr[ctx->device];
+    }
+    if (!table) return mk_fail(ctx, MK_ERR_STATE, "marching-cubes table not set up on device %d", ctx->device);
     A.v = ctx->view();
     A.x0 = x0; A.y0 = y0; A.z0 = z0;
     A.cx = x1 - x0 - 1; A.cy = y1 - y0 - 1; A.cz = z1 - z0 - 1;
     A.ncubes = (uint64_t)A.cx * A.cy * A.cz;
     A.iso = iso;
-    A.tri = (const int8_t*)ctx->d_mc_table;
-    A.cnt = ctx->d_mc_table + 256 * 16;
+    A.tri = (const int8_t*)table;
+    A.cnt = table + 256 * 16;
     const uint64_t nb = (A.ncubes + kMcBlock - 1) / kMcBlock;
     if (nb > 0x7FFFFFFFull) return mk_fail(ctx, MK_ERR_ARG, "box too large");
     nblocks = (uint32_t)nb;
     return MK_OK;
 }
 
-mk_status launch_count(mk_context* ctx, const McArgs& A, uint32_t nblocks, hipStream_t s) {
+// The context's scratch: per-block counts, then 64-byte aligned offsets.
+uint64_t* scratch_offsets(void* base, uint32_t nblocks) {
+    return (uint64_t*)(((uintptr_t)((uint32_t*)base + nblocks) + 63) & ~(uintptr_t)63);
+}
+
+// Calls of one context share its scratch: s waits on the device for the last
+// call's kernels when that call ran on another stream; the scratch grows here.
+mk_status acquire_scratch(mk_context* ctx, uint32_t nblocks, hipStream_t s) {
+    if (ctx->mc_used && ctx->mc_stream != s) MK_HIP(ctx, hipStreamWaitEvent(s, ctx->mc_done, 0));
     mk_status st = mk_ensure_buffer(ctx, &ctx->d_mc_scratch, &ctx->mc_scratch_cap,
                                     (size_t)nblocks * (sizeof(uint32_t) + sizeof(uint64_t)) + 64);
     if (st) return st;
-    uint32_t* sums = (uint32_t*)ctx->d_mc_scratch;
-    uint64_t* offs = (uint64_t*)(((uintptr_t)(sums + nblocks) + 63) & ~(uintptr_t)63);
-    unsigned long long* total = (unsigned long long*)ctx->d_mc_total;
-    hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(kMcBlock), 0, s, A, sums);
-    hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, s, sums, offs, nblocks, total);
-    MK_HIP(ctx, hipGetLastError());
+    if (!ctx->d_mc_total) MK_HIP(ctx, hipMalloc((void**)&ctx->d_mc_total, 64));
+    return MK_OK;
+}
+
+mk_status release_scratch(mk_context* ctx, hipStream_t s) {
+    MK_HIP(ctx, hipEventRecord(ctx->mc_done, s));
+    ctx->mc_stream = s;
+    ctx->mc_used = true;
     return MK_OK;
 }
 
 }  // namespace
 
+// Sets up, once per device (with that device current), the device address of
+// the triangle table, and fills it.
+mk_status mk_mc_device_init(int device) {
+    if (device < 0 || device >= kMcMaxDevices)
+        return mk_fail(nullptr, MK_ERR_ARG, "marching cubes: device %d beyond %d", device, kMcMaxDevices);
+    std::lock_guard<std::mutex> lk(g_mc_mu);
+    if (g_mc_table_ptr[device]) return MK_OK;
+    void* table = nullptr;
+    hipError_t e = hipGetSymbolAddress(&table, HIP_SYMBOL(mc_table));
+    if (e != hipSuccess || !table)
+        return mk_fail(nullptr, MK_ERR_HIP, "hipGetSymbolAddress (marching cubes, device %d): %s", device,
+                       hipGetErrorString(e));
+    int8_t tri[256][16];
+    uint8_t cnt[256];
+    mk_mc_expand(tri, cnt);
+    uint8_t buf[kMcTableBytes];
+    memcpy(buf, tri, 256 * 16);
+    memcpy(buf + 256 * 16, cnt, 256);
+    e = hipMemcpy(table, buf, sizeof(buf), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+        return mk_fail(nullptr, MK_ERR_HIP, "marching-cubes table upload (device %d): %s", device, hipGetErrorString(e));
+    g_mc_table_ptr[device] = (uint8_t*)table;
+    return MK_OK;
+}
+
 extern "C" mk_status mk_marching_cubes_async(mk_context* ctx, const int* box, float iso, mk_triangle* dev_out,
                                              uint64_t capacity, uint64_t* dev_count, void* stream) {
     if (!ctx) return MK_ERR_ARG;
     MK_HIP(ctx, hipSetDevice(ctx->device));
-    if (!ctx->d_mc_total) MK_HIP(ctx, hipMalloc((void**)&ctx->d_mc_total, 64));
     McArgs A;
     uint32_t nblocks = 0;
     mk_status st = prepare(ctx, box, iso, A, nblocks);
@@ -273,15 +314,19 @@ extern "C" mk_status mk_marching_cubes_async(mk_context* ctx, const int* box, fl
     hipStream_t s = (hipStream_t)stream;
     st = mk_order_after_context(ctx, s);
     if (st) return st;
-    st = launch_count(ctx, A, nblocks, s);
+    st = acquire_scratch(ctx, nblocks, s);
     if (st) return st;
     uint32_t* sums = (uint32_t*)ctx->d_mc_scratch;
-    uint64_t* offs = (uint64_t*)(((uintptr_t)(sums + nblocks) + 63) & ~(uintptr_t)63);
+    uint64_t* offs = scratch_offsets(ctx->d_mc_scratch, nblocks);
+    hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(kMcBlock), 0, s, A, sums);
+    hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, s, sums, offs, nblocks,
+                       (unsigned long long*)ctx->d_mc_total, (unsigned long long*)dev_count);
     if (capacity > 0)
         hipLaunchKernelGGL(k_mc_emit, dim3(nblocks), dim3(kMcBlock), 0, s, A, offs, (float*)dev_out, capacity);
-    if (dev_count) MK_HIP(ctx, hipMemcpyAsync(dev_count, ctx->d_mc_total, sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
     MK_HIP(ctx, hipGetLastError());
-    return MK_OK;
+    st = release_scratch(ctx, s);
+    if (st) return st;
+    return mk_note_reader(ctx, s);
 }
 
 extern "C" mk_status mk_marching_cubes(mk_context* ctx, const int* box, float iso, mk_triangle* host_out,
@@ -289,28 +334,31 @@ extern "C" mk_status mk_marching_cubes(mk_context* ctx, const int* box, float is
     if (!ctx) return MK_ERR_ARG;
     if (!n_triangles) return mk_fail(ctx, MK_ERR_ARG, "null count");
     MK_HIP(ctx, hipSetDevice(ctx->device));
-    if (!ctx->d_mc_total) MK_HIP(ctx, hipMalloc((void**)&ctx->d_mc_total, 64));
     McArgs A;
     uint32_t nblocks = 0;
     mk_status st = prepare(ctx, box, iso, A, nblocks);
     if (st) return st;
-    st = launch_count(ctx, A, nblocks, ctx->stream);
+    st = acquire_scratch(ctx, nblocks, ctx->stream);
     if (st) return st;
+    uint32_t* sums = (uint32_t*)ctx->d_mc_scratch;
+    uint64_t* offs = scratch_offsets(ctx->d_mc_scratch, nblocks);
+    hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(kMcBlock), 0, ctx->stream, A, sums);
+    hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, ctx->stream, sums, offs, nblocks,
+                       (unsigned long long*)ctx->d_mc_total, (unsigned long long*)nullptr);
+    MK_HIP(ctx, hipGetLastError());
     uint64_t total = 0;
     MK_HIP(ctx, hipMemcpyAsync(&total, ctx->d_mc_total, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
     MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *n_triangles = total;
-    if (!host_out || capacity == 0 || total == 0) return MK_OK;
+    if (!host_out || capacity == 0 || total == 0) return release_scratch(ctx, ctx->stream);
     const uint64_t n = total < capacity ? total : capacity;
     void* d_out = nullptr;
     MK_HIP(ctx, hipMalloc(&d_out, n * sizeof(mk_triangle)));
-    uint32_t* sums = (uint32_t*)ctx->d_mc_scratch;
-    uint64_t* offs = (uint64_t*)(((uintptr_t)(sums + nblocks) + 63) & ~(uintptr_t)63);
     hipLaunchKernelGGL(k_mc_emit, dim3(nblocks), dim3(kMcBlock), 0, ctx->stream, A, offs, (float*)d_out, n);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(host_out, d_out, n * sizeof(mk_triangle), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d_out);
     if (e != hipSuccess) return mk_hip_fail(ctx, e, "marching cubes emit");
-    return MK_OK;
+    return release_scratch(ctx, ctx->stream);
 }

@@ -315,6 +315,9 @@ extern "C" mk_status mk_set_orientation_mesh(mk_context* ctx, const uint8_t* stl
     if (n_facets < 0 || (n_facets > 0 && !stl_facets))
         return mk_fail(ctx, MK_ERR_ARG, "mk_set_orientation_mesh: bad facet list");
     MK_HIP(ctx, hipSetDevice(ctx->device));
+    // merges on callers' streams may still read the current figure
+    mk_status st = mk_wait_readers(ctx);
+    if (st) return st;
     MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->d_orient) (void)hipFree(ctx->d_orient);
     ctx->d_orient = nullptr;
@@ -387,5 +390,5 @@ extern "C" mk_status mk_merge_orientation_box_async(mk_context* ctx, const float
     hipLaunchKernelGGL(k_orient_compose, dim3(kBox * kBox / 256), dim3(256), 0, s, ws.keys, ws.diffuse, dev_rgb,
                        width, height);
     MK_HIP(ctx, hipGetLastError());
-    return MK_OK;
+    return mk_note_reader(ctx, s);
 }

@@ -99,7 +99,7 @@ extern "C" mk_status mk_render_plane_async(mk_context* ctx, const mk_plane_param
             break;
     }
     MK_HIP(ctx, hipGetLastError());
-    return MK_OK;
+    return mk_note_reader(ctx, s);
 }
 
 extern "C" mk_status mk_render_plane(mk_context* ctx, const mk_plane_params* p, int16_t* host_out) {
@@ -146,7 +146,7 @@ extern "C" mk_status mk_render_cpr_async(mk_context* ctx, int width, int height,
     else
         hipLaunchKernelGGL(k_cpr<false>, grid, block, 0, s, C);
     MK_HIP(ctx, hipGetLastError());
-    return MK_OK;
+    return mk_note_reader(ctx, s);
 }
 
 extern "C" mk_status mk_render_cpr(mk_context* ctx, int width, int height, const double* points, const double* dirs,

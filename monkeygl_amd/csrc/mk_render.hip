@@ -309,7 +309,10 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
         C.cap = (int)npix;
         C.budget = budget;
         C.long_steps = kLongSteps;
-        C.tail_waves = (uint32_t)((size_t)tiles * (size_t)(tw * tw / 64) * kTailPercent / 100);
+        // waves are 8x8 pixel blocks; the ones holding no pixel of the image
+        // return at once and never count themselves
+        const size_t live_waves = (size_t)((p->width + 7) / 8) * (size_t)((p->height + 7) / 8);
+        C.tail_waves = (uint32_t)(live_waves * kTailPercent / 100);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, C.counters, 4);
     }
     dim3 grid(tiles), block(tw * tw);
@@ -344,7 +347,7 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
 #undef MK_VR
 #undef MK_LAUNCH
     MK_HIP(ctx, hipGetLastError());
-    return MK_OK;
+    return mk_note_reader(ctx, s);
 }
 
 }  // namespace
@@ -354,6 +357,9 @@ mk_status mk_ensure_skip_tables(mk_context* ctx, int render_type, const mk_alpha
     if (sk.valid && sk.lut_version == ctx->lut_version && sk.vol_version == ctx->vol_version &&
         memcmp(sk.aw, aw, sizeof(sk.aw)) == 0)
         return MK_OK;
+    // frames on callers' streams may still read this render type's tables
+    mk_status st = mk_wait_readers(ctx);
+    if (st) return st;
     AwTable T;
     for (int l = 0; l < MK_NUM_LABELS; ++l) {
         T.alpha[l] = aw[l].alpha;
@@ -379,6 +385,9 @@ mk_status mk_ensure_skip_tables(mk_context* ctx, int render_type, const mk_alpha
 
 mk_status mk_ensure_steps(mk_context* ctx, float l1) {
     if (ctx->steps_l1 == l1 && ctx->d_steps) return MK_OK;
+    // frames on callers' streams may still read the current lattice
+    mk_status st = mk_wait_readers(ctx);
+    if (st) return st;
     std::vector<float> a;
     a.reserve((size_t)(1.75 / l1) + 8);
     float acc = 0.0f;
@@ -387,7 +396,7 @@ mk_status mk_ensure_steps(mk_context* ctx, float l1) {
         acc += l1;
         a.push_back(acc);
     }
-    mk_status st = mk_ensure_buffer(ctx, (void**)&ctx->d_steps, &ctx->steps_cap, a.size() * sizeof(float));
+    st = mk_ensure_buffer(ctx, (void**)&ctx->d_steps, &ctx->steps_cap, a.size() * sizeof(float));
     if (st) return st;
     MK_HIP(ctx, hipMemcpyAsync(ctx->d_steps, a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     MK_HIP(ctx, hipStreamSynchronize(ctx->stream));

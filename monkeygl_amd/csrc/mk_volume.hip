@@ -38,6 +38,9 @@ mk_status mk_hip_fail(mk_context* ctx, hipError_t e, const char* what) {
 mk_status mk_ensure_buffer(mk_context* ctx, void** buf, size_t* cap, size_t bytes) {
     if (*cap >= bytes && *buf) return MK_OK;
     if (*buf) {
+        // work on callers' streams may still read the old buffer
+        mk_status st = mk_wait_readers(ctx);
+        if (st) return st;
         MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
         MK_HIP(ctx, hipFree(*buf));
         *buf = nullptr;
@@ -85,9 +88,17 @@ extern "C" mk_status mk_context_create(int device, mk_context** out) {
     mk_context* ctx = new mk_context();
     ctx->device = device;
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->order_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->mc_done, hipEventDisableTiming);
     if (e != hipSuccess) {
-        delete ctx;
-        return mk_fail(nullptr, MK_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+        mk_context_destroy(ctx);
+        return mk_fail(nullptr, MK_ERR_HIP, "context stream/events: %s", hipGetErrorString(e));
+    }
+    // per-device set-up: the marching-cubes triangle table
+    mk_status st = mk_mc_device_init(device);
+    if (st) {
+        mk_context_destroy(ctx);
+        return st;
     }
     *out = ctx;
     return MK_OK;
@@ -114,14 +125,20 @@ static void mk_free_volume(mk_context* ctx) {
 extern "C" void mk_context_destroy(mk_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    for (auto& r : ctx->readers) {
+        (void)hipEventSynchronize(r.done);
+        (void)hipEventDestroy(r.done);
+    }
+    ctx->readers.clear();
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     mk_free_volume(ctx);
     mk_free(ctx->d_lut);
     mk_free(ctx->d_lutnz);
     mk_free(ctx->d_cont);
     if (ctx->cont_done) (void)hipEventDestroy(ctx->cont_done);
+    if (ctx->mc_done) (void)hipEventDestroy(ctx->mc_done);
+    if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
     mk_free(ctx->d_steps);
-    mk_free(ctx->d_mc_table);
     if (ctx->d_mc_scratch) (void)hipFree(ctx->d_mc_scratch);
     if (ctx->d_mc_total) (void)hipFree(ctx->d_mc_total);
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
@@ -141,12 +158,44 @@ extern "C" void* mk_context_stream(mk_context* ctx) { return ctx ? (void*)ctx->s
 
 mk_status mk_order_after_context(mk_context* ctx, hipStream_t s) {
     if (s == ctx->stream) return MK_OK;
-    hipEvent_t ev;
-    MK_HIP(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    hipError_t e = hipEventRecord(ev, ctx->stream);
-    if (e == hipSuccess) e = hipStreamWaitEvent(s, ev, 0);
-    (void)hipEventDestroy(ev);
+    // the wait takes the event's state at this call, so one event serves all
+    hipError_t e = hipEventRecord(ctx->order_ev, ctx->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->order_ev, 0);
     if (e != hipSuccess) return mk_hip_fail(ctx, e, "ordering after the context stream");
+    return MK_OK;
+}
+
+// Streams tracked per context before the list is drained (synchronized and
+// emptied): callers normally use a handful.
+static constexpr size_t kMaxReaders = 64;
+
+mk_status mk_note_reader(mk_context* ctx, hipStream_t s) {
+    if (s == ctx->stream) return MK_OK;
+    for (auto& r : ctx->readers)
+        if (r.stream == s) {
+            MK_HIP(ctx, hipEventRecord(r.done, s));
+            return MK_OK;
+        }
+    if (ctx->readers.size() >= kMaxReaders) {
+        for (auto& r : ctx->readers) {
+            MK_HIP(ctx, hipEventSynchronize(r.done));
+            (void)hipEventDestroy(r.done);
+        }
+        ctx->readers.clear();
+    }
+    mk_context::Reader r{s, nullptr};
+    MK_HIP(ctx, hipEventCreateWithFlags(&r.done, hipEventDisableTiming));
+    hipError_t e = hipEventRecord(r.done, s);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(r.done);
+        return mk_hip_fail(ctx, e, "recording a reader of the context");
+    }
+    ctx->readers.push_back(r);
+    return MK_OK;
+}
+
+mk_status mk_wait_readers(mk_context* ctx) {
+    for (auto& r : ctx->readers) MK_HIP(ctx, hipStreamWaitEvent(ctx->stream, r.done, 0));
     return MK_OK;
 }
 
@@ -364,6 +413,10 @@ static mk_status mk_check_dims(mk_context* ctx, int nx, int ny, int nz) {
 }
 
 static mk_status mk_volume_from_device(mk_context* ctx, const int16_t* d_lin, int nx, int ny, int nz) {
+    // renders on callers' streams may still read the old volume
+    mk_status st = mk_wait_readers(ctx);
+    if (st) return st;
+    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     mk_free_volume(ctx);
     ctx->nx = nx; ctx->ny = ny; ctx->nz = nz;
     ctx->nbx = (nx + 7) / 8; ctx->nby = (ny + 7) / 8; ctx->nbz = (nz + 7) / 8;
@@ -437,6 +490,9 @@ static mk_status mk_mask_from_device(mk_context* ctx, const uint8_t* d_lin, int 
     if (!ctx->d_vox) return mk_fail(ctx, MK_ERR_STATE, "mask upload before any volume");
     if (nx != ctx->nx || ny != ctx->ny || nz != ctx->nz)
         return mk_fail(ctx, MK_ERR_ARG, "mask dims (%d,%d,%d) differ from volume (%d,%d,%d)", nx, ny, nz, ctx->nx, ctx->ny, ctx->nz);
+    // the label statistics are rewritten below, the old mask freed
+    mk_status st = mk_wait_readers(ctx);
+    if (st) return st;
     size_t nb = ctx->nbricks();
     uint8_t* d_mask = nullptr;
     MK_HIP(ctx, hipMalloc((void**)&d_mask, nb * 512));
@@ -515,6 +571,8 @@ extern "C" mk_status mk_clear_mask(mk_context* ctx) {
     if (!ctx) return MK_ERR_ARG;
     MK_HIP(ctx, hipSetDevice(ctx->device));
     if (!ctx->d_mask) return MK_OK;
+    mk_status st = mk_wait_readers(ctx);
+    if (st) return st;
     MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     mk_free(ctx->d_mask);
     size_t nb = ctx->nbricks();
@@ -544,6 +602,9 @@ static mk_status mk_merge_from_device(mk_context* ctx, const uint8_t* d_lin, int
         return mk_fail(ctx, MK_ERR_RANGE, "label %d outside 0..%d", label, MK_MAX_OBJECTS);
     if (mode != MK_MASK_ADD && mode != MK_MASK_UPDATE) return mk_fail(ctx, MK_ERR_ARG, "unknown merge mode %d", mode);
     if (mode == MK_MASK_UPDATE && !ctx->d_mask) return mk_fail(ctx, MK_ERR_STATE, "mask update without a mask");
+    // the mask and its label statistics are rewritten in place
+    mk_status st = mk_wait_readers(ctx);
+    if (st) return st;
     const size_t nb = ctx->nbricks();
     uint8_t* d_mask = ctx->d_mask;
     const int has_old = d_mask != nullptr;
@@ -633,7 +694,7 @@ extern "C" mk_status mk_origin_slice_async(mk_context* ctx, int z, int16_t* dev_
     const uint32_t n = (uint32_t)ctx->nx * (uint32_t)ctx->ny;
     hipLaunchKernelGGL(k_origin_slice, dim3((n + 255) / 256), dim3(256), 0, s, ctx->view(), z, dev_out);
     MK_HIP(ctx, hipGetLastError());
-    return MK_OK;
+    return mk_note_reader(ctx, s);
 }
 
 // ---- transfer functions ---------------------------------------------------
@@ -650,6 +711,9 @@ extern "C" mk_status mk_set_transfer_func(mk_context* ctx, int label, const floa
 
 mk_status mk_upload_luts(mk_context* ctx) {
     if (!ctx->lut_dirty && ctx->d_lut) return MK_OK;
+    // frames on callers' streams may still read the current LUTs
+    mk_status st = mk_wait_readers(ctx);
+    if (st) return st;
     size_t total = 0, total_nz = 0;
     for (int l = 0; l < MK_NUM_LABELS; ++l) {
         int len = (int)(ctx->h_lut[l].size() / 4);
@@ -667,7 +731,7 @@ mk_status mk_upload_luts(mk_context* ctx) {
         p[0] = 0;
         for (int i = 0; i < len; ++i) p[i + 1] = p[i] + (ctx->h_lut[l][4 * (size_t)i + 3] != 0.0f ? 1 : 0);
     }
-    mk_status st = mk_ensure_buffer(ctx, (void**)&ctx->d_lut, &ctx->lut_cap, packed.size() * sizeof(float));
+    st = mk_ensure_buffer(ctx, (void**)&ctx->d_lut, &ctx->lut_cap, packed.size() * sizeof(float));
     if (st) return st;
     st = mk_ensure_buffer(ctx, (void**)&ctx->d_lutnz, &ctx->lutnz_cap, nz.size() * sizeof(int));
     if (st) return st;
