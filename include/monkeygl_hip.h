@@ -256,6 +256,22 @@ mk_status mk_marching_cubes(mk_context* ctx, const int* box, float iso, mk_trian
 mk_status mk_marching_cubes_async(mk_context* ctx, const int* box, float iso, mk_triangle* dev_out,
                                   uint64_t capacity, uint64_t* dev_count, void* stream);
 
+/* Allocation-free variant of mk_marching_cubes_async. No reference
+ * counterpart: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same kernels and the same triangle list and count; the
+ * per-block counts, offsets and total live in a fixed 12 MiB device array per
+ * GPU, set up by mk_context_create, instead of the context's grown scratch.
+ * The box may hold at most 2^28 cubes (2^20 blocks of 256); larger boxes
+ * return MK_ERR_ARG. The call allocates nothing, records no event and does
+ * not order itself after the context's own stream: upload the volume before
+ * the call and do not replace it while the call is in flight. Every call on
+ * one GPU shares that GPU's array (nothing in it needs clearing), so calls
+ * that overlap on one GPU, from any context or stream, race: run them one
+ * after another. */
+mk_status mk_marching_cubes_noalloc(mk_context* ctx, const int* box, float iso, mk_triangle* dev_out,
+                                    uint64_t capacity, uint64_t* dev_count, void* stream);
+
 /* ---- differentiable transfer function (no reference counterpart) -------- */
 /* Fitting rule: the VR compositing of mk_render on the same float step
  * lattice (base step 1/depth * step_scale), with a fixed step (no

@@ -29,7 +29,7 @@ SYMBOLS = [
     "mk_copy_mask_data", "mk_copy_mask_data_device", "mk_clear_mask", "mk_merge_object_mask",
     "mk_merge_object_mask_device", "mk_get_mask_data", "mk_origin_slice_async", "mk_set_transfer_func",
     "mk_render", "mk_render_async", "mk_set_vr_trip_budget", "mk_render_footprint", "mk_render_plane", "mk_render_plane_async",
-    "mk_render_cpr", "mk_render_cpr_async", "mk_marching_cubes", "mk_marching_cubes_async",
+    "mk_render_cpr", "mk_render_cpr_async", "mk_marching_cubes", "mk_marching_cubes_async", "mk_marching_cubes_noalloc",
     "mk_tf_forward", "mk_tf_backward", "mk_set_orientation_mesh", "mk_orientation_workspace_bytes",
     "mk_merge_orientation_box_async", "mk_draw_polyline_async", "mk_png_max_bytes", "mk_png_workspace_bytes", "mk_png_encode_async",
     "mk_base64_max_bytes", "mk_base64_encode_async", "mk_batcher_create", "mk_batcher_destroy", "mk_batcher_last_error",
@@ -130,6 +130,7 @@ def load() -> C.CDLL:
         "mk_render_cpr_async": (st, [vp, i, i, vp, vp, i, i, vp, vp]),
         "mk_marching_cubes": (st, [vp, vp, C.c_float, vp, C.c_uint64, C.POINTER(C.c_uint64)]),
         "mk_marching_cubes_async": (st, [vp, vp, C.c_float, vp, C.c_uint64, vp, vp]),
+        "mk_marching_cubes_noalloc": (st, [vp, vp, C.c_float, vp, C.c_uint64, vp, vp]),
         "mk_tf_forward": (st, [vp, C.POINTER(RenderParams), vp, i, C.c_float, vp, vp, vp]),
         "mk_tf_backward": (st, [vp, C.POINTER(RenderParams), vp, i, C.c_float, vp, vp, vp, vp, vp]),
         "mk_png_max_bytes": (C.c_uint64, [i, i, i]),
@@ -321,6 +322,12 @@ class Context:
         b = np.ascontiguousarray(box, dtype=np.int32)
         self._check(self.lib.mk_marching_cubes_async(self.h, _ptr(b), iso, _ptr(dev_out), capacity, _ptr(dev_count),
                                                      stream))
+
+    def marching_cubes_noalloc(self, box, iso: float, dev_out, capacity: int, dev_count, stream: int = 0) -> None:
+        """mk_marching_cubes_noalloc: as marching_cubes_async, with the GPU's fixed scratch (nothing allocated)."""
+        b = np.ascontiguousarray(box, dtype=np.int32)
+        self._check(self.lib.mk_marching_cubes_noalloc(self.h, _ptr(b), iso, _ptr(dev_out), capacity,
+                                                       _ptr(dev_count), stream))
 
     def tf_forward(self, p: RenderParams, dev_lut, n: int, scale: float, dev_rgb, stream: int = 0,
                    dev_rgb64=None) -> None:

@@ -169,6 +169,7 @@ mk_status mk_ensure_skip_tables(mk_context* ctx, int render_type, const mk_alpha
 // uploads the float lattice a_{k+1} = a_k + l1 (a_0 = 0) up to past 1.732
 mk_status mk_ensure_steps(mk_context* ctx, float l1);
 // once per device, with the device current: the marching-cubes triangle table
+// and the device address of the allocation-free entry point's scratch
 mk_status mk_mc_device_init(int device);
 
 #define MK_HIP(ctx, call)                                          \

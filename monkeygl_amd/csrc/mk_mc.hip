@@ -26,17 +26,28 @@ namespace {
 
 constexpr int kMcBlock = 256;
 
-// The expanded triangle table, one copy per device shared by every context
-// ([256][16] edge indices, then [256] counts), filled once per device by
-// mk_mc_device_init (called from mk_context_create).
+// Device-resident state shared by every context of a device, set up once per
+// device by mk_mc_device_init (called from mk_context_create):
+//   * the expanded triangle table: [256][16] edge indices, then [256] counts;
+//   * `scratch`, the workspace of mk_marching_cubes_noalloc: per-block
+//     triangle counts (uint32), per-block offsets (uint64, 256-byte aligned),
+//     then the total (uint64). Its size bounds that entry point's box to
+//     kMcNoAllocMaxBlocks blocks of kMcBlock cubes.
 constexpr size_t kMcTableBytes = 256 * 16 + 256;
+constexpr uint32_t kMcNoAllocMaxBlocks = 1u << 20;   // 2^28 cubes
+constexpr size_t kMcScratchOffs = ((size_t)kMcNoAllocMaxBlocks * 4 + 255) / 256 * 256;
+constexpr size_t kMcScratchTotal = kMcScratchOffs + (size_t)kMcNoAllocMaxBlocks * 8;
+constexpr size_t kMcScratchBytes = kMcScratchTotal + 256;   // 12 MiB + 256 B
 constexpr int kMcMaxDevices = 64;
 
 __device__ __align__(256) uint8_t mc_table[kMcTableBytes];
+__device__ __align__(256) uint8_t scratch[kMcScratchBytes];
 
-// Device address of the table per device (from hipGetSymbolAddress).
+// Device addresses of the two arrays per device (from hipGetSymbolAddress;
+// host code never passes the arrays' names to a kernel or a copy).
 std::mutex g_mc_mu;
 uint8_t* g_mc_table_ptr[kMcMaxDevices] = {};
+uint8_t* g_mc_scratch_ptr[kMcMaxDevices] = {};
 
 struct McArgs {
     MkVolView v;
@@ -277,16 +288,19 @@ mk_status release_scratch(mk_context* ctx, hipStream_t s) {
 
 }  // namespace
 
-// Sets up, once per device (with that device current), the device address of
-// the triangle table, and fills it.
+// Sets up, once per device (with that device current), the device addresses
+// of the triangle table and of the allocation-free entry point's scratch, and
+// fills the table.
 mk_status mk_mc_device_init(int device) {
     if (device < 0 || device >= kMcMaxDevices)
         return mk_fail(nullptr, MK_ERR_ARG, "marching cubes: device %d beyond %d", device, kMcMaxDevices);
     std::lock_guard<std::mutex> lk(g_mc_mu);
     if (g_mc_table_ptr[device]) return MK_OK;
     void* table = nullptr;
+    void* scr = nullptr;
     hipError_t e = hipGetSymbolAddress(&table, HIP_SYMBOL(mc_table));
-    if (e != hipSuccess || !table)
+    if (e == hipSuccess) e = hipGetSymbolAddress(&scr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !table || !scr)
         return mk_fail(nullptr, MK_ERR_HIP, "hipGetSymbolAddress (marching cubes, device %d): %s", device,
                        hipGetErrorString(e));
     int8_t tri[256][16];
@@ -299,6 +313,7 @@ mk_status mk_mc_device_init(int device) {
     if (e != hipSuccess)
         return mk_fail(nullptr, MK_ERR_HIP, "marching-cubes table upload (device %d): %s", device, hipGetErrorString(e));
     g_mc_table_ptr[device] = (uint8_t*)table;
+    g_mc_scratch_ptr[device] = (uint8_t*)scr;
     return MK_OK;
 }
 
@@ -327,6 +342,42 @@ extern "C" mk_status mk_marching_cubes_async(mk_context* ctx, const int* box, fl
     st = release_scratch(ctx, s);
     if (st) return st;
     return mk_note_reader(ctx, s);
+}
+
+extern "C" mk_status mk_marching_cubes_noalloc(mk_context* ctx, const int* box, float iso, mk_triangle* dev_out,
+                                               uint64_t capacity, uint64_t* dev_count, void* stream) {
+    if (!ctx) return MK_ERR_ARG;
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    McArgs A;
+    uint32_t nblocks = 0;
+    mk_status st = prepare(ctx, box, iso, A, nblocks);
+    if (st) return st;
+    if (nblocks > kMcNoAllocMaxBlocks)
+        return mk_fail(ctx, MK_ERR_ARG,
+                       "mk_marching_cubes_noalloc: box of %llu cubes exceeds the scratch limit of %llu cubes (%u blocks "
+                       "of %d)",
+                       (unsigned long long)A.ncubes, (unsigned long long)kMcNoAllocMaxBlocks * kMcBlock,
+                       kMcNoAllocMaxBlocks, kMcBlock);
+    if (capacity > 0 && !dev_out) return mk_fail(ctx, MK_ERR_ARG, "null triangle buffer");
+    uint8_t* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_mc_mu);
+        base = g_mc_scratch_ptr[ctx->device];
+    }
+    if (!base) return mk_fail(ctx, MK_ERR_STATE, "marching-cubes scratch not set up on device %d", ctx->device);
+    // every block's count, every offset and the total are written before they
+    // are read, so nothing needs clearing
+    uint32_t* sums = (uint32_t*)base;
+    uint64_t* offs = (uint64_t*)(base + kMcScratchOffs);
+    unsigned long long* total = (unsigned long long*)(base + kMcScratchTotal);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_mc_count, dim3(nblocks), dim3(kMcBlock), 0, s, A, sums);
+    hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, s, sums, offs, nblocks, total,
+                       (unsigned long long*)dev_count);
+    if (capacity > 0)
+        hipLaunchKernelGGL(k_mc_emit, dim3(nblocks), dim3(kMcBlock), 0, s, A, offs, (float*)dev_out, capacity);
+    MK_HIP(ctx, hipGetLastError());
+    return MK_OK;
 }
 
 extern "C" mk_status mk_marching_cubes(mk_context* ctx, const int* box, float iso, mk_triangle* host_out,

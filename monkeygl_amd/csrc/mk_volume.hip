@@ -94,7 +94,8 @@ extern "C" mk_status mk_context_create(int device, mk_context** out) {
         mk_context_destroy(ctx);
         return mk_fail(nullptr, MK_ERR_HIP, "context stream/events: %s", hipGetErrorString(e));
     }
-    // per-device set-up: the marching-cubes triangle table
+    // per-device set-up: the marching-cubes triangle table and the device
+    // address of the allocation-free entry point's scratch
     mk_status st = mk_mc_device_init(device);
     if (st) {
         mk_context_destroy(ctx);
