@@ -290,12 +290,15 @@ extern "C" mk_status mk_batch_submit(mk_batcher* b, int slot, const mk_batch_ite
             return bfail(b, MK_ERR_ARG, "item " + std::to_string(i) + ": bad context index");
         mk_context* c = b->ctxs[it.context];
         if (!c->d_vox) return bfail(b, MK_ERR_STATE, "item " + std::to_string(i) + ": context has no volume");
+        // the one-request entry points' checks, so an item fails alone and in a batch alike
+        mk_status vst = MK_OK;
+        if (it.kind == MK_ITEM_RENDER) vst = mk_validate_render(c, &it.render);
+        else if (it.kind == MK_ITEM_PLANE) vst = mk_validate_plane(c, &it.plane);
+        else if (it.kind == MK_ITEM_CPR) vst = mk_validate_cpr(c, it.cpr_width, it.cpr_height, it.cpr_sampling);
+        else return bfail(b, MK_ERR_ARG, "item " + std::to_string(i) + ": unknown kind");
+        if (vst) return bfail(b, vst, "item " + std::to_string(i) + ": " + c->err);
         if (it.kind == MK_ITEM_RENDER) {
             const mk_render_params& p = it.render;
-            if (p.render_type < MK_RENDER_VR || p.render_type > MK_RENDER_SURFACE)
-                return bfail(b, MK_ERR_ARG, "item " + std::to_string(i) + ": bad render type");
-            if (!(p.step_scale > 0.0f) || c->nz / p.step_scale > 1e5f)
-                return bfail(b, MK_ERR_ARG, "item " + std::to_string(i) + ": bad step_scale");
             int& fs = first_item[it.context * 4];
             int& fa = first_item[it.context * 4 + 1 + p.render_type];
             if (fs >= 0 && items[fs].render.step_scale != p.step_scale)
@@ -423,9 +426,6 @@ extern "C" mk_status mk_batch_submit(mk_batcher* b, int slot, const mk_batch_ite
             w = p.width;
             h = p.height;
             out_bytes = (size_t)w * h * 2;
-            if (w <= 0 || h <= 0 || w > 65536 || h > 65536 || p.mpr_type < MK_MPR_AVERAGE || p.mpr_type > MK_MPR_MINIP ||
-                !(p.half_num >= 0.0f) || p.half_num > 65536.0f || !(p.pixel_spacing > 0.0f))
-                return bfail(b, MK_ERR_ARG, "item " + std::to_string(i) + ": bad plane parameters");
             mkp::PlaneArgs& A = d.pa;
             A.v = c->view();
             A.out = (int16_t*)(dev_arena + it.out_offset);
@@ -447,8 +447,6 @@ extern "C" mk_status mk_batch_submit(mk_batcher* b, int slot, const mk_batch_ite
             w = it.cpr_width;
             h = it.cpr_height;
             out_bytes = (size_t)w * h * 2;
-            if (w <= 0 || h <= 0 || w > 65536 || h > 65536)
-                return bfail(b, MK_ERR_ARG, "item " + std::to_string(i) + ": bad CPR size");
             const uint64_t geo = (uint64_t)h * 3 * sizeof(double);
             if (it.cpr_points_offset % 8 || it.cpr_dirs_offset % 8 || it.cpr_points_offset > arena_bytes ||
                 geo > arena_bytes - it.cpr_points_offset || it.cpr_dirs_offset > arena_bytes ||

@@ -168,6 +168,11 @@ mk_status mk_wait_readers(mk_context* ctx);
 mk_status mk_ensure_skip_tables(mk_context* ctx, int render_type, const mk_alpha_wwwl* aw);
 // uploads the float lattice a_{k+1} = a_k + l1 (a_0 = 0) up to past 1.732
 mk_status mk_ensure_steps(mk_context* ctx, float l1);
+// Argument checks of the one-request entry points, shared with the batched
+// path so that a request gets the same status alone and in a batch.
+mk_status mk_validate_render(mk_context* ctx, const mk_render_params* p);
+mk_status mk_validate_plane(mk_context* ctx, const mk_plane_params* p);
+mk_status mk_validate_cpr(mk_context* ctx, int width, int height, int sampling);
 // once per device, with the device current: the marching-cubes triangle table
 // and the device address of the allocation-free entry point's scratch
 mk_status mk_mc_device_init(int device);

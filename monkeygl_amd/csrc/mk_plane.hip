@@ -40,7 +40,10 @@ __global__ void __launch_bounds__(256) k_cpr(CprArgs A) {
     cpr_pixel<NEAREST>(A, x, y);
 }
 
-mk_status validate_plane(mk_context* ctx, const mk_plane_params* p) {
+
+}  // namespace
+
+mk_status mk_validate_plane(mk_context* ctx, const mk_plane_params* p) {
     if (!p) return mk_fail(ctx, MK_ERR_ARG, "plane params are null");
     if (!ctx->d_vox) return mk_fail(ctx, MK_ERR_STATE, "plane render before any volume was uploaded");
     if (p->width <= 0 || p->height <= 0 || p->width > 65536 || p->height > 65536)
@@ -56,11 +59,18 @@ mk_status validate_plane(mk_context* ctx, const mk_plane_params* p) {
     return MK_OK;
 }
 
-}  // namespace
+mk_status mk_validate_cpr(mk_context* ctx, int width, int height, int sampling) {
+    if (!ctx->d_vox) return mk_fail(ctx, MK_ERR_STATE, "CPR before any volume was uploaded");
+    if (width <= 0 || height <= 0 || width > 65536 || height > 65536)
+        return mk_fail(ctx, MK_ERR_ARG, "bad CPR size %dx%d", width, height);
+    if (sampling != MK_SAMPLE_LINEAR && sampling != MK_SAMPLE_NEAREST)
+        return mk_fail(ctx, MK_ERR_ARG, "unknown sampling %d", sampling);
+    return MK_OK;
+}
 
 extern "C" mk_status mk_render_plane_async(mk_context* ctx, const mk_plane_params* p, int16_t* dev_out, void* stream) {
     if (!ctx) return MK_ERR_ARG;
-    mk_status st = validate_plane(ctx, p);
+    mk_status st = mk_validate_plane(ctx, p);
     if (st) return st;
     if (!dev_out) return mk_fail(ctx, MK_ERR_ARG, "null output");
     MK_HIP(ctx, hipSetDevice(ctx->device));
@@ -104,7 +114,7 @@ extern "C" mk_status mk_render_plane_async(mk_context* ctx, const mk_plane_param
 
 extern "C" mk_status mk_render_plane(mk_context* ctx, const mk_plane_params* p, int16_t* host_out) {
     if (!ctx) return MK_ERR_ARG;
-    mk_status st = validate_plane(ctx, p);
+    mk_status st = mk_validate_plane(ctx, p);
     if (st) return st;
     if (!host_out) return mk_fail(ctx, MK_ERR_ARG, "null output");
     MK_HIP(ctx, hipSetDevice(ctx->device));
@@ -122,12 +132,9 @@ extern "C" mk_status mk_render_cpr_async(mk_context* ctx, int width, int height,
                                          const double* dev_dirs, int invert_z, int sampling, int16_t* dev_out,
                                          void* stream) {
     if (!ctx) return MK_ERR_ARG;
-    if (!ctx->d_vox) return mk_fail(ctx, MK_ERR_STATE, "CPR before any volume was uploaded");
-    if (width <= 0 || height <= 0 || width > 65536 || height > 65536)
-        return mk_fail(ctx, MK_ERR_ARG, "bad CPR size %dx%d", width, height);
+    mk_status vst = mk_validate_cpr(ctx, width, height, sampling);
+    if (vst) return vst;
     if (!dev_points || !dev_dirs || !dev_out) return mk_fail(ctx, MK_ERR_ARG, "null CPR buffer");
-    if (sampling != MK_SAMPLE_LINEAR && sampling != MK_SAMPLE_NEAREST)
-        return mk_fail(ctx, MK_ERR_ARG, "unknown sampling %d", sampling);
     MK_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = (hipStream_t)stream;
     mk_status st = mk_order_after_context(ctx, s);

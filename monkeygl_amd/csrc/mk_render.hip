@@ -34,6 +34,18 @@
 #include "mk_context.h"
 #include "mk_render_dev.h"
 
+mk_status mk_validate_render(mk_context* ctx, const mk_render_params* p) {
+    if (!p) return mk_fail(ctx, MK_ERR_ARG, "render params are null");
+    if (!ctx->d_vox) return mk_fail(ctx, MK_ERR_STATE, "render before any volume was uploaded");
+    if (p->width <= 0 || p->height <= 0 || p->width > 16384 || p->height > 16384)
+        return mk_fail(ctx, MK_ERR_ARG, "bad image size %dx%d", p->width, p->height);
+    if (p->render_type < MK_RENDER_VR || p->render_type > MK_RENDER_SURFACE)
+        return mk_fail(ctx, MK_ERR_ARG, "unknown render type %d", p->render_type);
+    if (!(p->step_scale > 0.0f) || !std::isfinite(p->step_scale) || ctx->nz / p->step_scale > 1e5f)
+        return mk_fail(ctx, MK_ERR_ARG, "bad step_scale %g", (double)p->step_scale);
+    return MK_OK;
+}
+
 namespace {
 
 constexpr int kMaxLdsBytes = 64 * 1024;
@@ -170,21 +182,9 @@ __global__ void __launch_bounds__(256) k_macro(const uint8_t* __restrict__ occ, 
     else occ2[m] = any;
 }
 
-mk_status validate(mk_context* ctx, const mk_render_params* p) {
-    if (!p) return mk_fail(ctx, MK_ERR_ARG, "render params are null");
-    if (!ctx->d_vox) return mk_fail(ctx, MK_ERR_STATE, "render before any volume was uploaded");
-    if (p->width <= 0 || p->height <= 0 || p->width > 16384 || p->height > 16384)
-        return mk_fail(ctx, MK_ERR_ARG, "bad image size %dx%d", p->width, p->height);
-    if (p->render_type < MK_RENDER_VR || p->render_type > MK_RENDER_SURFACE)
-        return mk_fail(ctx, MK_ERR_ARG, "unknown render type %d", p->render_type);
-    if (!(p->step_scale > 0.0f) || !std::isfinite(p->step_scale) || ctx->nz / p->step_scale > 1e5f)
-        return mk_fail(ctx, MK_ERR_ARG, "bad step_scale %g", (double)p->step_scale);
-    return MK_OK;
-}
-
 mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb, void* stream,
                         uint8_t* touch_v, uint8_t* touch_m) {
-    mk_status st = validate(ctx, p);
+    mk_status st = mk_validate_render(ctx, p);
     if (st) return st;
     if (!dev_rgb) return mk_fail(ctx, MK_ERR_ARG, "null output");
     MK_HIP(ctx, hipSetDevice(ctx->device));
@@ -426,7 +426,7 @@ extern "C" mk_status mk_render_footprint(mk_context* ctx, const mk_render_params
 
 extern "C" mk_status mk_render(mk_context* ctx, const mk_render_params* p, uint8_t* host_rgb) {
     if (!ctx) return MK_ERR_ARG;
-    mk_status st = validate(ctx, p);
+    mk_status st = mk_validate_render(ctx, p);
     if (st) return st;
     if (!host_rgb) return mk_fail(ctx, MK_ERR_ARG, "null output");
     MK_HIP(ctx, hipSetDevice(ctx->device));
