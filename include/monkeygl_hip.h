@@ -292,7 +292,10 @@ mk_status mk_tf_forward(mk_context* ctx, const mk_render_params* p, const float*
  * accumulators. Each ray is re-marched once for the gradients, un-compositing
  * the rest of the ray with a (1 - alpha) division from its final colour:
  * dev_rgb64 as mk_tf_forward wrote it for the same view, LUT and scale, or
- * NULL, when a first march recomputes it. Nothing per sample is stored. */
+ * NULL, when a first march recomputes it. Nothing per sample is stored.
+ * Accepts the same 2..4096 entries as the forward: up to 1365 entries each
+ * workgroup sums its gradients in LDS first, above that the lanes add them to
+ * the accumulators in device memory directly (same sums, in another order). */
 mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, const float* dev_lut, int n,
                          float opacity_scale, const float* dev_grad_rgb, const double* dev_rgb64,
                          double* dev_grad_lut, double* dev_grad_scale, void* stream);

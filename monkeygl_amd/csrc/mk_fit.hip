@@ -167,6 +167,9 @@ __device__ inline void fit_tile(const FitArgs& A, int& x, int& y) {
 // the same (the loads are bound by the L1 cache's access rate: 64-lane
 // gathers touch ~34 cache lines each, and 99.8% hit) and need more VGPRs.
 constexpr int kFitGroup = 1;
+// dynamic LDS of one fitting workgroup: the LUT (16 B per entry, up to 4096
+// entries) and, in the adjoint, its gradient sums when they fit beside it
+constexpr size_t kFitLdsBytes = 64 * 1024;
 
 // Visits the ray's in-VOI lattice points in order: visit(S, k) gets the
 // sample and its shading factor and returns false to end the march. The
@@ -249,14 +252,20 @@ __global__ void __launch_bounds__(256) k_fit_forward(FitArgs A, float* __restric
 
 // Adjoint: given g = dL/dC per pixel, accumulate dL/dLUT (n x 4) and dL/ds.
 // color64 holds the forward pass's double colours, or is null.
+// LDS_GRAD: the workgroup sums its LUT and scale gradients in LDS (4n + 1
+// doubles after the LUT) and adds the nonzero sums to HBM at the end. Without
+// it (LUTs too long for both tables in LDS) the lanes add to HBM directly.
+template <bool LDS_GRAD>
 __global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __restrict__ grad_img,
                                                       const double* __restrict__ color64,
                                                       double* __restrict__ grad_lut, double* __restrict__ grad_scale) {
     extern __shared__ float4 smem[];
     float4* s_lut = smem;
-    double* s_g = (double*)(smem + A.n);  // 4*n + 1
+    double* s_g = LDS_GRAD ? (double*)(smem + A.n) : grad_lut;  // 4*n LUT entries, then the scale
+    double* s_scale = LDS_GRAD ? s_g + 4 * A.n : grad_scale;
     for (int i = threadIdx.x; i < A.n; i += blockDim.x) s_lut[i] = A.lut[i];
-    for (int i = threadIdx.x; i < 4 * A.n + 1; i += blockDim.x) s_g[i] = 0.0;
+    if (LDS_GRAD)
+        for (int i = threadIdx.x; i < 4 * A.n + 1; i += blockDim.x) s_g[i] = 0.0;
     __syncthreads();
     int x, y;
     fit_tile(A, x, y);
@@ -337,9 +346,10 @@ __global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __
                 return T != 0.0;
             });
             flush();
-            atomicAdd(&s_g[4 * A.n], gs);
+            if (gs != 0.0) atomicAdd(s_scale, gs);
         }
     }
+    if (!LDS_GRAD) return;
     __syncthreads();
     for (int i = threadIdx.x; i < 4 * A.n; i += blockDim.x)
         if (s_g[i] != 0.0) atomicAdd(&grad_lut[i], s_g[i]);
@@ -416,10 +426,15 @@ extern "C" mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, 
     st = mk_order_after_context(ctx, s);
     if (st) return st;
     const uint32_t tiles = (uint32_t)(((p->width + 15) / 16) * ((p->height + 15) / 16));
-    const size_t lds = (size_t)n * sizeof(float4) + (4 * (size_t)n + 1) * sizeof(double);
-    if (lds > 64 * 1024) return mk_fail(ctx, MK_ERR_ARG, "LUT of %d entries exceeds the LDS budget of the adjoint", n);
-    hipLaunchKernelGGL(k_fit_backward, dim3(tiles), dim3(256), lds, s, A, dev_grad_rgb, dev_rgb64, dev_grad_lut,
-                       dev_grad_scale);
+    // gradient sums in LDS beside the LUT when both fit (n <= 1365), else in HBM
+    const size_t lds_lut = (size_t)n * sizeof(float4);
+    const size_t lds_all = lds_lut + (4 * (size_t)n + 1) * sizeof(double);
+    if (lds_all <= kFitLdsBytes)
+        hipLaunchKernelGGL(k_fit_backward<true>, dim3(tiles), dim3(256), lds_all, s, A, dev_grad_rgb, dev_rgb64,
+                           dev_grad_lut, dev_grad_scale);
+    else
+        hipLaunchKernelGGL(k_fit_backward<false>, dim3(tiles), dim3(256), lds_lut, s, A, dev_grad_rgb, dev_rgb64,
+                           dev_grad_lut, dev_grad_scale);
     MK_HIP(ctx, hipGetLastError());
     return mk_note_reader(ctx, s);
 }

@@ -160,6 +160,42 @@ def centerline(dims, n: int = 200, seed: int = 2) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
+# Transfer-function fitting: the 256^3 blob volume (TF_FIT), 8 views, a
+# 256-entry RGBA LUT and an opacity scale, MSE to images of a seeded
+# ground-truth LUT
+
+FIT_LUT_ENTRIES = 256
+FIT_SCALE_TRUE, FIT_SCALE_START = 1.2, 1.5
+
+
+def fit_views(size: int = 512, n: int = 8):
+    """Render arguments of the fitting views (label 0 window 2200 / 0). The
+    angles are generic: at multiples of 90 degrees whole pixel columns run
+    exactly on a volume face, where float sample positions and an FP64 check's
+    positions fall on opposite sides of it."""
+    from . import params as P
+
+    out = []
+    for k in range(n):
+        info = P.RenderInfo()
+        info.set_vr_size(size, size)
+        info.rotate(45.0 * k + 15.0, 20.0 * (k % 3) - 17.0)
+        out.append(P.render_params(TF_FIT.dims, TF_FIT.spacing, info, {0: P.LabelState(1.0, 2200.0, 0.0)},
+                                   width=size, height=size))
+    return out
+
+
+def fit_lut(seed: int, n: int = FIT_LUT_ENTRIES) -> np.ndarray:
+    """Seeded (n, 4) float32 LUT: colours in [0.1, 1), alphas in [0, 0.08).
+    Seed 1 is the ground truth, seed 2 the start of the fit."""
+    rng = np.random.default_rng(seed)
+    a = np.empty((n, 4), np.float32)
+    a[:, :3] = rng.uniform(0.1, 1.0, (n, 3))
+    a[:, 3] = rng.uniform(0.0, 0.08, n)
+    return a
+
+
+# ---------------------------------------------------------------------------
 # Render serving: three resident volumes and a Poisson request stream
 
 SERVE_MIX = (("vr", 0.60), ("mip", 0.20), ("mpr", 0.15), ("cpr", 0.05))

@@ -522,6 +522,18 @@ int mko_fit_render(const mk_render_params* P, const int16_t* v, int nx, int ny, 
     return 0;
 }
 
+// Rows [y0, y1) of the fitting-rule image in double: out is (y1 - y0) * width * 3.
+int mko_fit_render_rows(const mk_render_params* P, const int16_t* v, int nx, int ny, int nz, const double* lut, int n,
+                        double s, int y0, int y1, double* out) {
+    if (!P || !v || !lut || !out || n < 2 || y0 < 0 || y1 > P->height || y0 > y1) return 1;
+    Volume vol{v, nullptr, nx, ny, nz, nullptr};
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int y = y0; y < y1; ++y)
+        for (int x = 0; x < P->width; ++x)
+            fit_pixel(*P, vol, lut, n, s, x, y, out + 3 * ((size_t)(y - y0) * P->width + x));
+    return 0;
+}
+
 // The expanded triangle table, for pinning against the reference's table.
 void mko_mc_table(int8_t* tri, uint8_t* cnt) {
     mk_mc_expand((int8_t(*)[16])tri, cnt);

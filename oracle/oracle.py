@@ -37,6 +37,8 @@ def lib() -> C.CDLL:
         L.mko_marching_cubes.argtypes = [vp, i, i, i, vp, C.c_float, vp, C.c_uint64]
         L.mko_fit_render.restype = i
         L.mko_fit_render.argtypes = [C.POINTER(RenderParams), vp, i, i, i, vp, i, C.c_double, vp]
+        L.mko_fit_render_rows.restype = i
+        L.mko_fit_render_rows.argtypes = [C.POINTER(RenderParams), vp, i, i, i, vp, i, C.c_double, i, i, vp]
         L.mko_mc_table.restype = None
         L.mko_mc_table.argtypes = [vp, vp]
         L.mko_tex3_linear.restype = C.c_double
@@ -158,5 +160,17 @@ def fit_render(p: RenderParams, vol_zyx, lut, scale: float) -> np.ndarray:
     out = np.zeros((p.height, p.width, 3), np.float64)
     rc = lib().mko_fit_render(C.byref(p), v.ctypes.data, nx, ny, nz, l.ctypes.data, l.shape[0], float(scale),
                               out.ctypes.data)
+    assert rc == 0
+    return out
+
+
+def fit_render_rows(p: RenderParams, vol_zyx, lut, scale: float, y0: int, y1: int) -> np.ndarray:
+    """Rows [y0, y1) of fit_render's image, (y1 - y0, width, 3) float64."""
+    v = np.ascontiguousarray(vol_zyx, dtype=np.int16)
+    nz, ny, nx = v.shape
+    l = np.ascontiguousarray(lut, dtype=np.float64).reshape(-1, 4)
+    out = np.zeros((y1 - y0, p.width, 3), np.float64)
+    rc = lib().mko_fit_render_rows(C.byref(p), v.ctypes.data, nx, ny, nz, l.ctypes.data, l.shape[0], float(scale),
+                                   int(y0), int(y1), out.ctypes.data)
     assert rc == 0
     return out
