@@ -49,14 +49,16 @@ struct FitArgs {
     int max_iter;
 };
 
+// The sample position of the VR kernels' ray_pos: the same fused operations
+// in the same order, so both rules place their samples identically.
 __device__ inline void fit_pos(const FitArgs& A, float u, float v, float acc, float p[3]) {
-    float fy = (float)(((double)acc - 0.866) * A.dscale);
-    float x = u * A.M[0] + fy * A.M[1] + v * A.M[2];
-    float y = u * A.M[3] + fy * A.M[4] + v * A.M[5];
-    float z = u * A.M[6] + fy * A.M[7] + v * A.M[8];
-    p[0] = x * A.mls[0] + 0.5f;
-    p[1] = y * A.mls[1] + 0.5f;
-    p[2] = z * A.mls[2] + 0.5f;
+    const float fy = (float)(((double)acc - 0.866) * A.dscale);
+    const float x = fmaf(v, A.M[2], fmaf(fy, A.M[1], u * A.M[0]));
+    const float y = fmaf(v, A.M[5], fmaf(fy, A.M[4], u * A.M[3]));
+    const float z = fmaf(v, A.M[8], fmaf(fy, A.M[7], u * A.M[6]));
+    p[0] = fmaf(x, A.mls[0], 0.5f);
+    p[1] = fmaf(y, A.mls[1], 0.5f);
+    p[2] = fmaf(z, A.mls[2], 0.5f);
     if (A.invert_z) p[2] = 1.0f - p[2];
 }
 

@@ -76,14 +76,20 @@ struct AwTable {
 // 1.732f is the largest float below 1.732, so the bound is t <= 1.732f.
 constexpr float kAccEnd = 1.732f;
 
+// Sample position at accumulated length acc (core/kernel.cu:197-210). The
+// multiply-adds are written as fused operations in a fixed order (the
+// reference's dot product left to right, then pos * maxLenSpacing + 0.5), so
+// the float positions are defined to the bit: in views along an axis they
+// fall exactly on voxel faces, where their last bit decides the VOI test and
+// the mask label of a whole column of samples.
 __device__ inline void ray_pos(const RenderArgs& A, float u, float v, float acc, float p[3]) {
-    float fy = (float)(((double)acc - 0.866) * A.scale);
-    float x = u * A.M[0] + fy * A.M[1] + v * A.M[2];
-    float y = u * A.M[3] + fy * A.M[4] + v * A.M[5];
-    float z = u * A.M[6] + fy * A.M[7] + v * A.M[8];
-    p[0] = x * A.mls[0] + 0.5f;
-    p[1] = y * A.mls[1] + 0.5f;
-    p[2] = z * A.mls[2] + 0.5f;
+    const float fy = (float)(((double)acc - 0.866) * A.scale);
+    const float x = fmaf(v, A.M[2], fmaf(fy, A.M[1], u * A.M[0]));
+    const float y = fmaf(v, A.M[5], fmaf(fy, A.M[4], u * A.M[3]));
+    const float z = fmaf(v, A.M[8], fmaf(fy, A.M[7], u * A.M[6]));
+    p[0] = fmaf(x, A.mls[0], 0.5f);
+    p[1] = fmaf(y, A.mls[1], 0.5f);
+    p[2] = fmaf(z, A.mls[2], 0.5f);
     if (A.invert_z) p[2] = 1.0f - p[2];
 }
 

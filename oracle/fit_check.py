@@ -1,10 +1,12 @@
-"""Shared check of the transfer-function adjoint against FP64 central finite
-differences of the oracle's fitting rule (h = 1e-4)."""
+"""TEST INFRASTRUCTURE ONLY: the transfer-function adjoint checked against
+FP64 central finite differences of the oracle's fitting rule (h = 1e-4).
+
+Used by tests/ and by bench.py's fitting block, as the checker."""
 from __future__ import annotations
 
 import numpy as np
 
-from oracle import oracle as O
+from . import oracle as O
 
 
 def fd_check(ctx, vol, views, lut_gt, lut0, s_gt: float, s0: float, rows=None, n_check: int = 64, seed: int = 7,

@@ -145,31 +145,41 @@ template <typename T> static inline void write_rgb(uint8_t* o, const T c[4]) {
 }
 
 template <typename T> struct Ray {
-    T u, v;
-    T M[9];
-    T mls[3];
+    float u, v;
+    float M[9];
+    float mls[3];
     int invz;
     // core/kernel.cu:197-210 position of the sample at accumulated length acc.
-    // acc is float in every instance: the reference defines its sample lattice
-    // by float accumulation (kernel.cu:169-172, 213, 238), so the FP64 oracle
-    // marches on that lattice and evaluates positions and values in double.
+    // Positions are float in every instance, as the reference computes them:
+    // the reference defines its sample lattice by float accumulation
+    // (kernel.cu:169-172, 213, 238) and its sample positions by float
+    // arithmetic, and in views along an axis those positions fall exactly on
+    // voxel faces, where only their float value decides the VOI test and the
+    // mask label of a whole column of samples. The multiply-adds are fused,
+    // the dot product left to right, as CUDA's default contraction compiles
+    // the reference's expressions (and as the HIP kernels write them). The
+    // FP64 oracle evaluates everything after the position in double.
     inline void pos(float acc, double scale, T p[3]) const {
-        T fy = (T)(((double)acc - 0.866) * scale);
-        T x = u * M[0] + fy * M[1] + v * M[2];
-        T y = u * M[3] + fy * M[4] + v * M[5];
-        T z = u * M[6] + fy * M[7] + v * M[8];
-        p[0] = x * mls[0] + T(0.5);
-        p[1] = y * mls[1] + T(0.5);
-        p[2] = z * mls[2] + T(0.5);
-        if (invz) p[2] = T(1) - p[2];
+        const float fy = (float)(((double)acc - 0.866) * scale);
+        const float x = std::fma(v, M[2], std::fma(fy, M[1], u * M[0]));
+        const float y = std::fma(v, M[5], std::fma(fy, M[4], u * M[3]));
+        const float z = std::fma(v, M[8], std::fma(fy, M[7], u * M[6]));
+        float pz = std::fma(z, mls[2], 0.5f);
+        if (invz) pz = 1.0f - pz;
+        p[0] = (T)std::fma(x, mls[0], 0.5f);
+        p[1] = (T)std::fma(y, mls[1], 0.5f);
+        p[2] = (T)pz;
     }
 };
 
 template <typename T> static inline bool in_voi(const mk_voi& voi, const Volume& vol, const T p[3], int& ix, int& iy, int& iz) {
-    // core/kernel.cu:208-211: truncation toward zero of pos*dim.
-    T fx = p[0] * T(vol.nx), fy = p[1] * T(vol.ny), fz = p[2] * T(vol.nz);
+    // core/kernel.cu:208-211: truncation toward zero of pos*dim, a float
+    // product there (pos is float3, the extent converts to float); p holds
+    // the float position (Ray::pos), so every instance takes the reference's
+    // voxel index and with it its VOI decision and mask label
+    const float fx = (float)p[0] * (float)vol.nx, fy = (float)p[1] * (float)vol.ny, fz = (float)p[2] * (float)vol.nz;
     // guard the int conversion; anything this far out fails the test anyway
-    if (!(fx > T(-1e9) && fx < T(1e9) && fy > T(-1e9) && fy < T(1e9) && fz > T(-1e9) && fz < T(1e9))) return false;
+    if (!(fx > -1e9f && fx < 1e9f && fy > -1e9f && fy < 1e9f && fz > -1e9f && fz < 1e9f)) return false;
     ix = (int)fx; iy = (int)fy; iz = (int)fz;
     return !(ix < voi.left || ix > voi.right || iy < voi.posterior || iy > voi.anterior || iz < voi.head || iz > voi.foot);
 }
@@ -199,11 +209,11 @@ template <typename T, bool Q8> static void gradient(const Volume& vol, const T p
 
 template <typename T> static Ray<T> make_ray(const mk_render_params& P, int x, int y) {
     Ray<T> r;
-    // core/kernel.cu:161-162
-    r.u = (T(x) - T(P.width) / T(2) - T(P.x_translate)) / T(P.height);
-    r.v = (T(y) - T(P.height) / T(2) - T(P.y_translate)) / T(P.height);
-    for (int i = 0; i < 9; ++i) r.M[i] = T(P.transform[i]);
-    for (int i = 0; i < 3; ++i) r.mls[i] = T(P.max_len_spacing[i]);
+    // core/kernel.cu:161-162, in float as there
+    r.u = ((float)x - (float)P.width / 2.0f - P.x_translate) / (float)P.height;
+    r.v = ((float)y - (float)P.height / 2.0f - P.y_translate) / (float)P.height;
+    for (int i = 0; i < 9; ++i) r.M[i] = P.transform[i];
+    for (int i = 0; i < 3; ++i) r.mls[i] = P.max_len_spacing[i];
     r.invz = P.invert_z;
     return r;
 }

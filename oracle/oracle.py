@@ -49,6 +49,29 @@ def lib() -> C.CDLL:
     return _lib
 
 
+def build_native(out_dir) -> Path:
+    """Compiles mk_oracle.cpp for the host this runs on (-O3 -march=native
+    -ffp-contract=off -fopenmp) into out_dir, for timing the CPU baseline. The
+    library built with the package targets x86-64-v3, since it is built on one
+    machine and run on another. Raises when the compiler fails."""
+    import subprocess
+
+    out = Path(out_dir) / "libmk_oracle_native.so"
+    src = Path(__file__).resolve().parent / "mk_oracle.cpp"
+    inc = Path(__file__).resolve().parent.parent / "include"
+    subprocess.run(["g++", "-O3", "-march=native", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-std=c++17",
+                    "-I", str(inc), str(src), "-o", str(out)], check=True, capture_output=True)
+    return out
+
+
+def use_library(path=None) -> None:
+    """Switches the binding to another build of mk_oracle.cpp (None: the one
+    built with the package); takes effect at the next call."""
+    global LIB, _lib
+    LIB = Path(path) if path else Path(__file__).resolve().parent / "_build" / "libmk_oracle.so"
+    _lib = None
+
+
 def set_threads(n: int) -> int:
     return lib().mko_set_threads(n)
 

@@ -1,10 +1,21 @@
 """Parity at the workload sizes the benchmark configurations name, through
-the C ABI: the cardiac-sized set (512x512x361: 1024^2 VR and MIP over 12 views
-rotating about z; axial, sagittal and coronal MPR at 512^2, nearest bit-exact
-and trilinear within 1 HU) and the body-sized set with its 4-label mask
-(1024^2 VR; stretched and straightened CPR along the seeded 200-point
-centreline). Full VR/MIP frames are checked on bands of rows against the FP64
-oracle (1/255 on >= 99.9% of the band's pixels); MPR and CPR images whole."""
+the C ABI, on whole images:
+* the cardiac-sized set (512x512x361): 1024^2 VR and MIP over 12 views
+  rotating about z; axial, sagittal and coronal MPR at 512^2, nearest
+  bit-exact and trilinear within 1 HU;
+* the body-sized set with its 4-label mask (512x512x1559): 1024^2 VR over
+  all 12 views with label 0's alpha cap at 0.4 and at 0.8; stretched and
+  straightened CPR along the seeded 200-point centreline; marching cubes at
+  iso 200 on the 256^3 block (128..384, 128..384, 651..907), bit-exact.
+
+VR/MIP bar: every RGB channel within 1/255 of the FP64 oracle on >= 99.9% of
+each frame's pixels. The few pixels beyond (a handful per 1024^2 body frame)
+are rays where a decision of the reference's rule sits on a knife edge that
+float and double rounding resolve differently -- mostly the per-label opacity
+cap (label 0's accumulated opacity ending at 0.399994 in one precision and
+0.400003 in the other against a cap of 0.4), also the 0.001 refinement
+threshold and a mask label read at a truncated index on a label boundary --
+so the ray composites one sample more or fewer."""
 import numpy as np
 import pytest
 
@@ -15,8 +26,6 @@ from monkeygl_amd._capi import MK_MPR_AVERAGE, MK_RENDER_MIP, MK_RENDER_VR, MK_S
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
-
-BANDS = ((256, 264), (508, 516), (760, 768))   # three 8-row bands of a 1024^2 frame
 
 
 @pytest.fixture(scope="module")
@@ -41,13 +50,11 @@ def _views(spec, labels, rt, n=12, size=1024):
     return out
 
 
-def _band_ok(img, p, vol, luts, mask=None):
-    worst = 1.0
-    for y0, y1 in BANDS:
-        ref = O.render_rows(p, vol, luts, y0, y1, mask_zyx=mask)
-        d = np.abs(img[y0:y1].astype(int) - ref.astype(int)).max(-1)
-        worst = min(worst, float((d <= 1).mean()))
-    return worst
+def _frame_ok(img, p, vol, luts, mask=None):
+    """Fraction of pixels within 1/255 of the FP64 oracle, pixels beyond, max difference."""
+    ref = O.render(p, vol, luts, mask_zyx=mask, precision="f64")
+    d = np.abs(img.astype(int) - ref.astype(int)).max(-1)
+    return float((d <= 1).mean()), int((d > 1).sum()), int(d.max())
 
 
 @pytest.mark.parametrize("rt", [MK_RENDER_VR, MK_RENDER_MIP])
@@ -60,7 +67,8 @@ def test_cardiac_vr_mip_12_views(gpu_ctx, cardiac, rt):
     for k, p in enumerate(_views(W.CARDIAC, labels, rt)):
         img = gpu_ctx.render(p)
         assert img.shape == (1024, 1024, 3)
-        frac = _band_ok(img, p, cardiac, luts)
+        frac, n_beyond, dmax = _frame_ok(img, p, cardiac, luts)
+        print(f"cardiac rt={rt} view {k}: {frac:.6f} within 1/255, {n_beyond} beyond, max {dmax}", flush=True)
         assert frac >= 0.999, (k, frac)
 
 
@@ -79,7 +87,8 @@ def test_cardiac_mpr_512(gpu_ctx, cardiac):
                 assert (d <= 1).mean() >= 0.999, (plane, int(d.max()))
 
 
-def test_body_vr_with_mask(gpu_ctx, body):
+@pytest.mark.parametrize("alpha0", [0.4, 0.8])
+def test_body_vr_with_mask_12_views(gpu_ctx, body, alpha0):
     O.set_threads(16)
     vol, mask = body
     gpu_ctx.set_volume(vol)
@@ -88,12 +97,29 @@ def test_body_vr_with_mask(gpu_ctx, body):
     for l in range(5):
         luts[l] = P.transfer_function(W.tf_points(100 + l))
         gpu_ctx.set_lut(l, luts[l])
-        labels[l] = P.LabelState(alpha=0.4 if l == 0 else 1.0, ww=1200.0 + 200.0 * l, wl=-200.0 + 60.0 * l)
-    for k, p in enumerate(_views(W.BODY, labels, MK_RENDER_VR, n=4)):
+        labels[l] = P.LabelState(alpha=alpha0 if l == 0 else 1.0, ww=1200.0 + 200.0 * l, wl=-200.0 + 60.0 * l)
+    for k, p in enumerate(_views(W.BODY, labels, MK_RENDER_VR)):
         img = gpu_ctx.render(p)
-        frac = _band_ok(img, p, vol, luts, mask)
+        frac, n_beyond, dmax = _frame_ok(img, p, vol, luts, mask)
+        print(f"body alpha0={alpha0} view {k}: {frac:.6f} within 1/255, {n_beyond} beyond, max {dmax}", flush=True)
         assert frac >= 0.999, (k, frac)
     gpu_ctx.clear_mask()
+
+
+def test_body_marching_cubes_256_block(gpu_ctx, body):
+    """The body set's own 256^3 block at iso 200: the triangle list bit-exact
+    against the oracle, and with a capacity below the total its first
+    triangles."""
+    vol, _ = body
+    gpu_ctx.set_volume(vol)
+    box = (128, 128, 651, 384, 384, 907)
+    got = gpu_ctx.marching_cubes(box, 200.0)
+    ref = O.marching_cubes(vol, box, 200.0)
+    print(f"body block: {len(ref)} triangles", flush=True)
+    assert got.shape == ref.shape and ref.shape[0] > 1000000
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    part = gpu_ctx.marching_cubes(box, 200.0, capacity=1000)
+    assert np.array_equal(part.view(np.uint32), ref[:1000].view(np.uint32))
 
 
 @pytest.mark.parametrize("which", [G.PLANE_STRETCHED_CPR, G.PLANE_STRAIGHTENED_CPR])

@@ -109,23 +109,6 @@ def test_gpu_marching_cubes_bit_exact(gpu_ctx, iso):
 
 
 @pytest.mark.gpu
-def test_gpu_marching_cubes_256_subblock_of_body(gpu_ctx):
-    from monkeygl_amd import workloads as W
-
-    spec = W.VolumeSpec("mcbody", (256, 256, 300), (0.7422, 0.7422, 1.0), 6, (8, 40), (800, 1400), n_vessels=10, seed=3)
-    vol = W.make_volume(spec)
-    gpu_ctx.set_volume(vol)
-    box = (0, 0, 20, 256, 256, 276)  # a 256^3 sub-block
-    got = gpu_ctx.marching_cubes(box, 200.0)
-    ref = O.marching_cubes(vol, box, 200.0)
-    assert got.shape == ref.shape and ref.shape[0] > 0
-    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
-    # capacity below the total: the first `cap` triangles, same bits
-    part = gpu_ctx.marching_cubes(box, 200.0, capacity=1000)
-    assert np.array_equal(part.view(np.uint32), ref[:1000].view(np.uint32))
-
-
-@pytest.mark.gpu
 def test_gpu_marching_cubes_rejects_bad_box(gpu_ctx):
     from monkeygl_amd._capi import MkError
 

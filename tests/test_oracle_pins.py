@@ -137,6 +137,38 @@ def test_vr_gradient_shading_sign(sign):
     assert abs(int(img[16, 16, 0]) - exp) <= 1, (img[16, 16], exp)
 
 
+@pytest.mark.parametrize("gx,gy", [(8, 6), (7, 24)])
+def test_vr_gradient_shading_oblique(gx, gy):
+    """Ramp v = gx*x + gy*y: the one-voxel central differences give the
+    normal (gx, gy, 0)/|g| exactly, the headlight at identity is +y, so
+    d = gy/|g| (0.6 and 0.96) and every sample is shaded with
+    k = 0.35 + 0.8 d + 0.16 d^8 (core/kernel.cu:69-77). At d = 0.96 the
+    specular exponent moves the pixel by >= 3 counts from d^2 or d^4, at
+    d = 0.6 the diffuse coefficient by 12; the bar is 1."""
+    dims = (40, 40, 24)
+    vol = ramp(dims, gx, gy, 0, 0)
+    rgb = np.array([0.7, 0.7, 0.7], np.float32)
+    info = P.RenderInfo()
+    info.set_vr_size(32, 32)
+    p = P.render_params(dims, (1.0, 1.0, 1.0), info, {0: P.LabelState(1.0, 1e6, 0.0)})
+    p.voi.left, p.voi.right, p.voi.posterior, p.voi.anterior = 10, 30, 10, 30
+    img = O.render(p, vol, {0: _flat_lut(rgb, 0.6)})
+    A = _saturating_alpha(float(np.float32(0.6)))
+    d = gy / np.hypot(gx, gy)
+    k = 0.35 + 0.8 * d + 0.16 * d ** 8
+    def pixel(kk):
+        return int(np.floor(min(1.0, kk * float(rgb[0]) * A) * 255))
+
+    exp = pixel(k)
+    # the pin tells the rule from its plausible mistakes
+    if gy == 24:
+        assert abs(pixel(0.35 + 0.8 * d + 0.16 * d ** 2) - exp) > 1
+        assert abs(pixel(0.35 + 0.8 * d + 0.16 * d ** 4) - exp) > 1
+    else:
+        assert abs(pixel(0.35 + 0.8 + 0.16 * d ** 8) - exp) > 1
+    assert abs(int(img[16, 16, 0]) - exp) <= 1, (img[16, 16], exp)
+
+
 def test_vr_empty_tf_background():
     vol = np.full((16, 16, 16), 500, np.int16)
     p, img = _vr(vol, np.zeros((100, 4), np.float32), background=(0.2, 0.4, 0.6, 1.0))
@@ -191,6 +223,50 @@ def test_mip_plateau_closed_form():
     p, img = _vr(vol, _flat_lut([1, 1, 1], 1), rt=MK_RENDER_MIP, labels={0: P.LabelState(1.0, 2000.0, 0.0)})
     exp = int(np.floor(min(1.0, (600 * K) / 2000.0 + 0.5) * 255))
     assert int(img[16, 16, 0]) == exp
+
+
+@pytest.mark.parametrize("alpha0,alpha1,expect", [(1.0, 0.0, "slab0"), (1.0, 0.3, "slab1"), (0.0, 0.3, "slab1"),
+                                                   (0.0, 0.0, "background")])
+def test_mip_ignores_labels_with_zero_alpha(alpha0, alpha1, expect):
+    """MIP takes the windowed value only of samples whose label has alpha > 0
+    (core/kernel.cu:334-343): a 600 HU slab of label 0 and a 1500 HU slab of
+    label 1 on the ray; the maximum is the slab of the labels that count, and
+    the background when none does."""
+    dims = (24, 24, 24)
+    vol = np.full(dims[::-1], -1000, np.int16)
+    vol[:, 4:8, :] = 600
+    vol[:, 14:18, :] = 1500
+    mask = np.zeros(dims[::-1], np.uint8)
+    mask[:, 14:18, :] = 1
+    info = P.RenderInfo()
+    info.set_vr_size(32, 32)
+    labels = {0: P.LabelState(alpha0, 2000.0, 0.0), 1: P.LabelState(alpha1, 2000.0, 0.0)}
+    p = P.render_params(dims, (1.0, 1.0, 1.0), info, labels, render_type=MK_RENDER_MIP, background=(0.2, 0.2, 0.2, 1.0))
+    img = O.render(p, vol, {0: _flat_lut([1, 1, 1], 1), 1: _flat_lut([1, 1, 1], 1)}, mask_zyx=mask)
+    want = {"slab0": int(np.floor(min(1.0, 600 * K / 2000.0 + 0.5) * 255)), "slab1": 255,
+            "background": int(np.float32(0.2) * 255)}[expect]
+    assert want in (204, 255, 51)
+    assert int(img[16, 16, 0]) == want, (img[16, 16], want)
+
+
+def test_surface_oblique_normal():
+    """Ramp v = 8x + 6y + 300 (every in-VOI sample above 400 HU): the first
+    sample's normal is (0.8, 0.6, 0), the headlight +y, so d = 0.6 and the
+    gray is d + 0.05 = 0.65 -> 165 (core/kernel.cu:441-462); without the 0.05
+    it would be 153, with the normal's sign flipped the ray finds no hit."""
+    dims = (40, 40, 24)
+    info = P.RenderInfo()
+    info.set_vr_size(32, 32)
+    for gy, want in [(6, 165), (-6, None)]:
+        vol = ramp(dims, 8, gy, 0, 300 if gy > 0 else 700)
+        p = P.render_params(dims, (1.0, 1.0, 1.0), info, {0: P.LabelState(1.0, 400.0, 40.0)},
+                            render_type=MK_RENDER_SURFACE, background=(0.0, 0.0, 0.0, 1.0))
+        p.voi.left, p.voi.right, p.voi.posterior, p.voi.anterior = 10, 30, 10, 30
+        img = O.render(p, vol, {0: _flat_lut([1, 1, 1], 1)})
+        if want is None:
+            assert int(img[16, 16, 0]) == 0
+        else:
+            assert int(img[16, 16, 0]) == want and int(img[16, 16, 1]) == want, img[16, 16]
 
 
 def test_surface_first_hit():

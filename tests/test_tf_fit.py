@@ -86,7 +86,7 @@ def _assert_fd(r, n_check=64):
 def test_tf_gradient_matches_fp64_finite_differences(gpu_ctx):
     """dLoss/dLUT and dLoss/dscale from the adjoint vs FP64 central differences
     (h = 1e-4) on 64 random live LUT entries: relative error <= 1e-3."""
-    from fit_util import fd_check
+    from oracle.fit_check import fd_check
 
     vol = W.make_volume(SPEC)
     gpu_ctx.set_volume(vol)
@@ -100,7 +100,7 @@ def test_tf_gradient_2048_entry_lut(gpu_ctx):
     finite-difference bar holds, and the forward matches FP64."""
     import torch
 
-    from fit_util import fd_check
+    from oracle.fit_check import fd_check
 
     vol = W.make_volume(SPEC)
     gpu_ctx.set_volume(vol)

@@ -2,14 +2,7 @@
 volume, a 256-entry RGBA LUT plus an opacity scale, 8 views at 512^2.
 
 * forward: all 8 whole views of the fitting rule against the FP64 oracle,
-  within 2e-4 of the image maximum on >= 99.99% of each view's pixels. The
-  few others (1 to 6 of 262,144 per view) are rays with one sample on a
-  volume face: the kernel computes sample positions in float, as the
-  reference does, the oracle in double, and a position such as
-  y = 0.99999999 (index 255, inside) becomes 1.0 (index 256, outside) in
-  float, so one sample is composited by one and not the other. Those pixels
-  stay within the largest contribution one sample can make,
-  max k * c * a = (0.35 + 0.8 + 0.16) * 1 * (0.08 * 1.5) < 0.16;
+  every pixel within 2e-4 of the image maximum;
 * adjoint: against FP64 central differences (h = 1e-4) on 64 random live LUT
   entries and the scale, relative error <= 1e-3. The loss is the mean squared
   error over a fixed band of 8 rows of every view (rows 252..259, through the
@@ -47,20 +40,16 @@ def test_fit_config_forward_all_views_match_fp64(gpu_ctx, fit_scene):
         img = render_tf(gpu_ctx, views, torch.tensor(lut0, device="cuda"),
                         torch.tensor(W.FIT_SCALE_START)).cpu().numpy()
     assert img.shape == (8, 512, 512, 3)
-    one_sample = (0.35 + 0.8 + 0.16) * 1.0 * (0.08 * W.FIT_SCALE_START)
     for k, p in enumerate(views):
         ref = O.fit_render(p, vol, lut0.astype(np.float64), W.FIT_SCALE_START)
         assert ref.max() > 0.05, k
-        err = np.abs(img[k].astype(np.float64) - ref).max(-1) / max(1.0, ref.max())
-        frac = float((err <= 2e-4).mean())
-        print(f"view {k}: {frac:.6f} of pixels within 2e-4, {int((err > 2e-4).sum())} beyond, max {err.max():.2e}",
-              flush=True)
-        assert frac >= 0.9999, (k, frac)
-        assert err.max() < one_sample, (k, err.max())
+        err = np.abs(img[k].astype(np.float64) - ref).max() / max(1.0, ref.max())
+        print(f"view {k}: max relative error {err:.2e}", flush=True)
+        assert err < 2e-4, (k, err)
 
 
 def test_fit_config_adjoint_matches_fp64_finite_differences(gpu_ctx, fit_scene):
-    from fit_util import fd_check
+    from oracle.fit_check import fd_check
 
     vol, views, lut_gt, lut0 = fit_scene
     gpu_ctx.set_volume(vol)
