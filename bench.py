@@ -15,9 +15,16 @@ The JSON line carries:
     + 3 B per output pixel) over the VR frame's mean duration from HIP events
     on the launch stream (both passes: k_vr, then k_vr_tail for the rays it
     queued), against the 8 TB/s HBM peak;
-  * parity: a band of rows of one frame against the FP64 oracle;
-  * cpu_baseline: the oracle's float instance (8-bit filter weights) on a
-    band of rows of the same frame, converted to frames per second;
+  * parity: one whole frame against the FP64 oracle (pixels within 1/255,
+    pixels beyond, their maximum difference and cause);
+  * cpu_baseline: the oracle compiled for the host it runs on (-march=native),
+    float instance with 8-bit filter weights, on whole frames of the same
+    views for --cpu-seconds, in frames per second; the FP64 oracle timed
+    beside it; threads from OMP_NUM_THREADS when set, with nproc recorded;
+  * fit: transfer-function fitting at its configuration (256^3, 8 views at
+    512^2, 256-entry LUT + opacity scale): Adam steps/s, forward and adjoint
+    ms per view, the loss reduction and the adjoint against FP64 finite
+    differences;
   * served: the render-serving mix over three resident volumes (the
     128^3 blob set, the 512x512x361 cardiac-sized set and the body set above
     with its mask): Poisson arrivals at --serve-qps, 60% VR / 20% MIP /
@@ -85,101 +92,114 @@ def view_params(spec, labels, size: int, n_views: int):
     return ps
 
 
-def fit_main(args) -> int:
-    """Transfer-function fitting: Adam steps per second through the adjoint,
-    loss reduction, and the adjoint against FP64 central differences."""
+def fit_section(args) -> dict:
+    """Transfer-function fitting at its configuration (256^3 16-blob volume,
+    8 views at 512^2, 256-entry RGBA LUT + opacity scale, MSE to a seeded
+    ground truth): Adam steps per second through the hand-written adjoint,
+    the forward and adjoint times per 512^2 view from HIP events, the loss
+    reduction, and the adjoint against FP64 central differences (h = 1e-4) on
+    64 live LUT entries and the scale, with the loss over rows 252..259 of
+    every 512^2 view."""
     import torch
 
-    from monkeygl_amd import params as P
     from monkeygl_amd import workloads as W
     from monkeygl_amd._capi import Context
     from monkeygl_amd.tf_fit import fit, render_tf
     from oracle import oracle as O
+    from oracle.fit_check import fd_check
 
     spec = W.TF_FIT
     vol = W.make_volume(spec)
     ctx = Context(0)
-    ctx.set_volume(vol)
-    ctx.synchronize()
-    n_lut, size = 256, 512
+    try:
+        ctx.set_volume(vol)
+        ctx.synchronize()
+        n_lut, size = W.FIT_LUT_ENTRIES, 512
+        vs = W.fit_views(size)
+        lut_gt, lut0 = W.fit_lut(1), W.fit_lut(2)
+        with torch.no_grad():
+            target = render_tf(ctx, vs, torch.tensor(lut_gt, device="cuda"), torch.tensor(W.FIT_SCALE_TRUE))
+        fit(ctx, vs, target, torch.tensor(lut0, device="cuda"), W.FIT_SCALE_START, steps=2)   # warm
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, s_f, losses = fit(ctx, vs, target, torch.tensor(lut0, device="cuda"), W.FIT_SCALE_START,
+                             steps=args.fit_steps, lr=0.01)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        log(f"fit: {args.fit_steps} Adam steps in {el:.2f}s; loss {losses[0]:.4g} -> {losses[-1]:.4g}")
 
-    def views(sz):
-        # generic angles: at multiples of 90 degrees whole pixel columns run
-        # exactly on a volume face, where float positions (the reference's
-        # and the kernel's) and the FP64 check place samples on opposite
-        # sides of the face
-        out = []
-        for k in range(8):
-            info = P.RenderInfo()
-            info.set_vr_size(sz, sz)
-            info.rotate(45.0 * k + 15.0, 20.0 * (k % 3) - 17.0)
-            out.append(P.render_params(spec.dims, spec.spacing, info, {0: P.LabelState(1.0, 2200.0, 0.0)},
-                                       width=sz, height=sz))
-        return out
+        # forward and adjoint per view, HIP events on the launch stream
+        stream = torch.cuda.Stream()
+        sh = stream.cuda_stream
+        dl = torch.tensor(lut0, device="cuda")
+        rgb = torch.empty((len(vs), size, size, 3), dtype=torch.float32, device="cuda")
+        rgb64 = torch.empty((len(vs), size, size, 3), dtype=torch.float64, device="cuda")
+        g = torch.randn((len(vs), size, size, 3), generator=torch.Generator(device="cuda").manual_seed(3),
+                        device="cuda")
+        g_lut = torch.zeros(n_lut * 4, dtype=torch.float64, device="cuda")
+        g_s = torch.zeros(1, dtype=torch.float64, device="cuda")
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        reps = 3
+        torch.cuda.synchronize()
+        ev[0].record(stream)
+        for _ in range(reps):
+            for i, p in enumerate(vs):
+                ctx.tf_forward(p, dl, n_lut, W.FIT_SCALE_START, rgb[i], stream=sh, dev_rgb64=rgb64[i])
+        ev[1].record(stream)
+        for _ in range(reps):
+            for i, p in enumerate(vs):
+                ctx.tf_backward(p, dl, n_lut, W.FIT_SCALE_START, g[i], g_lut, g_s, stream=sh, dev_rgb64=rgb64[i])
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        fwd_ms = ev[0].elapsed_time(ev[1]) / (reps * len(vs))
+        bwd_ms = ev[1].elapsed_time(ev[2]) / (reps * len(vs))
+        del rgb, rgb64, g
+        log(f"fit: forward {fwd_ms:.3f} ms, adjoint {bwd_ms:.3f} ms per {size}^2 view")
 
-    def lut(seed):
-        rng = np.random.default_rng(seed)
-        a = np.empty((n_lut, 4), np.float32)
-        a[:, :3] = rng.uniform(0.1, 1.0, (n_lut, 3))
-        a[:, 3] = rng.uniform(0.0, 0.08, n_lut)
-        return a
+        fd = None
+        if not args.no_cpu:
+            O.set_threads(cpu_threads()[0])
+            t = time.perf_counter()
+            r = fd_check(ctx, vol, vs, lut_gt, lut0, W.FIT_SCALE_TRUE, W.FIT_SCALE_START, rows=(252, 260))
+            fd = {"entries": r["checked"], "compared": r["big"], "live_entries": r["live"], "h": 1e-4,
+                  "image": [size, size], "views": len(vs), "loss_rows": [252, 260],
+                  "max_rel_err": r["max_rel_err"], "scale_rel_err": r["scale_rel_err"], "bound": 1e-3,
+                  "oracle_s": round(time.perf_counter() - t, 1)}
+            log(f"fit: FD check {fd}")
+        return {"steps_per_s": round(args.fit_steps / el, 3), "steps": args.fit_steps,
+                "ms_per_step": round(el * 1e3 / args.fit_steps, 3), "forward_ms_per_view": round(fwd_ms, 4),
+                "adjoint_ms_per_view": round(bwd_ms, 4), "views": len(vs), "image": [size, size],
+                "volume": list(spec.dims), "lut_entries": n_lut, "loss_first": losses[0], "loss_last": losses[-1],
+                "loss_ratio": losses[-1] / losses[0], "scale_final": s_f, "fd_check": fd}
+    finally:
+        ctx.close()
 
-    lut_gt, lut0, s_gt, s0 = lut(1), lut(2), 1.2, 1.5
-    vs = views(size)
-    with torch.no_grad():
-        target = render_tf(ctx, vs, torch.tensor(lut_gt, device="cuda"), torch.tensor(s_gt))
-    fit(ctx, vs, target, torch.tensor(lut0, device="cuda"), s0, steps=2)   # warm
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    lut_f, s_f, losses = fit(ctx, vs, target, torch.tensor(lut0, device="cuda"), s0, steps=args.fit_steps, lr=0.01)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t
-    log(f"{args.fit_steps} Adam steps in {el:.2f}s; loss {losses[0]:.4g} -> {losses[-1]:.4g}; scale {s_f:.4f}")
 
-    # adjoint vs FP64 central differences (h = 1e-4) on 64 live entries, 64^2 views
-    small = views(64)
-    tgt = np.stack([O.fit_render(p, vol, lut_gt.astype(np.float64), s_gt) for p in small])
-    lt = torch.tensor(lut0, device="cuda", requires_grad=True)
-    sc = torch.tensor(s0, dtype=torch.float32, requires_grad=True)
-    img = render_tf(ctx, small, lt, sc)
-    loss = torch.mean((img - torch.tensor(tgt, dtype=torch.float32, device="cuda")) ** 2)
-    loss.backward()
-    g_an = lt.grad.detach().cpu().numpy().astype(np.float64).reshape(-1)
-    ncores = min(len(os.sched_getaffinity(0)), 16)
-    O.set_threads(ncores)
-
-    def l64(l, s):
-        return np.mean((np.stack([O.fit_render(p, vol, l, s) for p in small]) - tgt) ** 2)
-
-    base, h = lut0.astype(np.float64), 1e-4
-    rng = np.random.default_rng(7)
-    live = np.flatnonzero(np.abs(g_an) > 1e-5 * np.abs(g_an).max())
-    idx = rng.choice(live, size=min(64, live.size), replace=False)
-    rel = []
-    for flat in idx:
-        lp, lm = base.copy(), base.copy()
-        lp.flat[flat] += h
-        lm.flat[flat] -= h
-        fd = (l64(lp, s0) - l64(lm, s0)) / (2 * h)
-        rel.append(abs(g_an[flat] - fd) / max(abs(fd), 1e-30))
-    gs_fd = (l64(base, s0 + h) - l64(base, s0 - h)) / (2 * h)
-    rel_s = abs(float(sc.grad) - gs_fd) / abs(gs_fd)
-    log(f"FD check: {len(idx)} entries, max rel err {max(rel):.3e}; scale rel err {rel_s:.3e}")
+def fit_main(args) -> int:
+    """--fit: the fitting block alone, as its own JSON line."""
+    f = fit_section(args)
     line = {
         "metric": "TF fitting Adam steps/s (adjoint), 256^3 i16, 8 views 512^2, 256-entry LUT + opacity scale",
-        "value": round(args.fit_steps / el, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.fit_steps,
-        "warmup": 2, "ms_per_step": round(el * 1e3 / args.fit_steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 render, fp64 gradient accumulation",
-        "data": "synthetic",
+        "value": f["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": f["steps"], "warmup": 2,
+        "ms_per_step": f["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 render, fp64 gradient accumulation", "data": "synthetic",
         "config": {"workload": "TF fitting: 256^3 16-blob volume, 8 views at 512^2, MSE to a seeded ground-truth TF",
-                   "volume": list(spec.dims), "views": 8, "image": [size, size], "lut_entries": n_lut},
-        "loss_first": losses[0], "loss_last": losses[-1], "loss_ratio": losses[-1] / losses[0],
-        "fd_check": {"entries": int(len(idx)), "h": h, "image": [64, 64], "max_rel_err": float(max(rel)),
-                     "scale_rel_err": float(rel_s), "bound": 1e-3},
+                   "volume": f["volume"], "views": f["views"], "image": f["image"], "lut_entries": f["lut_entries"]},
+        "fit": f,
     }
     print(json.dumps(line), flush=True)
-    ctx.close()
     return 0
+
+
+def cpu_threads():
+    """(threads, note): OpenMP threads for the CPU legs. OMP_NUM_THREADS when
+    set (the GPU box sets it to its CPU share for one GPU), else every core
+    this process may run on."""
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env), f"OMP_NUM_THREADS={env}"
+    return aff, "all cores in the affinity mask"
 
 
 def series_section(ctx, spec, n_images: int = 256, size: int = 512):
@@ -523,15 +543,19 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--views", type=int, default=12)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="time budget of the float CPU baseline (whole frames of the views)")
+    ap.add_argument("--cpu-seconds-fp64", type=float, default=8.0,
+                    help="time budget of the FP64 oracle timed beside it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-footprint", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed frames, copy this many MiB device-to-device once (a known-size "
                          "kernel against which FETCH_SIZE/WRITE_SIZE counters are scaled)")
     ap.add_argument("--fit", action="store_true",
-                    help="measure transfer-function fitting instead (256^3 blob volume, 8 views at 512^2, "
-                         "256-entry LUT + opacity scale, 200 Adam steps, FD check of the adjoint at 64^2)")
+                    help="measure only transfer-function fitting (256^3 blob volume, 8 views at 512^2, "
+                         "256-entry LUT + opacity scale, Adam steps, FD check of the adjoint at 512^2)")
+    ap.add_argument("--no-fit", action="store_true", help="skip the fitting block of the default line")
     ap.add_argument("--fit-steps", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-qps", type=float, default=500.0)
@@ -669,50 +693,73 @@ def main() -> int:
         enc.encode_async(out, W_, H_, 3, stream=sh)
     e1.record(stream)
     torch.cuda.synchronize()
-    encode = {"gpu_png_base64_ms": round(e0.elapsed_time(e1) / 10, 4), "png_bytes": int(enc.lens[0].item()),
-              "base64_bytes": int(enc.lens[1].item()), "image": [W_, H_, 3]}
+    png_n, b64_n = enc.lengths()
+    encode = {"gpu_png_base64_ms": round(e0.elapsed_time(e1) / 10, 4), "png_bytes": png_n, "base64_bytes": b64_n,
+              "image": [W_, H_, 3]}
     log(f"device PNG+base64 of a {W_}x{H_} frame: {encode}")
     del enc
 
-    # parity band and CPU baseline on view 1 (an oblique one)
+    # parity (one whole frame) and CPU baseline on view 1 (an oblique one)
     p = views[1 % len(views)]
     ctx.render_async(p, out, stream=torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     gpu_img = out.cpu().numpy().reshape(H_, W_, 3)
-    ncores = min(len(os.sched_getaffinity(0)), 16)
-    O.set_threads(ncores)
+    threads, thread_note = cpu_threads()
+    O.set_threads(threads)
     parity = None
     cpu = None
     if not args.no_cpu:
-        # rows through the middle of the frame: the volume covers them fully
-        band = 16
-        y0 = H_ // 2 - band // 2
         t = time.perf_counter()
-        ref = O.render_rows(p, vol, luts, y0, y0 + band, mask_zyx=mask, precision="f64")
+        ref = O.render(p, vol, luts, mask_zyx=mask, precision="f64")
         t_ref = time.perf_counter() - t
-        d = np.abs(gpu_img[y0:y0 + band].astype(int) - ref.astype(int)).max(-1)
-        parity = {"oracle": "fp64", "rows": band, "frac_within_1_255": float((d <= 1).mean()),
-                  "max_diff": int(d.max()), "oracle_s": round(t_ref, 2)}
-        log(f"parity on {band} rows: {parity}")
-        # CPU baseline: float instance with 8-bit weights over whole frames of the same views
-        # (bands of 64 rows) until the time budget is spent
-        rows_done, t_cpu, nviews = 0, 0.0, 0
-        band_rows = 64
-        for q in views:
-            for y in range(0, H_, band_rows):
-                t = time.perf_counter()
-                O.render_rows(q, vol, luts, y, min(H_, y + band_rows), mask_zyx=mask, precision="f32", weights8=True)
-                t_cpu += time.perf_counter() - t
-                rows_done += min(H_, y + band_rows) - y
-                if t_cpu >= args.cpu_seconds:
+        d = np.abs(gpu_img.astype(int) - ref.astype(int)).max(-1)
+        parity = {"oracle": "fp64", "image": [W_, H_], "view": 1 % len(views),
+                  "frac_within_1_255": float((d <= 1).mean()), "pixels_beyond_1_255": int((d > 1).sum()),
+                  "max_diff": int(d.max()), "oracle_s": round(t_ref, 2),
+                  "beyond_cause": None if not (d > 1).any() else
+                  "rays whose per-label opacity cap or 0.001 refinement threshold is decided differently by "
+                  "float and double rounding of the accumulated values, so they composite one sample more or fewer"}
+        log(f"parity on the whole frame: {parity}")
+        # CPU baseline: the oracle compiled for this host (-O3 -march=native
+        # -ffp-contract=off -fopenmp), whole frames of the same views, for
+        # the float instance with 8-bit filter weights (the reference's
+        # arithmetic) and for the FP64 instance, each until its time budget
+        import tempfile
+        tmpd = tempfile.mkdtemp(prefix="mk_oracle_")
+        try:
+            O.use_library(O.build_native(tmpd))
+            build = "-O3 -march=native -ffp-contract=off -fopenmp, built on this host"
+        except Exception as e:   # no compiler: the package's x86-64-v3 build
+            build = f"-O3 -march=x86-64-v3 -ffp-contract=off -fopenmp (native build failed: {e})"
+        O.set_threads(threads)
+
+        def frames_per_s(precision, weights8, budget):
+            rows, spent, nv = 0, 0.0, 0
+            for q in views:
+                for y in range(0, H_, 64):
+                    t0 = time.perf_counter()
+                    O.render_rows(q, vol, luts, y, min(H_, y + 64), mask_zyx=mask, precision=precision,
+                                  weights8=weights8)
+                    spent += time.perf_counter() - t0
+                    rows += min(H_, y + 64) - y
+                    if spent >= budget:
+                        break
+                nv += 1
+                if spent >= budget:
                     break
-            nviews += 1
-            if t_cpu >= args.cpu_seconds:
-                break
-        cpu_fps = (rows_done / H_) / t_cpu
-        cpu = {"value": round(cpu_fps, 4), "unit": "frames/s", "cores": ncores, "kind": "port",
-               "sample": f"{rows_done / H_:.2f} frames ({rows_done} rows of {W_}x{H_} over {nviews} views), float "
-                         f"oracle with 8-bit filter weights, {t_cpu:.1f}s on {ncores} threads"}
+            return rows / H_ / spent, rows, spent, nv
+
+        f32, rows, spent, nv = frames_per_s("f32", True, args.cpu_seconds)
+        f64, rows64, spent64, _ = frames_per_s("f64", False, args.cpu_seconds_fp64)
+        O.use_library(None)
+        O.set_threads(threads)
+        cpu = {"value": round(f32, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": f"{rows / H_:.2f} frames ({rows} rows of {W_}x{H_} over {nv} views), float oracle with "
+                         f"8-bit filter weights, {spent:.1f}s on {threads} threads",
+               "threads_from": thread_note, "nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)),
+               "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "build": build,
+               "fp64_oracle": {"value": round(f64, 4), "unit": "frames/s",
+                               "sample": f"{rows64 / H_:.2f} frames, {spent64:.1f}s on {threads} threads"}}
         log(f"cpu baseline: {cpu}")
         # the reference's own encoder (core/fpng/fpng.cpp, built into oracle/_ref) on the same frame
         ref_lib = ROOT / "oracle" / "_ref" / "libmkref.so"
@@ -731,6 +778,7 @@ def main() -> int:
             encode["fpng_cores"] = 1
             log(f"fpng on the host: {encode}")
 
+    fitb = None if args.no_fit else fit_section(args)
     series = series_section(ctx, spec)
     mask_ingest = mask_ingest_section(spec, vol, mask)
     overlay = overlay_section(ctx, W_, not args.no_cpu)
@@ -772,6 +820,7 @@ def main() -> int:
         "parity": parity,
         "cpu_baseline": cpu,
         "encode": encode,
+        "fit": fitb,
         "series": series,
         "mask_ingest": mask_ingest,
         "orientation_figure": overlay,

@@ -146,6 +146,17 @@ void* mk_context_stream(mk_context* ctx);
 /* Blocks until all work queued on the context's stream is done. */
 mk_status mk_synchronize(mk_context* ctx);
 
+/* ---- device memory for bindings (no reference counterpart) ------------- */
+/* For callers that hold no device-array library of their own (the pyMonkeyGL
+ * module, plain C): memory on the context's device, and copies on the
+ * context's stream that return when they are done (after the work queued
+ * before them on that stream, e.g. an _async call given mk_context_stream).
+ * mk_device_free waits for the context's stream before it frees. */
+mk_status mk_device_alloc(mk_context* ctx, uint64_t bytes, void** dev_out);
+mk_status mk_device_free(mk_context* ctx, void* dev);
+mk_status mk_copy_to_host(mk_context* ctx, void* host, const void* dev, uint64_t bytes);
+mk_status mk_copy_to_device(mk_context* ctx, void* dev, const void* host, uint64_t bytes);
+
 /* ---- resident data ----------------------------------------------------- */
 
 /* cu_copyVolumeData (core/CuDataManager.cu:28-73). host is int16, x fastest
@@ -332,6 +343,38 @@ mk_status mk_merge_orientation_box_async(mk_context* ctx, const float* rotate9, 
  * Fewer than 2 points draw nothing. */
 mk_status mk_draw_polyline_async(mk_context* ctx, const float* dev_xy, int n, const float* color3, uint8_t* dev_rgb,
                                  int width, int height, void* stream);
+
+/* ---- VR overlay: annotation text ---------------------------------------- */
+
+/* core/Defines.h FontSize and AnnotationFormat. */
+typedef enum { MK_FONT_NONE = -1, MK_FONT_SMALL = 0, MK_FONT_MIDDLE = 1, MK_FONT_BIG = 2 } mk_font_size;
+typedef enum { MK_ANNO_NONE = -1, MK_ANNO_LEFT = 0, MK_ANNO_RIGHT = 1, MK_ANNO_CENTER = 2 } mk_anno_format;
+
+/* The coverage mask of one character (a byte) in one font size: width x
+ * height bytes, row-major, as AnnotationUtils keeps its character masks
+ * (core/AnnotationUtils.cpp:42-376, Mask in core/AnnotationUtils.h:31-46).
+ * The reference compiles its masks into its library; here the caller
+ * registers them (the same data, or any font). width and height are
+ * 1..1024; width 0 removes the glyph. A byte without a glyph behaves as the
+ * reference's default mask: width and height -1, so it draws nothing and
+ * moves the text cursor one pixel left. */
+mk_status mk_set_glyph(mk_context* ctx, int font_size, int code, int width, int height, const uint8_t* mask);
+
+/* AnnotationUtils::Textout2Image (core/AnnotationUtils.cpp:419-464) with the
+ * font size set first (AnnotationUtils::SetFontSize), as
+ * AnnotationLayer::GetRGBData draws each annotation (core/AnnotationLayer.cpp:
+ * 41-61), on an RGB24 device image on the given stream. text is a
+ * NUL-terminated byte string, one glyph per byte; (x, y) is the top-left of
+ * the text, moved left by its width (the sum of its glyph widths, -1 per
+ * byte without one) for MK_ANNO_RIGHT and by half of it (integer division)
+ * for MK_ANNO_CENTER. Every glyph pixel inside the image whose mask byte m is
+ * nonzero becomes m * color3 per channel, computed in float and stored
+ * truncated toward zero, modulo 256 (colour components in [0, 1] keep it in
+ * range). Later glyphs overwrite earlier ones where they overlap. Nothing is
+ * drawn when width or height is below 10, for MK_ANNO_NONE, or for
+ * MK_FONT_NONE. */
+mk_status mk_draw_text_async(mk_context* ctx, const char* text, int x, int y, int font_size, int format,
+                             const float* color3, uint8_t* dev_rgb, int width, int height, void* stream);
 
 /* ---- output encoding: PNG + base64 on the device ------------------------- */
 /* Replaces the host encoding of served images: fpng_encode_image_to_memory

@@ -6,6 +6,9 @@ library has not been built: there is no CPU fallback anywhere in the product.
 from __future__ import annotations
 
 import ctypes as C
+import importlib.util
+import sys
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -25,13 +28,14 @@ MK_MASK_ADD, MK_MASK_UPDATE = range(2)
 # exported symbols, in header order
 SYMBOLS = [
     "mk_device_count", "mk_context_create", "mk_context_destroy", "mk_last_error",
-    "mk_context_stream", "mk_synchronize", "mk_copy_volume_data", "mk_copy_volume_data_device",
+    "mk_context_stream", "mk_synchronize", "mk_device_alloc", "mk_device_free", "mk_copy_to_host",
+    "mk_copy_to_device", "mk_copy_volume_data", "mk_copy_volume_data_device",
     "mk_copy_mask_data", "mk_copy_mask_data_device", "mk_clear_mask", "mk_merge_object_mask",
     "mk_merge_object_mask_device", "mk_get_mask_data", "mk_origin_slice_async", "mk_set_transfer_func",
     "mk_render", "mk_render_async", "mk_set_vr_trip_budget", "mk_render_footprint", "mk_render_plane", "mk_render_plane_async",
     "mk_render_cpr", "mk_render_cpr_async", "mk_marching_cubes", "mk_marching_cubes_async", "mk_marching_cubes_noalloc",
     "mk_tf_forward", "mk_tf_backward", "mk_set_orientation_mesh", "mk_orientation_workspace_bytes",
-    "mk_merge_orientation_box_async", "mk_draw_polyline_async", "mk_png_max_bytes", "mk_png_workspace_bytes", "mk_png_encode_async",
+    "mk_merge_orientation_box_async", "mk_draw_polyline_async", "mk_set_glyph", "mk_draw_text_async", "mk_png_max_bytes", "mk_png_workspace_bytes", "mk_png_encode_async",
     "mk_base64_max_bytes", "mk_base64_encode_async", "mk_batcher_create", "mk_batcher_destroy", "mk_batcher_last_error",
     "mk_batch_submit", "mk_batch_query", "mk_batch_wait",
 ]
@@ -88,15 +92,23 @@ def load() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is not built; run `python -m monkeygl_amd.build`")
-    # One HIP runtime per process: torch's bundled runtime has the same SONAME
-    # (libamdhip64.so.7) as ROCm's, but torch asks for it as libamdhip64.so. If
-    # this library loaded ROCm's copy first, torch would load a second runtime
-    # that cannot see the GPU. Importing torch first makes the loader hand our
-    # library the runtime torch already holds.
-    try:
-        import torch  # noqa: F401
-    except ImportError:
-        pass
+    # One HIP runtime per process. torch ships its own copy of the runtime with
+    # the same SONAME (libamdhip64.so.7) as ROCm's, and asks for it as
+    # libamdhip64.so: if this library bound ROCm's copy, a later `import torch`
+    # would load a second runtime that cannot see the GPU. So when torch is
+    # installed and not yet imported, its copy is loaded first, by path,
+    # without importing torch: this library binds to it by SONAME, and torch,
+    # when imported, finds the same file already loaded. Without torch the
+    # library binds ROCm's runtime.
+    if "torch" not in sys.modules:
+        try:
+            spec = importlib.util.find_spec("torch")
+        except (ImportError, ValueError):
+            spec = None
+        if spec is not None and spec.origin:
+            rt = Path(spec.origin).resolve().parent / "lib" / "libamdhip64.so"
+            if rt.exists():
+                C.CDLL(str(rt), mode=C.RTLD_GLOBAL)
     lib = C.CDLL(str(LIB_PATH))
     vp, i, st = C.c_void_p, C.c_int, C.c_int
     sig = {
@@ -106,6 +118,10 @@ def load() -> C.CDLL:
         "mk_last_error": (C.c_char_p, [vp]),
         "mk_context_stream": (vp, [vp]),
         "mk_synchronize": (st, [vp]),
+        "mk_device_alloc": (st, [vp, C.c_uint64, C.POINTER(vp)]),
+        "mk_device_free": (st, [vp, vp]),
+        "mk_copy_to_host": (st, [vp, vp, vp, C.c_uint64]),
+        "mk_copy_to_device": (st, [vp, vp, vp, C.c_uint64]),
         "mk_copy_volume_data": (st, [vp, vp, i, i, i]),
         "mk_copy_volume_data_device": (st, [vp, vp, i, i, i]),
         "mk_copy_mask_data": (st, [vp, vp, i, i, i]),
@@ -115,6 +131,8 @@ def load() -> C.CDLL:
         "mk_orientation_workspace_bytes": (C.c_uint64, [vp]),
         "mk_merge_orientation_box_async": (st, [vp, vp, vp, i, i, vp, C.c_uint64, vp]),
         "mk_draw_polyline_async": (st, [vp, vp, i, vp, vp, i, i, vp]),
+        "mk_set_glyph": (st, [vp, i, i, i, i, vp]),
+        "mk_draw_text_async": (st, [vp, C.c_char_p, i, i, i, i, vp, vp, i, i, vp]),
         "mk_merge_object_mask": (st, [vp, vp, i, i, i, i, i]),
         "mk_merge_object_mask_device": (st, [vp, vp, i, i, i, i, i]),
         "mk_get_mask_data": (st, [vp, vp]),
@@ -160,7 +178,75 @@ def _ptr(a) -> int:
         return a
     if isinstance(a, np.ndarray):
         return a.ctypes.data
-    return int(a.data_ptr())  # torch tensor on the device
+    return int(a.data_ptr())  # DeviceBuffer, or a device array that exposes data_ptr()
+
+
+def _nbytes(a) -> int:
+    if isinstance(a, DeviceBuffer):
+        return a.nbytes
+    return int(a.numel() * a.element_size())
+
+
+class DeviceBuffer:
+    """Device memory of a context's device (mk_device_alloc), for callers that
+    use no device-array library. Copies run on the context's stream and
+    return when done, after the work queued before them there. The context
+    frees its live buffers before it is destroyed."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        if nbytes <= 0:
+            raise ValueError("a device buffer needs at least one byte")
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        h = C.c_void_p()
+        ctx._check(ctx.lib.mk_device_alloc(ctx.h, self.nbytes, C.byref(h)))
+        self.ptr = int(h.value)
+        ctx._buffers.add(self)
+        self._keep = None   # device or host objects a queued call still reads
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def numel(self) -> int:
+        return self.nbytes
+
+    def element_size(self) -> int:
+        return 1
+
+    def to_host(self, dtype=np.uint8, count: int | None = None, offset: int = 0) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = (self.nbytes - offset) // dt.itemsize if count is None else int(count)
+        out = np.empty(n, dt)
+        if n:
+            if offset + n * dt.itemsize > self.nbytes:
+                raise ValueError("read past the end of the device buffer")
+            self.ctx._check(self.ctx.lib.mk_copy_to_host(self.ctx.h, out.ctypes.data, self.ptr + offset,
+                                                         n * dt.itemsize))
+        return out
+
+    def from_host(self, arr: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(arr)
+        if a.nbytes > self.nbytes:
+            raise ValueError("host array larger than the device buffer")
+        self.ctx._check(self.ctx.lib.mk_copy_to_device(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes))
+        return self
+
+    @classmethod
+    def of(cls, ctx: "Context", arr: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(arr)
+        return cls(ctx, max(a.nbytes, 1)).from_host(a)
+
+    def close(self) -> None:
+        if self.ptr and self.ctx.h:
+            self.ctx.lib.mk_device_free(self.ctx.h, self.ptr)
+            self.ctx._buffers.discard(self)
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Context:
@@ -174,9 +260,12 @@ class Context:
             raise MkError(st, self.lib.mk_last_error(None).decode())
         self.h = h
         self.shape = None
+        self._buffers = weakref.WeakSet()
 
     def close(self) -> None:
         if getattr(self, "h", None):
+            for b in list(self._buffers):
+                b.close()
             self.lib.mk_context_destroy(self.h)
             self.h = None
 
@@ -253,7 +342,7 @@ class Context:
     def merge_orientation_box_async(self, rotate9, dev_rgb, width: int, height: int, dev_workspace,
                                     stream: int = 0) -> None:
         m = np.ascontiguousarray(rotate9, dtype=np.float32).reshape(9)
-        ws = dev_workspace.numel() * dev_workspace.element_size()
+        ws = _nbytes(dev_workspace)
         self._check(self.lib.mk_merge_orientation_box_async(self.h, _ptr(m), _ptr(dev_rgb), width, height,
                                                             _ptr(dev_workspace), ws, stream))
 
@@ -261,6 +350,20 @@ class Context:
         c = np.ascontiguousarray(color3, dtype=np.float32).reshape(3)
         self._check(self.lib.mk_draw_polyline_async(self.h, _ptr(dev_xy), int(n), _ptr(c), _ptr(dev_rgb), width, height,
                                                     stream))
+
+    def set_glyph(self, font_size: int, code: int, mask) -> None:
+        """mk_set_glyph: mask is a (height, width) uint8 array, or None to remove the glyph."""
+        if mask is None:
+            self._check(self.lib.mk_set_glyph(self.h, int(font_size), int(code), 0, 0, None))
+            return
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        self._check(self.lib.mk_set_glyph(self.h, int(font_size), int(code), m.shape[1], m.shape[0], _ptr(m)))
+
+    def draw_text_async(self, text: bytes, x: int, y: int, font_size: int, fmt: int, color3, dev_rgb, width: int,
+                        height: int, stream: int = 0) -> None:
+        c = np.ascontiguousarray(color3, dtype=np.float32).reshape(3)
+        self._check(self.lib.mk_draw_text_async(self.h, bytes(text), int(x), int(y), int(font_size), int(fmt),
+                                                _ptr(c), _ptr(dev_rgb), int(width), int(height), stream))
 
     def origin_slice_async(self, z: int, dev_out, stream: int = 0) -> None:
         self._check(self.lib.mk_origin_slice_async(self.h, int(z), _ptr(dev_out), stream))

@@ -34,14 +34,14 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-I", str(ROOT / "include"),
 ]
 # files whose arithmetic must match the host restatement bit for bit
-NO_CONTRACT = {"mk_plane.hip", "mk_mc.hip", "mk_batch_plane.hip", "mk_overlay.hip"}
+NO_CONTRACT = {"mk_plane.hip", "mk_mc.hip", "mk_batch_plane.hip", "mk_overlay.hip", "mk_text.hip"}
 # files whose kernels must agree bit for bit with each other (one- and two-pass
 # VR, plain and batched): multiply-adds are fused only within one source
 # expression, never across expressions by the backend
 CONTRACT_ON = {"mk_render.hip", "mk_batch.hip"}
 
 HIP_SOURCES = ["mk_volume.hip", "mk_render.hip", "mk_plane.hip", "mk_mc.hip", "mk_fit.hip", "mk_batch.hip",
-               "mk_batch_plane.hip", "mk_encode.hip", "mk_overlay.hip"]
+               "mk_batch_plane.hip", "mk_encode.hip", "mk_overlay.hip", "mk_text.hip"]
 
 
 def _run(cmd: list[str]) -> None:

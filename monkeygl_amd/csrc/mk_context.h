@@ -131,6 +131,19 @@ struct mk_context {
     float* d_orient = nullptr;
     int orient_n = 0;
 
+    // annotation glyphs, index font * 256 + byte: width, height (-1 when the
+    // byte has no glyph in that font) and mask bytes, packed into one device
+    // atlas (glyph_off) when a draw finds them changed
+    struct Glyph {
+        int w = -1, h = -1;
+        std::vector<uint8_t> mask;
+    };
+    std::vector<Glyph> glyphs;
+    std::vector<size_t> glyph_off;
+    bool glyphs_dirty = false;
+    uint8_t* d_glyphs = nullptr;
+    size_t glyphs_cap = 0;
+
     // staging for synchronous host-output calls
     void* d_stage = nullptr;
     size_t stage_cap = 0;

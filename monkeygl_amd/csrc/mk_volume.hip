@@ -145,6 +145,7 @@ extern "C" void mk_context_destroy(mk_context* ctx) {
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
     if (ctx->d_stage2) (void)hipFree(ctx->d_stage2);
     if (ctx->d_orient) (void)hipFree(ctx->d_orient);
+    if (ctx->d_glyphs) (void)hipFree(ctx->d_glyphs);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -203,6 +204,48 @@ mk_status mk_wait_readers(mk_context* ctx) {
 extern "C" mk_status mk_synchronize(mk_context* ctx) {
     if (!ctx) return MK_ERR_ARG;
     MK_HIP(ctx, hipSetDevice(ctx->device));
+    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MK_OK;
+}
+
+// ---- device memory for bindings ------------------------------------------
+
+extern "C" mk_status mk_device_alloc(mk_context* ctx, uint64_t bytes, void** dev_out) {
+    if (!ctx) return MK_ERR_ARG;
+    if (!dev_out) return mk_fail(ctx, MK_ERR_ARG, "mk_device_alloc: null out");
+    *dev_out = nullptr;
+    if (bytes == 0) return mk_fail(ctx, MK_ERR_ARG, "mk_device_alloc: zero bytes");
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    MK_HIP(ctx, hipMalloc(dev_out, (size_t)bytes));
+    return MK_OK;
+}
+
+extern "C" mk_status mk_device_free(mk_context* ctx, void* dev) {
+    if (!ctx) return MK_ERR_ARG;
+    if (!dev) return MK_OK;
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    MK_HIP(ctx, hipFree(dev));
+    return MK_OK;
+}
+
+extern "C" mk_status mk_copy_to_host(mk_context* ctx, void* host, const void* dev, uint64_t bytes) {
+    if (!ctx) return MK_ERR_ARG;
+    if (bytes == 0) return MK_OK;
+    if (!host || !dev) return mk_fail(ctx, MK_ERR_ARG, "mk_copy_to_host: null pointer");
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    MK_HIP(ctx, hipMemcpyAsync(host, dev, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+    MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MK_OK;
+}
+
+extern "C" mk_status mk_copy_to_device(mk_context* ctx, void* dev, const void* host, uint64_t bytes) {
+    if (!ctx) return MK_ERR_ARG;
+    if (bytes == 0) return MK_OK;
+    if (!host || !dev) return mk_fail(ctx, MK_ERR_ARG, "mk_copy_to_device: null pointer");
+    MK_HIP(ctx, hipSetDevice(ctx->device));
+    MK_HIP(ctx, hipMemcpyAsync(dev, host, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+    // the host buffer may go away at return
     MK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return MK_OK;
 }

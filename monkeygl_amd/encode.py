@@ -9,7 +9,7 @@ leaves HBM until its (smaller) base64 text is copied back.
 """
 from __future__ import annotations
 
-import ctypes as C
+import numpy as np
 
 from . import _capi
 from ._capi import MK_OK, MkError
@@ -21,24 +21,44 @@ def _check(lib, st: int) -> None:
 
 
 class Encoder:
-    """Device buffers for images up to max_width x max_height x channels."""
+    """Device buffers for images up to max_width x max_height x channels.
 
-    def __init__(self, max_width: int, max_height: int, channels: int, device: str = "cuda"):
-        import torch
+    With a context (``ctx``) the buffers are the context's device memory
+    (mk_device_alloc) and the text comes back through mk_copy_to_host: no
+    device-array library is needed, and callers encode on the context's
+    stream. Without one they are torch tensors on ``device``."""
 
+    def __init__(self, max_width: int, max_height: int, channels: int, device: str = "cuda", ctx=None):
         self.lib = _capi.load()
-        self.torch = torch
+        self.ctx = ctx
+        self.device = device
         self.channels = channels
         self.png_cap = int(self.lib.mk_png_max_bytes(max_width, max_height, channels))
         ws = int(self.lib.mk_png_workspace_bytes(max_width, max_height, channels))
         if self.png_cap == 0 or ws == 0:
             raise ValueError(f"unsupported image {max_width}x{max_height}x{channels}")
         self.b64_cap = int(self.lib.mk_base64_max_bytes(self.png_cap))
-        self.workspace = torch.empty(ws, dtype=torch.uint8, device=device)
-        self.png = torch.empty((self.png_cap + 255) // 256 * 256, dtype=torch.uint8, device=device)
-        self.b64 = torch.empty(self.b64_cap, dtype=torch.uint8, device=device)
-        self.lens = torch.zeros(2, dtype=torch.int64, device=device)   # png length, base64 length
+        self.workspace = self._buffer(ws)
+        self.png = self._buffer((self.png_cap + 255) // 256 * 256)
+        self.b64 = self._buffer(self.b64_cap)
+        self.lens = self._buffer(16)   # png length, base64 length (int64 each)
+        if ctx is not None:
+            self.lens.from_host(np.zeros(2, np.int64))
+        else:
+            self.lens.zero_()
         self.max_width, self.max_height = max_width, max_height
+
+    def _buffer(self, nbytes: int):
+        if self.ctx is not None:
+            return _capi.DeviceBuffer(self.ctx, nbytes)
+        import torch
+
+        return torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+
+    def _host(self, buf, n: int) -> bytes:
+        if self.ctx is not None:
+            return buf.to_host(np.uint8, n).tobytes()
+        return bytes(buf[:n].cpu().numpy())
 
     def encode_async(self, dev_pixels, width: int, height: int, channels: int | None = None, stream: int = 0,
                      base64: bool = True) -> None:
@@ -56,15 +76,23 @@ class Encoder:
                                                              int(self.lib.mk_base64_max_bytes(cap)), lens + 8,
                                                              stream))
 
+    def lengths(self) -> tuple:
+        """(PNG bytes, base64 bytes) of the last encode (synchronises)."""
+        if self.ctx is not None:
+            a = self.lens.to_host(np.int64, 2)
+        else:
+            import torch
+
+            a = self.lens.view(torch.int64).cpu().numpy()
+        return int(a[0]), int(a[1])
+
     def png_bytes(self) -> bytes:
         """The last PNG (synchronises)."""
-        n = int(self.lens[0].item())
-        return bytes(self.png[:n].cpu().numpy())
+        return self._host(self.png, self.lengths()[0])
 
     def base64_text(self) -> str:
         """The last base64 text (synchronises)."""
-        n = int(self.lens[1].item())
-        return bytes(self.b64[:n].cpu().numpy()).decode("ascii")
+        return self._host(self.b64, self.lengths()[1]).decode("ascii")
 
 
 def encode_vr_png_base64(enc: Encoder, dev_rgb, width: int, height: int, stream: int = 0) -> str:

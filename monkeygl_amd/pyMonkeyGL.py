@@ -29,8 +29,11 @@ a list of byte values (pybind11's std::vector<uint8_t>).
 Differences, by design: images come back as PNGs built on the device (same
 pixels as fpng's, other bytes); the mask of a first AddNewObjectMaskArray is
 zero outside the object (the reference leaves that memory uninitialised,
-VolumeInfo.cpp:100-107); the host overlays (orientation box, annotations,
-CPR line, core/VRImageLayer.cpp:104-194) are not drawn; LoadVolumeFile and
+VolumeInfo.cpp:100-107); the VR overlays are drawn on the device in the
+reference's layer order (the orientation figure from ./resources/human.stl,
+then the annotation text, then the CPR line), and the text's glyph masks are
+data (./resources/annotation_glyphs.npz or SetAnnotationGlyph), since the
+reference compiles its own into its library; LoadVolumeFile and
 AddObjectMaskFile need ITK readers, which are not part of this build;
 SaveVR2Png writes the PNG (the reference writes width*height*3 bytes of it).
 """
@@ -301,6 +304,7 @@ class HelloMonkey:
         self._plane_label = 0
         self._dirty_vol = self._dirty_lut = False
         self._encoders: dict = {}
+        self._bufs: dict = {}
         self._series_renderer = None
         # the orientation figure: MarchingCube loads ./resources/human.stl at
         # construction (core/MarchingCube.cpp:297-298); without the file there
@@ -311,6 +315,21 @@ class HelloMonkey:
         except OSError:
             self._orient_stl = None
         self._orient_ws = None
+        # annotation glyphs {(font size, byte): (h, w) uint8 mask}. The
+        # reference compiles its character masks into the library
+        # (AnnotationUtils::Init); here they are data: read from
+        # ./resources/annotation_glyphs.npz when present (arrays named
+        # "f<font>_c<byte>"), or registered with SetAnnotationGlyph
+        self._glyphs: dict = {}
+        self._glyphs_dirty = False
+        try:
+            with np.load("./resources/annotation_glyphs.npz", allow_pickle=False) as z:
+                for name in z.files:
+                    f, c = name[1:].split("_c")
+                    self._glyphs[(int(f), int(c))] = np.ascontiguousarray(z[name], np.uint8)
+            self._glyphs_dirty = True
+        except (OSError, ValueError):
+            pass
 
     # -- device sync ----------------------------------------------------------
     def _device(self):
@@ -324,6 +343,10 @@ class HelloMonkey:
                 except ValueError:
                     self._orient_stl = None   # not a binary STL: no figure, as the reference logs and goes on
         c = self._ctx
+        if self._glyphs_dirty:
+            for (font, code), m in self._glyphs.items():
+                c.set_glyph(font, code, m)
+            self._glyphs_dirty = False
         if self._vol is None:
             raise RuntimeError("no volume: call SetVolumeArray first")
         W, H, D = self._dims
@@ -358,8 +381,23 @@ class HelloMonkey:
 
         key = (w, h, ch)
         if key not in self._encoders:
-            self._encoders[key] = Encoder(w, h, ch)
+            self._encoders[key] = Encoder(w, h, ch, ctx=self._device())
         return self._encoders[key]
+
+    def _dbuf(self, name: str, nbytes: int):
+        """A device buffer of the context, kept per use and grown on demand.
+        Every device call of this module runs on the context's stream and the
+        results come back through synchronous copies, so a buffer is free
+        again when the call that used it returns."""
+        from ._capi import DeviceBuffer
+
+        b = self._bufs.get(name)
+        if b is None or b.nbytes < nbytes or b.ctx is not self._ctx:
+            if b is not None:
+                b.close()
+            b = DeviceBuffer(self._ctx, max(int(nbytes), 1))
+            self._bufs[name] = b
+        return b
 
     # -- settings ---------------------------------------------------------------
     def SetLogLevel(self, level) -> None:
@@ -649,7 +687,7 @@ class HelloMonkey:
     def SetCPRLineColor(self, clr) -> None:
         self._cpr_color = clr
 
-    # -- layers and annotations (annotation text keeps its state only) ---------------------
+    # -- layers and annotations ------------------------------------------------------------
     def ShowPlaneInVR(self, bShow: bool) -> None:
         """DataManager::ShowPlaneInVR (core/DataManager.cpp:769-812): three slabs
         added as a label at 0.6 alpha."""
@@ -667,6 +705,21 @@ class HelloMonkey:
     def AddAnnotation(self, planeType, txt, x, y, fontSize, annoFormat, clr) -> bool:
         self._annotations.append((PlaneType.PlaneVR, txt, x, y, fontSize, annoFormat, clr))
         return True
+
+    def SetAnnotationGlyph(self, fontSize, char, mask) -> None:
+        """The mask of one character (a one-byte string or a byte value) in one
+        font size, (height, width) uint8; None removes it. No reference
+        counterpart: the reference's masks are compiled into its library."""
+        code = char if isinstance(char, int) else ord(char.encode("latin-1") if isinstance(char, str) else char)
+        key = (int(fontSize), int(code))
+        if mask is None:
+            self._glyphs.pop(key, None)
+        else:
+            self._glyphs[key] = np.ascontiguousarray(mask, np.uint8)
+        if self._ctx is not None:
+            self._ctx.set_glyph(key[0], key[1], None if mask is None else self._glyphs[key])
+        else:
+            self._glyphs_dirty = True   # registered with the context when it is made
 
     def RemovePlaneAnnotations(self, planeType) -> bool:
         self._annotations = [a for a in self._annotations if a[0] != int(planeType)]
@@ -686,27 +739,31 @@ class HelloMonkey:
     # -- output ------------------------------------------------------------------------
     def _vr_device(self):
         """The VR frame at the VR size in a device buffer (cu_render's path)."""
-        import torch
+        from ._capi import DeviceBuffer
 
         c = self._device()
         w, h = self._ri.width, self._ri.height
         p = self._render_params(w, h)
-        out = torch.empty(w * h * 3, dtype=torch.uint8, device="cuda")
-        st = torch.cuda.current_stream().cuda_stream
+        out = self._dbuf("vr", w * h * 3)
+        st = c.stream
         c.render_async(p, out, stream=st)
         if self._orient_stl:
             # VRImageLayer::GetRGBData ends with MergeOrientationBox (core/VRImageLayer.cpp:99)
-            nb = c.orientation_workspace_bytes()
-            if self._orient_ws is None or self._orient_ws.numel() < nb:
-                self._orient_ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
-            c.merge_orientation_box_async(self._ri.rot, out, w, h, self._orient_ws, stream=st)
-        # the annotation layer (text) keeps its state only; the CPR line layer
-        # draws the CPR line's voxel points projected into the image
+            ws = self._dbuf("orient", c.orientation_workspace_bytes())
+            c.merge_orientation_box_async(self._ri.rot, out, w, h, ws, stream=st)
+        # AnnotationLayer::GetRGBData (core/AnnotationLayer.cpp:41-61): the VR
+        # plane's annotations in order, on frames wider and taller than 10
+        if self._layers.get(int(LayerType.LayerTypeAnnotation), True) and w > 10 and h > 10:
+            for plane, txt, x, y, fs, fmt, clr in self._annotations:
+                if plane == PlaneType.PlaneVR:
+                    c.draw_text_async(str(txt).encode("utf-8"), int(x), int(y), int(fs), int(fmt),
+                                      clr.tuple()[:3], out, w, h, stream=st)
+        # the CPR line layer draws the CPR line's voxel points projected into the image
         line = self._cprinfo.line
         if self._layers.get(int(LayerType.LayerTypeCPRLine), True) and len(line) >= 2:
-            xy = torch.tensor(self._voxel_to_vr(line, w, h), device="cuda")
+            xy = DeviceBuffer.of(c, self._voxel_to_vr(line, w, h))
             c.draw_polyline_async(xy, len(line), self._cpr_color.tuple()[:3], out, w, h, stream=st)
-            out._mk_keep = xy
+            out._keep = xy
         return out, w, h
 
     def _voxel_to_vr(self, pts, w: int, h: int) -> np.ndarray:
@@ -737,24 +794,20 @@ class HelloMonkey:
         if nWidth * nHeight != w * h:
             raise ValueError(f"GetVRArray reads the VR frame ({w}x{h}); ask for a shape of the same size")
         out, w, h = self._vr_device()
-        return out.cpu().numpy().reshape(3, nWidth, nHeight)
+        return out.to_host(np.uint8, w * h * 3).reshape(3, nWidth, nHeight)
 
     def _vr_png(self) -> bytes:
-        import torch
-
         out, w, h = self._vr_device()
         enc = self._encoder(w, h, 3)
-        enc.encode_async(out, w, h, 3, stream=torch.cuda.current_stream().cuda_stream, base64=False)
+        enc.encode_async(out, w, h, 3, stream=self._ctx.stream, base64=False)
         return enc.png_bytes()
 
     def GetVRData_pngString(self, *ignored) -> str:
         if not self._image_layer():
             return ""
-        import torch
-
         out, w, h = self._vr_device()
         enc = self._encoder(w, h, 3)
-        enc.encode_async(out, w, h, 3, stream=torch.cuda.current_stream().cuda_stream)
+        enc.encode_async(out, w, h, 3, stream=self._ctx.stream)
         return enc.base64_text()
 
     def GetVRData_png(self, nWidth: int = 0, nHeight: int = 0) -> list:
@@ -768,14 +821,14 @@ class HelloMonkey:
 
     def _plane_device(self, planeType):
         """(int16 device image, width, height) of an MPR or CPR plane, or None."""
-        import torch
+        from ._capi import DeviceBuffer
 
         p = int(planeType)
         c = self._device()
-        st = torch.cuda.current_stream().cuda_stream
+        st = c.stream
         if PlaneType.PlaneAxial <= p <= PlaneType.PlaneCoronalOblique:
             q = self._mpr_state().plane_params(p, invert_z=self._need_invert_z(), sampling=MK_SAMPLE_LINEAR)
-            out = torch.empty(q.width * q.height, dtype=torch.int16, device="cuda")
+            out = self._dbuf("plane", q.width * q.height * 2)
             c.render_plane_async(q, out, stream=st)
             return out, q.width, q.height
         if p in (PlaneType.PlaneStretchedCPR, PlaneType.PlaneStraightenedCPR):
@@ -784,11 +837,11 @@ class HelloMonkey:
             if g is None:
                 return None
             pts, dirs, w, h = g
-            dp = torch.tensor(pts, dtype=torch.float64, device="cuda")
-            dd = torch.tensor(dirs, dtype=torch.float64, device="cuda")
-            out = torch.empty(w * h, dtype=torch.int16, device="cuda")
+            dp = DeviceBuffer.of(c, np.asarray(pts, np.float64))
+            dd = DeviceBuffer.of(c, np.asarray(dirs, np.float64))
+            out = self._dbuf("plane", w * h * 2)
             c.render_cpr_async(w, h, dp, dd, self._need_invert_z(), MK_SAMPLE_LINEAR, out, stream=st)
-            out._mk_keep = (dp, dd)
+            out._keep = (dp, dd)
             return out, w, h
         return None
 
@@ -798,13 +851,11 @@ class HelloMonkey:
         if r is None:
             return None
         out, w, h = r
-        return out.cpu().numpy().reshape(h, w), w, h
+        return out.to_host(np.int16, w * h).reshape(h, w), w, h
 
     def _int16_png_b64(self, dev, w, h) -> str:
-        import torch
-
         enc = self._encoder(w // 2, h, 4)
-        enc.encode_async(dev, w // 2, h, 4, stream=torch.cuda.current_stream().cuda_stream)
+        enc.encode_async(dev, w // 2, h, 4, stream=self._ctx.stream)
         return enc.base64_text()
 
     def GetPlaneData_pngString(self, planeType) -> str:
@@ -819,12 +870,10 @@ class HelloMonkey:
     def GetOriginData_pngString(self, slice_: int) -> str:
         """An original axial slice, -2048 outside the mask when there is one
         (core/HelloMonkey.cpp:261-319)."""
-        import torch
-
         W, H, D = self._dims
         c = self._device()
-        dev = torch.empty(W * H, dtype=torch.int16, device="cuda")
-        c.origin_slice_async(int(slice_), dev, stream=torch.cuda.current_stream().cuda_stream)
+        dev = self._dbuf("origin", W * H * 2)
+        c.origin_slice_async(int(slice_), dev, stream=c.stream)
         return self._int16_png_b64(dev, W, H)
 
     # -- batch series (core/HelloMonkey.cpp:415-420, 614-619) ---------------------

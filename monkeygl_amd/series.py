@@ -42,8 +42,9 @@ class SeriesRenderer:
             raise ValueError("the images of a series share one size")
         img_bytes = w * h * 2
         need = len(params) * img_bytes
-        if dev_out.numel() * dev_out.element_size() < need:
-            raise ValueError(f"output holds {dev_out.numel() * dev_out.element_size()} bytes; the series needs {need}")
+        have = _capi._nbytes(dev_out)
+        if have < need:
+            raise ValueError(f"output holds {have} bytes; the series needs {need}")
         tiles_per = ((w + 15) // 16) * ((h + 15) // 16)
         if tiles_per > self.max_tiles:
             raise ValueError(f"an image needs {tiles_per} tiles; the batcher holds {self.max_tiles}")
@@ -63,10 +64,14 @@ class SeriesRenderer:
             slot ^= 1
 
     def render(self, params: list):
-        """The series as a host (num, height, width) int16 array."""
-        import torch
+        """The series as a host (num, height, width) int16 array, rendered on
+        the context's stream into a buffer of the context's device."""
+        import numpy as np
 
-        out = torch.empty(len(params) * params[0].width * params[0].height, dtype=torch.int16, device="cuda")
-        s = torch.cuda.current_stream()
-        self.render_async(params, out, stream=s.cuda_stream)
-        return out.cpu().numpy().reshape(len(params), params[0].height, params[0].width)
+        n = len(params) * params[0].width * params[0].height
+        out = _capi.DeviceBuffer(self.ctx, 2 * n)
+        try:
+            self.render_async(params, out, stream=self.ctx.stream)
+            return out.to_host(np.int16, n).reshape(len(params), params[0].height, params[0].width)
+        finally:
+            out.close()

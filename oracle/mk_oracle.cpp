@@ -544,6 +544,59 @@ int mko_fit_render_rows(const mk_render_params* P, const int16_t* v, int nx, int
     return 0;
 }
 
+// AnnotationUtils::Textout2Image (core/AnnotationUtils.cpp:419-464) and
+// GetSize (466-475) after SetFontSize(font), over a glyph table instead of
+// the reference's compiled-in masks: for glyph k = font * 256 + byte, width
+// gw[k] and height gh[k] (-1 for a byte without a glyph, the reference's
+// default Mask) and its bytes at atlas + goff[k]. img is RGB24, in place.
+// Returns 0 when the reference returns false (nothing drawn), else 1.
+int mko_textout(const char* text, int x, int y, int font, int format, const float* clr, const int* gw, const int* gh,
+                const int64_t* goff, const uint8_t* atlas, uint8_t* img, int width, int height) {
+    if (!img || width < 10 || height < 10) return 0;
+    auto glyph = [&](unsigned char c, int& w, int& h, const uint8_t*& m) {
+        w = -1; h = -1; m = nullptr;
+        if (font < 0 || font > 2) return;   // FontSizeNotDefined: every mask is the default
+        const int k = font * 256 + c;
+        if (gw[k] > 0) { w = gw[k]; h = gh[k]; m = atlas + goff[k]; }
+    };
+    const size_t n = strlen(text);
+    int xStart = x, yStart = y;
+    if (format == -1) return 0;
+    if (format == 2 || format == 1) {
+        int w = 0;
+        for (size_t i = 0; i < n; ++i) {   // GetSize
+            int gw1, gh1;
+            const uint8_t* m;
+            glyph((unsigned char)text[i], gw1, gh1, m);
+            w += gw1;
+        }
+        if (format == 2) xStart -= w / 2;
+        else xStart -= w;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        int w, h;
+        const uint8_t* m;
+        glyph((unsigned char)text[i], w, h, m);
+        for (int yy = yStart; yy < yStart + h; yy++) {
+            for (int xx = xStart; xx < xStart + w; xx++) {
+                if (xx < 0 || xx >= width || yy < 0 || yy >= height) continue;
+                const uint8_t v = m[(yy - yStart) * w + xx - xStart];
+                if (v > 0) {
+                    // unsigned char * float in float, to unsigned char by
+                    // truncation toward zero keeping the low 8 bits
+                    for (int ch = 0; ch < 3; ++ch) {
+                        float f = (float)v * clr[ch];
+                        f = std::fmin(std::fmax(f, -2147483648.0f), 2147483520.0f);
+                        img[(yy * (size_t)width + xx) * 3 + ch] = (uint8_t)(int)f;
+                    }
+                }
+            }
+        }
+        xStart += w;
+    }
+    return 1;
+}
+
 // The expanded triangle table, for pinning against the reference's table.
 void mko_mc_table(int8_t* tri, uint8_t* cnt) {
     mk_mc_expand((int8_t(*)[16])tri, cnt);

@@ -39,6 +39,8 @@ def lib() -> C.CDLL:
         L.mko_fit_render.argtypes = [C.POINTER(RenderParams), vp, i, i, i, vp, i, C.c_double, vp]
         L.mko_fit_render_rows.restype = i
         L.mko_fit_render_rows.argtypes = [C.POINTER(RenderParams), vp, i, i, i, vp, i, C.c_double, i, i, vp]
+        L.mko_textout.restype = i
+        L.mko_textout.argtypes = [C.c_char_p, i, i, i, i, vp, vp, vp, vp, vp, vp, i, i]
         L.mko_mc_table.restype = None
         L.mko_mc_table.argtypes = [vp, vp]
         L.mko_tex3_linear.restype = C.c_double
@@ -197,3 +199,31 @@ def fit_render_rows(p: RenderParams, vol_zyx, lut, scale: float, y0: int, y1: in
                                    int(y0), int(y1), out.ctypes.data)
     assert rc == 0
     return out
+
+
+def glyph_table(glyphs: dict):
+    """{(font, byte): (h, w) uint8 mask} -> (widths, heights, offsets, atlas)
+    arrays over font * 256 + byte; -1 where there is no glyph."""
+    gw = np.full(3 * 256, -1, np.int32)
+    gh = np.full(3 * 256, -1, np.int32)
+    go = np.zeros(3 * 256, np.int64)
+    parts, off = [], 0
+    for (font, code), m in sorted(glyphs.items()):
+        m = np.ascontiguousarray(m, np.uint8)
+        k = font * 256 + code
+        gh[k], gw[k] = m.shape
+        go[k] = off
+        parts.append(m.reshape(-1))
+        off += m.size
+    atlas = np.concatenate(parts) if parts else np.zeros(1, np.uint8)
+    return gw, gh, go, np.ascontiguousarray(atlas)
+
+
+def textout(img: np.ndarray, text: bytes, x: int, y: int, font: int, fmt: int, color, table) -> bool:
+    """AnnotationUtils::Textout2Image on an (h, w, 3) uint8 image in place."""
+    gw, gh, go, atlas = table
+    c = np.ascontiguousarray(color, np.float32).reshape(3)
+    assert img.dtype == np.uint8 and img.flags["C_CONTIGUOUS"]
+    h, w = img.shape[:2]
+    return bool(lib().mko_textout(bytes(text), int(x), int(y), int(font), int(fmt), c.ctypes.data, gw.ctypes.data,
+                                  gh.ctypes.data, go.ctypes.data, atlas.ctypes.data, img.ctypes.data, w, h))

@@ -135,3 +135,41 @@ def test_two_instances_are_independent(volume):
     fb = _vr_frame(b)
     fa2 = _vr_frame(a)
     assert np.array_equal(fa1, fa2) and not np.array_equal(fa1, fb)
+
+
+def test_annotations_drawn_in_vr_frames(volume):
+    """AddAnnotation's texts on the VR frame (core/AnnotationLayer.cpp:41-61,
+    core/VRProvider.cpp:30-36): the frame equals the plain render with the
+    oracle's Textout2Image of each annotation applied in order; removed
+    annotations and a disabled layer leave the plain render."""
+    hm = _hm(volume)
+    hm.SetVRSize(160, 120)
+    hm.Rotate(15, 10)
+    rng = np.random.default_rng(4)
+    glyphs = {}
+    for font, h in ((0, 22), (2, 44)):
+        for ch in "HiOK!0123456789 ":
+            w = int(rng.integers(3, 14))
+            m = rng.integers(0, 256, size=(h, w)).astype(np.uint8)
+            m[rng.random((h, w)) < 0.3] = 0
+            glyphs[(font, ord(ch))] = m
+            hm.SetAnnotationGlyph(font, ch, m)
+    annos = [("Hi 42", 4, 3, mk.FontSizeSmall, mk.AnnotationFormatLeft, mk.RGBA(1.0, 1.0, 0.0)),
+             ("OK!", 156, 70, mk.FontSizeBig, mk.AnnotationFormatRight, mk.RGBA(0.2, 0.9, 0.5)),
+             ("W?9", 80, 100, mk.FontSizeSmall, mk.AnnotationFormatCenter, mk.RGBA(0.5, 0.5, 1.0))]
+    for a in annos:
+        assert hm.AddAnnotation(mk.PlaneVR, *a)
+    img = _vr_frame(hm)
+    p = hm._render_params(hm._ri.width, hm._ri.height)
+    plain = hm._ctx.render(p)
+    exp = plain.copy()
+    table = O.glyph_table(glyphs)
+    for txt, x, y, fs, fmt, clr in annos:
+        O.textout(exp, txt.encode(), x, y, int(fs), int(fmt), clr.tuple()[:3], table)
+    assert not np.array_equal(exp, plain)
+    assert np.array_equal(img, exp)
+    hm.EnableLayer(mk.LayerTypeAnnotation, False)
+    assert np.array_equal(_vr_frame(hm), plain)
+    hm.EnableLayer(mk.LayerTypeAnnotation, True)
+    hm.RemoveAllAnnotations()
+    assert np.array_equal(_vr_frame(hm), plain)
