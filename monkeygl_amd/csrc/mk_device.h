@@ -24,8 +24,13 @@ struct MkVolView {
     int nbx, nby, nbz;    // brick dims
 };
 
-__host__ __device__ inline uint32_t mk_brick_index(const MkVolView& v, int x, int y, int z) {
-    return ((uint32_t)(z >> MK_BRICK_SHIFT) * (uint32_t)v.nby + (uint32_t)(y >> MK_BRICK_SHIFT)) * (uint32_t)v.nbx +
+// Index products are written as 24-bit multiplies, which the compiler can
+// issue as one full-rate instruction (a 32-bit multiply is quarter-rate). They
+// are exact: a volume holds at most 2^32 padded voxels, so brick counts and
+// every partial brick index stay below 2^23.
+__device__ inline uint32_t mk_brick_index(const MkVolView& v, int x, int y, int z) {
+    return __umul24(__umul24((uint32_t)(z >> MK_BRICK_SHIFT), (uint32_t)v.nby) + (uint32_t)(y >> MK_BRICK_SHIFT),
+                    (uint32_t)v.nbx) +
            (uint32_t)(x >> MK_BRICK_SHIFT);
 }
 
@@ -34,18 +39,18 @@ __host__ __device__ inline uint32_t mk_brick_index(const MkVolView& v, int x, in
 // 64 samples of a wave lie on a small patch of a plane at any tilt; blocks
 // give that patch about half the cache lines that 8x8 z-slices of the brick
 // would. Bits of (x, y, z) & 7 in the in-brick offset: x0 x1 y0 y1 z0 z1 x2 y2 z2.
-__host__ __device__ inline uint32_t mk_inbrick(int x, int y, int z) {
+__device__ inline uint32_t mk_inbrick(int x, int y, int z) {
     return ((uint32_t)(z & 4) << 6) | ((uint32_t)(y & 4) << 5) | ((uint32_t)(x & 4) << 4) | ((uint32_t)(z & 3) << 4) |
            ((uint32_t)(y & 3) << 2) | (uint32_t)(x & 3);
 }
 
-__host__ __device__ inline uint32_t mk_voxel_offset(const MkVolView& v, int x, int y, int z) {
+__device__ inline uint32_t mk_voxel_offset(const MkVolView& v, int x, int y, int z) {
     return (mk_brick_index(v, x, y, z) << 9) | mk_inbrick(x, y, z);
 }
 
 // The label mask keeps x-rows inside its bricks (8 bytes per row, z slowest):
 // its label kernels handle a brick as 64 rows.
-__host__ __device__ inline uint32_t mk_mask_offset(const MkVolView& v, int x, int y, int z) {
+__device__ inline uint32_t mk_mask_offset(const MkVolView& v, int x, int y, int z) {
     return (mk_brick_index(v, x, y, z) << 9) | ((uint32_t)(z & 7) << 6) | ((uint32_t)(y & 7) << 3) | (uint32_t)(x & 7);
 }
 
@@ -55,10 +60,10 @@ __device__ inline uint32_t mk_term_x(int x) {
     return ((uint32_t)(x >> 3) << 9) | ((uint32_t)(x & 4) << 4) | (uint32_t)(x & 3);
 }
 __device__ inline uint32_t mk_term_y(const MkVolView& v, int y) {
-    return (uint32_t)(y >> 3) * ((uint32_t)v.nbx << 9) + (((uint32_t)(y & 4) << 5) | ((uint32_t)(y & 3) << 2));
+    return (__umul24((uint32_t)(y >> 3), (uint32_t)v.nbx) << 9) + (((uint32_t)(y & 4) << 5) | ((uint32_t)(y & 3) << 2));
 }
 __device__ inline uint32_t mk_term_z(const MkVolView& v, int z) {
-    return (uint32_t)(z >> 3) * (((uint32_t)v.nbx * (uint32_t)v.nby) << 9) +
+    return (__umul24((uint32_t)(z >> 3), __umul24((uint32_t)v.nbx, (uint32_t)v.nby)) << 9) +
            (((uint32_t)(z & 4) << 6) | ((uint32_t)(z & 3) << 4));
 }
 
@@ -140,4 +145,24 @@ __device__ inline uint32_t mk_xcd_tile(uint32_t b, uint32_t n) {
     uint32_t q = n / X, r = n % X;
     uint32_t x = b % X, i = b / X;
     return x * q + (x < r ? x : r) + i;
+}
+
+// Maps a linear workgroup id to a tile of a tiles_x x tiles_y grid so that XCD
+// k (workgroups go round-robin over the 8 XCDs) takes every 8th row of tiles,
+// rows k, k + 8, ..., each left to right. A frame's work varies mostly from
+// band to band of rows (an object's outline against the background): with
+// contiguous bands per XCD the busiest XCD of a body view held 3.5x the work
+// of the idlest and set the frame time. Each XCD's rows still cover disjoint
+// slabs of the volume. Bijective on [0, tiles_x * tiles_y).
+__device__ inline uint32_t mk_xcd_row_tile(uint32_t b, uint32_t tiles_x, uint32_t tiles_y) {
+    const uint32_t j = mk_xcd_tile(b, tiles_x * tiles_y);
+    // j is a position in the order of rows grouped by their residue mod 8
+    uint32_t k = 0, start = 0;
+    for (; k < 7; ++k) {
+        const uint32_t len = (tiles_y + 7 - k) / 8 * tiles_x;   // rows r < tiles_y with r % 8 == k
+        if (j < start + len) break;
+        start += len;
+    }
+    const uint32_t local = j - start;
+    return (k + 8 * (local / tiles_x)) * tiles_x + local % tiles_x;
 }

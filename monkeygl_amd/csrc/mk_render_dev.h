@@ -164,7 +164,8 @@ __device__ inline bool in_voi(const RenderArgs& A, const float p[3], int& ix, in
 }
 
 __device__ inline uint32_t macro_index(const RenderArgs& A, int cx, int cy, int cz) {
-    return ((uint32_t)(cz >> 5) * (uint32_t)A.nmy + (uint32_t)(cy >> 5)) * (uint32_t)A.nmx + (uint32_t)(cx >> 5);
+    return __umul24(__umul24((uint32_t)(cz >> 5), (uint32_t)A.nmy) + (uint32_t)(cy >> 5), (uint32_t)A.nmx) +
+           (uint32_t)(cx >> 5);
 }
 
 // tex1D linear/clamp at normalized s of the LUT of n entries at off.
@@ -400,8 +401,7 @@ __device__ inline float4 bg4(const RenderArgs& A) { return make_float4(A.bg[0], 
 template <int TW>
 __device__ inline bool tile_pixel(const RenderArgs& A, int& x, int& y) {
     const uint32_t tiles_x = (A.width + TW - 1) / TW;
-    const uint32_t tiles = tiles_x * ((A.height + TW - 1) / TW);
-    uint32_t t = mk_xcd_tile(blockIdx.x, tiles);
+    const uint32_t t = mk_xcd_row_tile(blockIdx.x, tiles_x, (A.height + TW - 1) / TW);
     int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     constexpr int WPR = TW / 8;  // waves per tile row
     x = (int)(t % tiles_x) * TW + (wave % WPR) * 8 + (lane & 7);

@@ -303,3 +303,33 @@ def test_skipping_is_exact_with_capped_label(gpu_ctx, small):
             frac, dmax = frac_within(got, ref)
             assert frac >= MIN_FRAC, (frac, dmax)
     gpu_ctx.clear_mask()
+
+
+@pytest.mark.parametrize("rt", [MK_RENDER_VR, MK_RENDER_MIP, MK_RENDER_SURFACE])
+def test_output_at_any_byte_offset(gpu_ctx, small, rt):
+    """The RGB output may start at any byte address: the same frame rendered
+    to an aligned buffer and to one a byte off alignment must be equal, bit
+    for bit, and neither render may write outside its width * height * 3
+    bytes (the sizes leave partial 8x8 pixel blocks at the right and bottom
+    edges)."""
+    import torch
+
+    vol, lut = small
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_lut(0, lut)
+    s = torch.cuda.current_stream().cuda_stream
+    for w, h in [(260, 132), (256, 256), (516, 300)]:
+        p = _params(rt, (20, 35), width=w, height=h)
+        n = w * h * 3
+        frames = []
+        for off in (0, 1):
+            buf = torch.full((n + 16,), 0xA5, dtype=torch.uint8, device="cuda")
+            gpu_ctx.render_async(p, buf[off:off + n], stream=s)
+            torch.cuda.synchronize()
+            host = buf.cpu().numpy()
+            assert (host[:off] == 0xA5).all() and (host[off + n:] == 0xA5).all(), (w, h, off)
+            frames.append(host[off:off + n].reshape(h, w, 3))
+        assert np.array_equal(frames[0], frames[1]), (w, h, int((frames[0] != frames[1]).any(-1).sum()))
+        ref = O.render(p, vol, {0: lut})
+        frac, dmax = frac_within(frames[0], ref)
+        assert frac >= MIN_FRAC, (w, h, frac, dmax)
