@@ -155,8 +155,7 @@ __device__ inline float3 fit_light(const FitArgs& A) {
 
 __device__ inline void fit_tile(const FitArgs& A, int& x, int& y) {
     const uint32_t tiles_x = (A.width + 15) / 16;
-    const uint32_t tiles = tiles_x * ((A.height + 15) / 16);
-    uint32_t t = mk_xcd_tile(blockIdx.x, tiles);
+    const uint32_t t = mk_xcd_row_tile(blockIdx.x, tiles_x, (A.height + 15) / 16);
     int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     x = (int)(t % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
     y = (int)(t / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
