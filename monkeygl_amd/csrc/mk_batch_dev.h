@@ -90,7 +90,10 @@ __device__ inline bool batch_item(const uint8_t* __restrict__ block, uint32_t ti
     // fields load through the scalar unit
     lo = __builtin_amdgcn_readfirstlane(lo);
     item = items + lo;
-    const uint32_t t = g - base[lo];
+    // the item's tiles in rows per XCD, as a one-request launch maps them
+    // (workgroup g runs on XCD g % 8, so equal (g - base) % 8 share an XCD)
+    const uint32_t t = mk_xcd_row_tile(g - base[lo], (uint32_t)item->tiles_x,
+                                       (uint32_t)((item->height + kTile - 1) / kTile));
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     x = (int)(t % (uint32_t)item->tiles_x) * kTile + (wave & 1) * 8 + (lane & 7);
     y = (int)(t / (uint32_t)item->tiles_x) * kTile + (wave >> 1) * 8 + (lane >> 3);
