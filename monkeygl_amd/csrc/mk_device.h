@@ -24,10 +24,11 @@ struct MkVolView {
     int nbx, nby, nbz;    // brick dims
 };
 
-// Index products are written as 24-bit multiplies, which the compiler can
-// issue as one full-rate instruction (a 32-bit multiply is quarter-rate). They
-// are exact: a volume holds at most 2^32 padded voxels, so brick counts and
-// every partial brick index stay below 2^23.
+// Index products are 24-bit multiplies, one full-rate instruction each (a
+// 32-bit multiply is quarter-rate). They are exact: a volume holds at most
+// 2^32 padded voxels, so brick counts and every partial brick index stay below
+// 2^23, and no dimension exceeds 2^18 voxels (MK_MAX_DIM), so a row or slab of
+// bricks scaled by 512 voxels stays below 2^24.
 __device__ inline uint32_t mk_brick_index(const MkVolView& v, int x, int y, int z) {
     return __umul24(__umul24((uint32_t)(z >> MK_BRICK_SHIFT), (uint32_t)v.nby) + (uint32_t)(y >> MK_BRICK_SHIFT),
                     (uint32_t)v.nbx) +
@@ -60,14 +61,16 @@ __device__ inline uint32_t mk_term_x(int x) {
     return ((uint32_t)(x >> 3) << 9) | ((uint32_t)(x & 4) << 4) | (uint32_t)(x & 3);
 }
 __device__ inline uint32_t mk_term_y(const MkVolView& v, int y) {
-    return (__umul24((uint32_t)(y >> 3), (uint32_t)v.nbx) << 9) + (((uint32_t)(y & 4) << 5) | ((uint32_t)(y & 3) << 2));
+    return __umul24((uint32_t)(y >> 3) << 9, (uint32_t)v.nbx) + (((uint32_t)(y & 4) << 5) | ((uint32_t)(y & 3) << 2));
 }
 __device__ inline uint32_t mk_term_z(const MkVolView& v, int z) {
-    return (__umul24((uint32_t)(z >> 3), __umul24((uint32_t)v.nbx, (uint32_t)v.nby)) << 9) +
+    return __umul24((uint32_t)(z >> 3) << 9, __umul24((uint32_t)v.nbx, (uint32_t)v.nby)) +
            (((uint32_t)(z & 4) << 6) | ((uint32_t)(z & 3) << 4));
 }
 
-__device__ inline int mk_clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// lo <= hi: max then min (two instructions, where the nested conditional
+// compiles to three)
+__device__ inline int mk_clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
 // NaN-safe saturate: NaN maps to 0.
 __device__ inline float mk_sat(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }

@@ -133,8 +133,10 @@ __device__ inline bool ray_interval(const RenderArgs& A, float u, float v, RayGe
 
 // Accumulated length below which the ray is certain to stay in the cell of
 // 2^shift voxels per axis that holds voxel (cx, cy, cz): the earliest exit
-// over the three axes, with the cell shrunk by a margin that covers the
-// rounding of the sample positions.
+// over the three axes, with the cell shrunk by a margin (1e-5 of the volume's
+// extent) that covers the rounding of the sample positions. Quotients are
+// taken as products with hardware reciprocals: their error, a few ulp of a
+// distance below 2, stays ~25x inside the margin.
 __device__ inline float cell_exit(const RenderArgs& A, const RayGeom& g, int cx, int cy, int cz, int shift) {
     const int c[3] = {cx >> shift, cy >> shift, cz >> shift};
     const float n[3] = {(float)A.v.nx, (float)A.v.ny, (float)A.v.nz};
@@ -143,11 +145,11 @@ __device__ inline float cell_exit(const RenderArgs& A, const RayGeom& g, int cx,
     float t = INFINITY;
     for (int i = 0; i < 3; ++i) {
         if (g.d[i] > 1e-12f) {
-            float face = ((float)(c[i] + 1) * size) / n[i] - margin;
-            t = fminf(t, (face - g.p0[i]) / g.d[i]);
+            float face = ((float)(c[i] + 1) * size) * __builtin_amdgcn_rcpf(n[i]) - margin;
+            t = fminf(t, (face - g.p0[i]) * __builtin_amdgcn_rcpf(g.d[i]));
         } else if (g.d[i] < -1e-12f) {
-            float face = ((float)c[i] * size) / n[i] + margin;
-            t = fminf(t, (face - g.p0[i]) / g.d[i]);
+            float face = ((float)c[i] * size) * __builtin_amdgcn_rcpf(n[i]) + margin;
+            t = fminf(t, (face - g.p0[i]) * __builtin_amdgcn_rcpf(g.d[i]));
         }
     }
     return t;

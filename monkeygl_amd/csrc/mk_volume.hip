@@ -451,6 +451,8 @@ __global__ void k_fill_u8(uint8_t* p, uint8_t val, uint32_t n) {
 
 static mk_status mk_check_dims(mk_context* ctx, int nx, int ny, int nz) {
     if (nx <= 0 || ny <= 0 || nz <= 0) return mk_fail(ctx, MK_ERR_ARG, "volume dims must be positive (%d,%d,%d)", nx, ny, nz);
+    if (nx > MK_MAX_DIM || ny > MK_MAX_DIM || nz > MK_MAX_DIM)
+        return mk_fail(ctx, MK_ERR_ARG, "volume dims (%d,%d,%d) above %d", nx, ny, nz, MK_MAX_DIM);
     size_t nb = (size_t)((nx + 7) / 8) * ((ny + 7) / 8) * ((nz + 7) / 8);
     if (nb * 512 > 0xFFFFFFFFull) return mk_fail(ctx, MK_ERR_ARG, "volume too large: %zu padded voxels (limit 2^32)", nb * 512);
     return MK_OK;

@@ -333,3 +333,18 @@ def test_output_at_any_byte_offset(gpu_ctx, small, rt):
         ref = O.render(p, vol, {0: lut})
         frac, dmax = frac_within(frames[0], ref)
         assert frac >= MIN_FRAC, (w, h, frac, dmax)
+
+
+def test_volume_dims_above_the_limit_are_refused(gpu_ctx, small):
+    """A dimension above MK_MAX_DIM is an argument error before anything is
+    allocated, and the context keeps rendering its volume."""
+    from monkeygl_amd._capi import MK_MAX_DIM
+
+    vol, lut = small
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_lut(0, lut)
+    p = _params(MK_RENDER_VR, (10, 20), size=64)
+    before = gpu_ctx.render(p)
+    with pytest.raises(MkError):
+        gpu_ctx.set_volume(np.zeros((MK_MAX_DIM + 1, 1, 1), np.int16))
+    assert np.array_equal(gpu_ctx.render(p), before)
