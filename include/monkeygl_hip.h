@@ -44,8 +44,8 @@ extern "C" {
 #define MK_NUM_LABELS (MK_MAX_OBJECTS + 1)
 
 /* Largest volume dimension, per axis (the reference sets none); a volume also
- * holds at most 2^32 voxels once padded to whole 8^3 bricks. */
-#define MK_MAX_DIM 262144
+ * holds at most 2^31 voxels (4 GiB of int16) once padded to whole 8^3 bricks. */
+#define MK_MAX_DIM 131072
 
 typedef enum {
     MK_OK = 0,
@@ -166,7 +166,7 @@ mk_status mk_copy_to_device(mk_context* ctx, void* dev, const void* host, uint64
 /* cu_copyVolumeData (core/CuDataManager.cu:28-73). host is int16, x fastest
  * (index z*nx*ny + y*nx + x), as the reference's make_cudaPitchedPtr layout.
  * Builds 8^3 bricks and per-brick min/max on the device. Drops any mask.
- * MK_ERR_ARG for a dimension above MK_MAX_DIM or more than 2^32 padded voxels. */
+ * MK_ERR_ARG for a dimension above MK_MAX_DIM or more than 2^31 padded voxels. */
 mk_status mk_copy_volume_data(mk_context* ctx, const int16_t* host, int nx, int ny, int nz);
 
 /* Same, from a device buffer of the same layout (already in HBM). */

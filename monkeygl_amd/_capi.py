@@ -17,7 +17,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libmonkeygl_hip.so"
 
 MK_MAX_OBJECTS = 64
 MK_NUM_LABELS = MK_MAX_OBJECTS + 1
-MK_MAX_DIM = 262144
+MK_MAX_DIM = 131072
 
 MK_OK, MK_ERR_ARG, MK_ERR_STATE, MK_ERR_HIP, MK_ERR_NOMEM, MK_ERR_RANGE = range(6)
 MK_RENDER_VR, MK_RENDER_MIP, MK_RENDER_SURFACE = range(3)

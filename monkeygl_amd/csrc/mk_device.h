@@ -7,7 +7,7 @@
 // wave's 8x8 ray tile stepping through the volume reuses the same lines for
 // several steps whatever the ray direction. The uint8 label mask uses the same
 // brick order (512 B per brick). Offsets are 32-bit: a volume may hold up to
-// 2^32 padded voxels.
+// 2^31 padded voxels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,9 +26,9 @@ struct MkVolView {
 
 // Index products are 24-bit multiplies, one full-rate instruction each (a
 // 32-bit multiply is quarter-rate). They are exact: a volume holds at most
-// 2^32 padded voxels, so brick counts and every partial brick index stay below
-// 2^23, and no dimension exceeds 2^18 voxels (MK_MAX_DIM), so a row or slab of
-// bricks scaled by 512 voxels stays below 2^24.
+// 2^31 padded voxels, so brick counts and every partial brick index stay below
+// 2^22, and no dimension exceeds 2^17 voxels (MK_MAX_DIM), so a row or slab of
+// bricks scaled by 1024 bytes stays below 2^24.
 __device__ inline uint32_t mk_brick_index(const MkVolView& v, int x, int y, int z) {
     return __umul24(__umul24((uint32_t)(z >> MK_BRICK_SHIFT), (uint32_t)v.nby) + (uint32_t)(y >> MK_BRICK_SHIFT),
                     (uint32_t)v.nbx) +
@@ -55,17 +55,24 @@ __device__ inline uint32_t mk_mask_offset(const MkVolView& v, int x, int y, int 
     return (mk_brick_index(v, x, y, z) << 9) | ((uint32_t)(z & 7) << 6) | ((uint32_t)(y & 7) << 3) | (uint32_t)(x & 7);
 }
 
-// mk_voxel_offset as a sum of one term per axis, for samplers that combine
-// several indices per axis.
+// The byte offset of a voxel (mk_voxel_offset * 2) as a sum of one term per
+// axis, for samplers that combine several indices per axis. Byte offsets fit
+// 32 bits (a volume holds at most 2^31 padded voxels), so loads take the
+// volume's base address from scalar registers and a 32-bit offset per lane.
 __device__ inline uint32_t mk_term_x(int x) {
-    return ((uint32_t)(x >> 3) << 9) | ((uint32_t)(x & 4) << 4) | (uint32_t)(x & 3);
+    return ((uint32_t)(x >> 3) << 10) | ((uint32_t)(x & 4) << 5) | ((uint32_t)(x & 3) << 1);
 }
 __device__ inline uint32_t mk_term_y(const MkVolView& v, int y) {
-    return __umul24((uint32_t)(y >> 3) << 9, (uint32_t)v.nbx) + (((uint32_t)(y & 4) << 5) | ((uint32_t)(y & 3) << 2));
+    return __umul24((uint32_t)(y >> 3) << 10, (uint32_t)v.nbx) + (((uint32_t)(y & 4) << 6) | ((uint32_t)(y & 3) << 3));
 }
 __device__ inline uint32_t mk_term_z(const MkVolView& v, int z) {
-    return __umul24((uint32_t)(z >> 3) << 9, __umul24((uint32_t)v.nbx, (uint32_t)v.nby)) +
-           (((uint32_t)(z & 4) << 6) | ((uint32_t)(z & 3) << 4));
+    return __umul24((uint32_t)(z >> 3) << 10, __umul24((uint32_t)v.nbx, (uint32_t)v.nby)) +
+           (((uint32_t)(z & 4) << 7) | ((uint32_t)(z & 3) << 5));
+}
+
+// The voxel at byte offset boff (a sum of mk_term_* values).
+__device__ inline int mk_vox_at(const MkVolView& v, uint32_t boff) {
+    return *(const int16_t*)((const uint8_t*)v.vox + boff);
 }
 
 // lo <= hi: max then min (two instructions, where the nested conditional
@@ -95,8 +102,8 @@ __device__ inline void mk_split(float f, int n, int& i0, int& i1, float& a) {
     i1 = mk_clampi(i + 1, 0, n - 1);
 }
 
-__device__ inline float mk_rawo(const MkVolView& v, uint32_t off) {
-    int s = v.vox[off];
+__device__ inline float mk_rawo(const MkVolView& v, uint32_t boff) {
+    int s = mk_vox_at(v, boff);
     return (float)(s < -32767 ? -32767 : s);
 }
 

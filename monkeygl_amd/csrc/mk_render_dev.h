@@ -233,14 +233,14 @@ __device__ inline void sample_fetch(const MkVolView& v, const float p[3], TriFet
     const uint32_t X0 = mk_term_x(mk_clampi(F.ix, 0, v.nx - 1)), X1 = mk_term_x(mk_clampi(F.ix + 1, 0, v.nx - 1));
     const uint32_t Y0 = mk_term_y(v, mk_clampi(F.iy, 0, v.ny - 1)), Y1 = mk_term_y(v, mk_clampi(F.iy + 1, 0, v.ny - 1));
     const uint32_t Z0 = mk_term_z(v, mk_clampi(F.iz, 0, v.nz - 1)), Z1 = mk_term_z(v, mk_clampi(F.iz + 1, 0, v.nz - 1));
-    F.r[0] = v.vox[X0 + Y0 + Z0];
-    F.r[1] = v.vox[X1 + Y0 + Z0];
-    F.r[2] = v.vox[X0 + Y1 + Z0];
-    F.r[3] = v.vox[X1 + Y1 + Z0];
-    F.r[4] = v.vox[X0 + Y0 + Z1];
-    F.r[5] = v.vox[X1 + Y0 + Z1];
-    F.r[6] = v.vox[X0 + Y1 + Z1];
-    F.r[7] = v.vox[X1 + Y1 + Z1];
+    F.r[0] = mk_vox_at(v, X0 + Y0 + Z0);
+    F.r[1] = mk_vox_at(v, X1 + Y0 + Z0);
+    F.r[2] = mk_vox_at(v, X0 + Y1 + Z0);
+    F.r[3] = mk_vox_at(v, X1 + Y1 + Z0);
+    F.r[4] = mk_vox_at(v, X0 + Y0 + Z1);
+    F.r[5] = mk_vox_at(v, X1 + Y0 + Z1);
+    F.r[6] = mk_vox_at(v, X0 + Y1 + Z1);
+    F.r[7] = mk_vox_at(v, X1 + Y1 + Z1);
 }
 
 // The trilinear value of a fetched sample, keeping what the gradient reuses

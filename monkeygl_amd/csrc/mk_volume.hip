@@ -454,7 +454,7 @@ static mk_status mk_check_dims(mk_context* ctx, int nx, int ny, int nz) {
     if (nx > MK_MAX_DIM || ny > MK_MAX_DIM || nz > MK_MAX_DIM)
         return mk_fail(ctx, MK_ERR_ARG, "volume dims (%d,%d,%d) above %d", nx, ny, nz, MK_MAX_DIM);
     size_t nb = (size_t)((nx + 7) / 8) * ((ny + 7) / 8) * ((nz + 7) / 8);
-    if (nb * 512 > 0xFFFFFFFFull) return mk_fail(ctx, MK_ERR_ARG, "volume too large: %zu padded voxels (limit 2^32)", nb * 512);
+    if (nb * 512 > 0x80000000ull) return mk_fail(ctx, MK_ERR_ARG, "volume too large: %zu padded voxels (limit 2^31)", nb * 512);
     return MK_OK;
 }
 
