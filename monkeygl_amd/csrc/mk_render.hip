@@ -52,7 +52,7 @@ constexpr int kMaxLdsBytes = 64 * 1024;
 constexpr int kLutLdsEntries = 1024;  // stage LUTs in LDS up to 16 KiB
 constexpr size_t kContHeader = 256;                  // queue counters
 constexpr size_t kContMaxBytes = size_t(2) << 30;    // larger frames render in one pass
-constexpr size_t kTailBlocks = 2048;                 // second-pass workgroups (8 per CU)
+constexpr size_t kTailBlocksPerCu = 4;              // second-pass workgroups: one resident round (16 waves per CU)
 constexpr float kLongSteps = 256.0f;                 // second-pass rays claimed first
 constexpr int kTailGroup = 8;                        // second-pass lanes per ray
 constexpr size_t kTailPercent = 95;                  // first-pass waves finished before the rest hand off
@@ -273,8 +273,8 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     const int cslots = mask ? std::max(A.nslot, 1) : 1;
     const size_t cont_bytes = kContHeader + npix * (sizeof(float4) + 6 * sizeof(float) + (size_t)cslots * sizeof(float));
     int budget = ctx->vr_budget;
+    if (ctx->n_cu <= 0) MK_HIP(ctx, hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     if (budget == kVrTripBudgetAuto) {
-        if (ctx->n_cu <= 0) MK_HIP(ctx, hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
         const size_t waves = (size_t)tiles * (size_t)(tw * tw / 64);
         budget = waves <= (size_t)ctx->n_cu * kVrResidentWavesPerCu ? kVrTripBudgetSmall : kVrTripBudgetLarge;
     }
@@ -329,7 +329,8 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
         if (two_pass) {
             const size_t groups = 256 / kTailGroup;
             const size_t lds2 = kLabelLdsBytes + (size_t)A.nlut_lds * sizeof(float4) + groups * (size_t)cslots * sizeof(float);
-            const dim3 grid2((uint32_t)std::min<size_t>((npix + groups - 1) / groups, kTailBlocks)), block2(256);
+            const size_t tail_blocks = (size_t)ctx->n_cu * kTailBlocksPerCu;
+            const dim3 grid2((uint32_t)std::min<size_t>((npix + groups - 1) / groups, tail_blocks)), block2(256);
 #define MK_TAIL(K) hipLaunchKernelGGL(K, grid2, block2, lds2, s, A, T)
             if (mask) { if (lds_lut) MK_TAIL((k_vr_tail<true, true>)); else MK_TAIL((k_vr_tail<true, false>)); }
             else { if (lds_lut) MK_TAIL((k_vr_tail<false, true>)); else MK_TAIL((k_vr_tail<false, false>)); }
