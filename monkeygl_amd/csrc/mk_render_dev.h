@@ -577,11 +577,11 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
                 }
                 empty_shift = cur_macro_empty ? 5 : (cur_occ ? 0 : 3);
             }
-            float a = 0.f;
-            float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-            int label = 0;
-            TriSample tri;
+            // The empty and the sampled cases each run getNextStep
+            // (kernel.cu:84-125) on their own: the sample's state (corners,
+            // colour, label) then never has to meet the empty case's zeros.
             if (!empty_shift) {
+                int label = 0;
                 if (MASK) label = (TOUCH || cur_label == 0xFF) ? A.v.mask[mk_mask_offset(A.v, cx, cy, cz)] : cur_label;
                 if (TOUCH) {
                     if (MASK) touch_mask(A, cx, cy, cz);
@@ -589,24 +589,26 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
                 }
                 const float4 aw = s.aw[label];
                 const int4 lb = s.lab[label];
+                TriSample tri;
                 float t = sample_keep(A.v, p, tri);
                 t = (t - aw.z) / aw.y + 0.5f;
                 t = t > 1.0f ? 1.0f : t;
-                col = lut_sample<LDS_LUT>(A, s.lut, lb.x, lb.y, t);
-                a = col.w;
-            }
-            // getNextStep (kernel.cu:84-125)
-            a = vr_step_alpha(a, step, L1, L4);
-            if (vr_step_back(acc, step, aPre, a, L1, L4, L8)) continue;
-            if (a > 0.0005f) {
-                const float cap = s.aw[label].x;
-                float* slot = MASK ? &s.aobj[s.lab[label].z * BS + tid] : &aobj0;
-                if (*slot < cap) {
-                    // tracing (kernel.cu:49-81)
-                    if (TOUCH) touch_gradient(A, p);
-                    const float k = vr_shade(A, gradient_kept(A, tri));
-                    vr_composite(sum, aAcc, *slot, a, col.x, col.y, col.z, k);
+                const float4 col = lut_sample<LDS_LUT>(A, s.lut, lb.x, lb.y, t);
+                const float a = vr_step_alpha(col.w, step, L1, L4);
+                if (vr_step_back(acc, step, aPre, a, L1, L4, L8)) continue;
+                if (a > 0.0005f) {
+                    const float cap = aw.x;
+                    float* slot = MASK ? &s.aobj[lb.z * BS + tid] : &aobj0;
+                    if (*slot < cap) {
+                        // tracing (kernel.cu:49-81)
+                        if (TOUCH) touch_gradient(A, p);
+                        const float k = vr_shade(A, gradient_kept(A, tri));
+                        vr_composite(sum, aAcc, *slot, a, col.x, col.y, col.z, k);
+                    }
                 }
+            } else {
+                // alpha 0: no correction, no composite
+                if (vr_step_back(acc, step, aPre, 0.f, L1, L4, L8)) continue;
             }
             if (aAcc > A.exit_alpha) break;
             // steady state in an empty cell: every further lattice point inside
