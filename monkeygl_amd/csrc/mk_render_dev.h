@@ -472,18 +472,22 @@ __device__ inline float vr_step_alpha(float a, float step, float L1, float L4) {
 // alpha a at acc: true when it backs up (acc and step changed, the sample is
 // dropped); otherwise the sample is accepted, aPre becomes a and acc advances
 // by the (possibly changed) step.
+// Written as selects: the step is one of L1, L4, L8 exactly, and each branch's
+// arithmetic is the same as in the nested form (acc - (L1 - L4) or
+// acc - (L4 - L8) on backing up, acc + the new step otherwise).
 __device__ inline bool vr_step_back(float& acc, float& step, float& aPre, float a, float L1, float L4, float L8) {
-    if (acc > 0.f) {
-        if (fmaxf(a, aPre) > 0.001f) {
-            if (step == L1) { acc -= (L1 - L4); step = L4; return true; }
-            if (step == L4) { acc -= (L4 - L8); step = L8; return true; }
-        } else {
-            step = (step == L8) ? L4 : L1;
-        }
-    }
-    aPre = a;
-    acc += step;
-    return false;
+    const bool act = acc > 0.f;
+    const bool hi = fmaxf(a, aPre) > 0.001f;
+    const bool back = act && hi && step != L8;
+    const float fine = step == L1 ? L4 : L8;      // the step after backing up from L1 or L4
+    const float coarse = step == L8 ? L4 : L1;    // the step after a low sample
+    const float next = (act && !hi) ? coarse : step;
+    const float acc_back = acc - (step - fine);
+    const float acc_on = acc + next;
+    acc = back ? acc_back : acc_on;
+    aPre = back ? aPre : a;
+    step = back ? fine : next;
+    return back;
 }
 
 // tracing's shading factor (core/kernel.cu:49-81) for normal N.
