@@ -59,8 +59,10 @@ constexpr size_t kTailPercent = 95;                  // first-pass waves finishe
 
 using namespace mkr;
 
+// With the LUTs in LDS the first pass fits 96 VGPRs without spilling: 5 waves
+// per SIMD instead of 4 (1% faster on the body views).
 template <int TW, bool MASK, bool LDS_LUT, bool TOUCH>
-__global__ void __launch_bounds__(TW * TW) k_vr(RenderArgs A, AwTable T) {
+__global__ void __launch_bounds__(TW * TW, (LDS_LUT && !TOUCH) ? 5 : 1) k_vr(RenderArgs A, AwTable T) {
     Smem s = stage_shared(A, T, LDS_LUT);
     int x, y;
     if (!tile_pixel<TW>(A, x, y)) return;
