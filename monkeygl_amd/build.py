@@ -85,12 +85,25 @@ def build_oracle() -> Path:
     return ORACLE_LIB
 
 
+def build_ref() -> Path | None:
+    """Builds ``oracle/_ref/libmkref.so`` (the reference's own sources plus the
+    harness, by ``oracle/build_ref.sh``) where the reference is mounted; the
+    built library then travels with the tree. Returns None elsewhere."""
+    ref = Path(os.environ.get("MONKEYGL_REFERENCE", "/root/reference")) / "core"
+    if not (ref / "TransferFunction.cpp").exists():
+        return None
+    _run(["sh", str(ORACLE_DIR / "build_ref.sh")])
+    return ORACLE_DIR / "_ref" / "libmkref.so"
+
+
 def build_all(jobs: int = 4) -> None:
-    with cf.ThreadPoolExecutor(max_workers=2) as ex:
+    with cf.ThreadPoolExecutor(max_workers=3) as ex:
         f1 = ex.submit(build_hip, jobs)
         f2 = ex.submit(build_oracle)
+        f3 = ex.submit(build_ref)
         f1.result()
         f2.result()
+        f3.result()
 
 
 if __name__ == "__main__":
