@@ -17,6 +17,9 @@
 //   * LUTs and the per-label window/alpha palette staged in LDS, per-label
 //     opacity accumulators in LDS instead of a 65-entry private array;
 //   * 16x16 pixel tiles of four 8x8 waves, mapped XCD-aware;
+//   * the bricked layout's per-axis address terms of clamped voxel indices
+//     read from tables in LDS (one instruction per axis for a sample's two
+//     terms) instead of composed from bit fields;
 //   * VR in two passes: k_vr marches every ray up to a trip budget (chosen
 //     per frame) or until most of its waves have finished, and queues the
 //     rays still marching with the state the reference's loop carries;
@@ -56,17 +59,20 @@ constexpr size_t kTailBlocksPerCu = 4;              // second-pass workgroups: o
 constexpr float kLongSteps = 256.0f;                 // second-pass rays claimed first
 constexpr int kTailGroup = 8;                        // second-pass lanes per ray
 constexpr size_t kTailPercent = 95;                  // first-pass waves finished before the rest hand off
+constexpr size_t kTermsLdsMax = 32 * 1024;           // axis term tables staged in LDS up to this
 
 using namespace mkr;
 
-// With the LUTs in LDS the first pass fits 96 VGPRs without spilling: 5 waves
-// per SIMD instead of 4 (1% faster on the body views).
-template <int TW, bool MASK, bool LDS_LUT, bool TOUCH>
+// With the LUTs in LDS the first pass fits 96 VGPRs: 5 waves per SIMD instead
+// of 4 (1% faster on the body views). TAB: the samplers read the axis term
+// tables staged after the accumulators (one dword spilled in the shading
+// branch at 96 VGPRs; still faster than composing the terms).
+template <int TW, bool MASK, bool LDS_LUT, bool TOUCH, bool TAB>
 __global__ void __launch_bounds__(TW * TW, (LDS_LUT && !TOUCH) ? 5 : 1) k_vr(RenderArgs A, AwTable T) {
-    Smem s = stage_shared(A, T, LDS_LUT);
+    Smem s = stage_shared(A, T, LDS_LUT, A.nslot * TW * TW, TAB);
     int x, y;
     if (!tile_pixel<TW>(A, x, y)) return;
-    vr_pixel<TW * TW, MASK, LDS_LUT, TOUCH>(A, s, x, y, threadIdx.x);
+    vr_pixel<TW * TW, MASK, LDS_LUT, TOUCH, TAB>(A, s, x, y, threadIdx.x);
     // count the finished waves (rays queued or done): the first pass's last
     // waves hand off once most have finished
     if (!TOUCH && A.cont.cap > 0) {
@@ -77,22 +83,22 @@ __global__ void __launch_bounds__(TW * TW, (LDS_LUT && !TOUCH) ? 5 : 1) k_vr(Ren
 
 // Second VR pass over the rays the first pass queued: 256 / kTailGroup rays in
 // flight per workgroup (kTailGroup lanes per ray), claimed from the queue.
-template <bool MASK, bool LDS_LUT>
+template <bool MASK, bool LDS_LUT, bool TAB>
 __global__ void __launch_bounds__(256) k_vr_tail(RenderArgs A, AwTable T) {
-    Smem s = stage_shared(A, T, LDS_LUT);
-    vr_tail<MASK, LDS_LUT, kTailGroup>(A, s, s.aobj + (threadIdx.x / kTailGroup) * (MASK ? A.nslot : 0));
+    Smem s = stage_shared(A, T, LDS_LUT, (256 / kTailGroup) * (MASK ? A.nslot : 0), TAB);
+    vr_tail<MASK, LDS_LUT, kTailGroup, TAB>(A, s, s.aobj + (threadIdx.x / kTailGroup) * (MASK ? A.nslot : 0));
 }
 
 __global__ void k_zero_u32(uint32_t* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
-template <int TW, bool MASK, bool TOUCH>
+template <int TW, bool MASK, bool TOUCH, bool TAB>
 __global__ void __launch_bounds__(TW * TW) k_mip(RenderArgs A, AwTable T) {
-    Smem s = stage_shared(A, T, false);
+    Smem s = stage_shared(A, T, false, 0, TAB);
     int x, y;
     if (!tile_pixel<TW>(A, x, y)) return;
-    mip_pixel<MASK, TOUCH>(A, s, x, y);
+    mip_pixel<MASK, TOUCH, TAB>(A, s, x, y);
 }
 
 template <int TW, bool TOUCH>
@@ -255,13 +261,24 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     const bool lds_lut = p->render_type == MK_RENDER_VR && total_lut <= kLutLdsEntries;
     A.nlut_lds = lds_lut ? total_lut : 0;
 
+    // VR keeps a per-label accumulator per thread in LDS: with many labels
+    // its tiles shrink to 8x8 (the MIP and Surface kernels keep none and
+    // always take 16x16 tiles)
+    const size_t acc_slots = p->render_type == MK_RENDER_VR ? (size_t)A.nslot : 0;
     int tw = 16;
-    size_t lds = kLabelLdsBytes + (size_t)A.nlut_lds * sizeof(float4) + (size_t)A.nslot * 256 * sizeof(float);
+    size_t lds = kLabelLdsBytes + (size_t)A.nlut_lds * sizeof(float4) + acc_slots * 256 * sizeof(float);
     if (lds > kMaxLdsBytes) {
         tw = 8;
-        lds = kLabelLdsBytes + (size_t)A.nlut_lds * sizeof(float4) + (size_t)A.nslot * 64 * sizeof(float);
+        lds = kLabelLdsBytes + (size_t)A.nlut_lds * sizeof(float4) + acc_slots * 64 * sizeof(float);
         if (lds > kMaxLdsBytes) return mk_fail(ctx, MK_ERR_ARG, "too many labels for the LDS budget");
     }
+    // the VR and MIP samplers take their address terms from LDS tables when
+    // those fit (the footprint variant composes them)
+    const size_t terms_bytes = sizeof(uint32_t) * (size_t)(axis_term_len(ctx->nx) + axis_term_len(ctx->ny) +
+                                                           axis_term_len(ctx->nz));
+    const bool tab = p->render_type != MK_RENDER_SURFACE && terms_bytes <= kTermsLdsMax &&
+                     lds + terms_bytes <= kMaxLdsBytes && touch_v == nullptr;
+    if (tab) lds += terms_bytes;
     const uint32_t tiles = (uint32_t)(((p->width + tw - 1) / tw) * ((p->height + tw - 1) / tw));
     hipStream_t s = (hipStream_t)stream;
     // the skip tables were (re)built on the context stream
@@ -319,31 +336,38 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     }
     dim3 grid(tiles), block(tw * tw);
 #define MK_LAUNCH(K) hipLaunchKernelGGL(K, grid, block, lds, s, A, T)
-#define MK_VR(TOUCH)                                                                              \
+#define MK_VR(TOUCH, TAB)                                                                         \
     if (tw == 16) {                                                                               \
-        if (mask) { if (lds_lut) MK_LAUNCH((k_vr<16, true, true, TOUCH>)); else MK_LAUNCH((k_vr<16, true, false, TOUCH>)); } \
-        else { if (lds_lut) MK_LAUNCH((k_vr<16, false, true, TOUCH>)); else MK_LAUNCH((k_vr<16, false, false, TOUCH>)); } \
+        if (mask) { if (lds_lut) MK_LAUNCH((k_vr<16, true, true, TOUCH, TAB>)); else MK_LAUNCH((k_vr<16, true, false, TOUCH, TAB>)); } \
+        else { if (lds_lut) MK_LAUNCH((k_vr<16, false, true, TOUCH, TAB>)); else MK_LAUNCH((k_vr<16, false, false, TOUCH, TAB>)); } \
     } else {                                                                                      \
-        if (lds_lut) MK_LAUNCH((k_vr<8, true, true, TOUCH>)); else MK_LAUNCH((k_vr<8, true, false, TOUCH>)); \
+        if (lds_lut) MK_LAUNCH((k_vr<8, true, true, TOUCH, TAB>)); else MK_LAUNCH((k_vr<8, true, false, TOUCH, TAB>)); \
     }
     if (p->render_type == MK_RENDER_VR) {
-        if (touch) { MK_VR(true) } else { MK_VR(false) }
+        if (touch) { MK_VR(true, false) } else if (tab) { MK_VR(false, true) } else { MK_VR(false, false) }
         if (two_pass) {
             const size_t groups = 256 / kTailGroup;
-            const size_t lds2 = kLabelLdsBytes + (size_t)A.nlut_lds * sizeof(float4) + groups * (size_t)cslots * sizeof(float);
+            const size_t lds2 = kLabelLdsBytes + (size_t)A.nlut_lds * sizeof(float4) + groups * (size_t)cslots * sizeof(float) +
+                                (tab ? terms_bytes : 0);
             const size_t tail_blocks = (size_t)ctx->n_cu * kTailBlocksPerCu;
             const dim3 grid2((uint32_t)std::min<size_t>((npix + groups - 1) / groups, tail_blocks)), block2(256);
 #define MK_TAIL(K) hipLaunchKernelGGL(K, grid2, block2, lds2, s, A, T)
-            if (mask) { if (lds_lut) MK_TAIL((k_vr_tail<true, true>)); else MK_TAIL((k_vr_tail<true, false>)); }
-            else { if (lds_lut) MK_TAIL((k_vr_tail<false, true>)); else MK_TAIL((k_vr_tail<false, false>)); }
+            if (tab) {
+                if (mask) { if (lds_lut) MK_TAIL((k_vr_tail<true, true, true>)); else MK_TAIL((k_vr_tail<true, false, true>)); }
+                else { if (lds_lut) MK_TAIL((k_vr_tail<false, true, true>)); else MK_TAIL((k_vr_tail<false, false, true>)); }
+            } else {
+                if (mask) { if (lds_lut) MK_TAIL((k_vr_tail<true, true, false>)); else MK_TAIL((k_vr_tail<true, false, false>)); }
+                else { if (lds_lut) MK_TAIL((k_vr_tail<false, true, false>)); else MK_TAIL((k_vr_tail<false, false, false>)); }
+            }
 #undef MK_TAIL
             MK_HIP(ctx, hipEventRecord(ctx->cont_done, s));
             ctx->cont_stream = s;
             ctx->cont_used = true;
         }
     } else if (p->render_type == MK_RENDER_MIP) {
-        if (touch) { if (mask) MK_LAUNCH((k_mip<16, true, true>)); else MK_LAUNCH((k_mip<16, false, true>)); }
-        else { if (mask) MK_LAUNCH((k_mip<16, true, false>)); else MK_LAUNCH((k_mip<16, false, false>)); }
+        if (touch) { if (mask) MK_LAUNCH((k_mip<16, true, true, false>)); else MK_LAUNCH((k_mip<16, false, true, false>)); }
+        else if (tab) { if (mask) MK_LAUNCH((k_mip<16, true, false, true>)); else MK_LAUNCH((k_mip<16, false, false, true>)); }
+        else { if (mask) MK_LAUNCH((k_mip<16, true, false, false>)); else MK_LAUNCH((k_mip<16, false, false, false>)); }
     } else {
         if (touch) MK_LAUNCH((k_surface<16, true>)); else MK_LAUNCH((k_surface<16, false>));
     }

@@ -273,6 +273,62 @@ __device__ inline float sample_keep(const MkVolView& v, const float p[3], TriSam
     return sample_finish(F, T);
 }
 
+// The per-axis byte-offset terms of clamped indices (mk_term_x / _y / _z of
+// clamp(i, 0, n - 1)) as tables in LDS, for every index a sample or its
+// gradient taps can reach: grad_axis_base limits a sample's floor index to
+// [-3, n + 2], its corners use i and i + 1, its taps i - 1 .. i + 2, so an axis
+// holds indices -4 .. n + 4. A sample then reads each axis's two terms with
+// one LDS instruction instead of composing them from bit fields (~10 VALU
+// instructions a term); the values, and so the addresses, are the same.
+constexpr int kTermPad = 4;
+__host__ __device__ inline int axis_term_len(int n) { return n + 2 * kTermPad + 1; }
+
+struct AxisTerms {
+    const uint32_t* x;   // x[i] for i in [-kTermPad, nx + kTermPad]
+    const uint32_t* y;
+    const uint32_t* z;
+};
+
+// Fills the tables (all threads of the workgroup; the caller synchronizes).
+__device__ inline AxisTerms stage_axis_terms(const MkVolView& v, uint32_t* t) {
+    const int lx = axis_term_len(v.nx), ly = axis_term_len(v.ny), lz = axis_term_len(v.nz);
+    for (int i = threadIdx.x; i < lx + ly + lz; i += blockDim.x) {
+        uint32_t w;
+        if (i < lx) w = mk_term_x(mk_clampi(i - kTermPad, 0, v.nx - 1));
+        else if (i < lx + ly) w = mk_term_y(v, mk_clampi(i - lx - kTermPad, 0, v.ny - 1));
+        else w = mk_term_z(v, mk_clampi(i - lx - ly - kTermPad, 0, v.nz - 1));
+        t[i] = w;
+    }
+    return AxisTerms{t + kTermPad, t + lx + kTermPad, t + lx + ly + kTermPad};
+}
+
+// sample_fetch with the terms read from the tables.
+__device__ inline void sample_fetch(const MkVolView& v, const AxisTerms& A, const float p[3], TriFetch& F) {
+    grad_axis_base(p[0], v.nx, F.ix, F.ax);
+    grad_axis_base(p[1], v.ny, F.iy, F.ay);
+    grad_axis_base(p[2], v.nz, F.iz, F.az);
+    const uint32_t X0 = A.x[F.ix], X1 = A.x[F.ix + 1];
+    const uint32_t Y0 = A.y[F.iy], Y1 = A.y[F.iy + 1];
+    const uint32_t Z0 = A.z[F.iz], Z1 = A.z[F.iz + 1];
+    F.r[0] = mk_vox_at(v, X0 + Y0 + Z0);
+    F.r[1] = mk_vox_at(v, X1 + Y0 + Z0);
+    F.r[2] = mk_vox_at(v, X0 + Y1 + Z0);
+    F.r[3] = mk_vox_at(v, X1 + Y1 + Z0);
+    F.r[4] = mk_vox_at(v, X0 + Y0 + Z1);
+    F.r[5] = mk_vox_at(v, X1 + Y0 + Z1);
+    F.r[6] = mk_vox_at(v, X0 + Y1 + Z1);
+    F.r[7] = mk_vox_at(v, X1 + Y1 + Z1);
+}
+
+// sample_keep from a fetch through the tables (TAB) or by composing the terms.
+template <bool TAB>
+__device__ inline float sample_keep(const MkVolView& v, const AxisTerms& A, const float p[3], TriSample& T) {
+    TriFetch F;
+    if constexpr (TAB) sample_fetch(v, A, p, F);
+    else sample_fetch(v, p, F);
+    return sample_finish(F, T);
+}
+
 // The 24 voxels around a sample's centre block that its central differences
 // read (the outer layers at index 0 and 3 of each axis), loaded but not yet
 // used, so that a caller can put them in the same round trip as other loads.
@@ -289,6 +345,30 @@ __device__ inline void grad_taps_fetch(const MkVolView& v, int ix, int iy, int i
         OX[k] = mk_term_x(mk_clampi(ix - 1 + k, 0, v.nx - 1));
         OY[k] = mk_term_y(v, mk_clampi(iy - 1 + k, 0, v.ny - 1));
         OZ[k] = mk_term_z(v, mk_clampi(iz - 1 + k, 0, v.nz - 1));
+    }
+#define MK_V(a, b, c) mk_rawo(v, OX[a] + OY[b] + OZ[c])
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            G.x[k][j][0] = MK_V(0, 1 + j, 1 + k);
+            G.x[k][j][1] = MK_V(3, 1 + j, 1 + k);
+            G.y[k][j][0] = MK_V(1 + j, 0, 1 + k);
+            G.y[k][j][1] = MK_V(1 + j, 3, 1 + k);
+            G.z[k][j][0] = MK_V(1 + j, 1 + k, 0);
+            G.z[k][j][1] = MK_V(1 + j, 1 + k, 3);
+        }
+#undef MK_V
+}
+
+// grad_taps_fetch with the terms read from the tables.
+__device__ inline void grad_taps_fetch(const MkVolView& v, const AxisTerms& A, int ix, int iy, int iz, GradTaps& G) {
+    uint32_t OX[4], OY[4], OZ[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        OX[k] = A.x[ix - 1 + k];
+        OY[k] = A.y[iy - 1 + k];
+        OZ[k] = A.z[iz - 1 + k];
     }
 #define MK_V(a, b, c) mk_rawo(v, OX[a] + OY[b] + OZ[c])
 #pragma unroll
@@ -350,10 +430,18 @@ __device__ inline void grad_diff_kept(const MkVolView& v, const TriSample& T, fl
     grad_diff_finish(T, G, N);
 }
 
-// The gradient from a kept sample (normalized, z flipped with the volume).
-__device__ inline float3 gradient_kept(const RenderArgs& A, const TriSample& T) {
+// The gradient from a kept sample (normalized, z flipped with the volume);
+// TAB: tap addresses from the tables.
+template <bool TAB = false>
+__device__ inline float3 gradient_kept(const RenderArgs& A, const TriSample& T, const AxisTerms& X = AxisTerms{}) {
     float N[3];
-    grad_diff_kept(A.v, T, N);
+    if constexpr (TAB) {
+        GradTaps G;
+        grad_taps_fetch(A.v, X, T.ix, T.iy, T.iz, G);
+        grad_diff_finish(T, G, N);
+    } else {
+        grad_diff_kept(A.v, T, N);
+    }
     if (A.invert_z) N[2] = -N[2];
     float r = rsqrtf(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
     return make_float3(N[0] * r, N[1] * r, N[2] * r);
@@ -420,6 +508,7 @@ struct Smem {
     int4* lab;      // kLabelRows x (LUT offset, LUT length, accumulator slot, -)
     float4* lut;    // nlut_lds entries
     float* aobj;    // nslot x blockDim
+    AxisTerms terms;  // per-axis address terms (kernels that stage them)
 };
 
 // LDS bytes of the carve-up ahead of the LUT entries.
@@ -430,7 +519,10 @@ __device__ inline void stage_labels(const RenderArgs& A, int4* lab) {
         lab[i] = make_int4(A.tab.off[i], A.tab.len[i], (int)A.slot_of[i], 0);
 }
 
-__device__ inline Smem stage_shared(const RenderArgs& A, const AwTable& T, bool stage_lut) {
+// Stages the label rows, the LUTs (stage_lut) and, after aobj_floats floats
+// of per-label accumulators, the axis term tables (stage_terms).
+__device__ inline Smem stage_shared(const RenderArgs& A, const AwTable& T, bool stage_lut, int aobj_floats = 0,
+                                    bool stage_terms = false) {
     extern __shared__ float4 smem[];
     Smem s;
     s.aw = smem;
@@ -441,6 +533,7 @@ __device__ inline Smem stage_shared(const RenderArgs& A, const AwTable& T, bool 
     stage_labels(A, s.lab);
     if (stage_lut)
         for (int i = threadIdx.x; i < A.nlut_lds; i += blockDim.x) s.lut[i] = A.lut[i];
+    s.terms = stage_terms ? stage_axis_terms(A.v, (uint32_t*)(s.aobj + aobj_floats)) : AxisTerms{};
     __syncthreads();
     return s;
 }
@@ -530,7 +623,7 @@ __device__ inline void vr_composite_k(float4& sum, float& aAcc, float& slot, flo
 // d_render (core/kernel.cu:133-264) for pixel (x, y); tid indexes the
 // per-label accumulators of a block of BS threads. TOUCH: measurement variant
 // that skips nothing and marks every voxel the reference rule reads.
-template <int BS, bool MASK, bool LDS_LUT, bool TOUCH>
+template <int BS, bool MASK, bool LDS_LUT, bool TOUCH, bool TAB = false>
 __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y, int tid) {
 
     const float u = ((float)x - (float)A.width / 2.0f - A.xT) / (float)A.height;
@@ -594,7 +687,7 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
                 const float4 aw = s.aw[label];
                 const int4 lb = s.lab[label];
                 TriSample tri;
-                float t = sample_keep(A.v, p, tri);
+                float t = sample_keep<TAB>(A.v, s.terms, p, tri);
                 t = (t - aw.z) / aw.y + 0.5f;
                 t = t > 1.0f ? 1.0f : t;
                 const float4 col = lut_sample<LDS_LUT>(A, s.lut, lb.x, lb.y, t);
@@ -606,7 +699,7 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
                     if (*slot < cap) {
                         // tracing (kernel.cu:49-81)
                         if (TOUCH) touch_gradient(A, p);
-                        const float k = vr_shade(A, gradient_kept(A, tri));
+                        const float k = vr_shade(A, gradient_kept<TAB>(A, tri, s.terms));
                         vr_composite(sum, aAcc, *slot, a, col.x, col.y, col.z, k);
                     }
                 }
@@ -668,7 +761,7 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
 // composited in order up to there with the same per-sample code as vr_pixel.
 // The frame is the same, bit for bit, as one pass's. gobj: this group's
 // per-label accumulators in LDS (MASK).
-template <bool MASK, bool LDS_LUT, int G>
+template <bool MASK, bool LDS_LUT, int G, bool TAB = false>
 __device__ inline void vr_tail(const RenderArgs& A, const Smem& s, float* gobj) {
     static_assert(G == 4 || G == 8 || G == 16 || G == 32 || G == 64, "group size");
     const int lane = __lane_id();
@@ -748,7 +841,7 @@ __device__ inline void vr_tail(const RenderArgs& A, const Smem& s, float* gobj) 
                 const float4 aw = s.aw[label];
                 const int4 lb = s.lab[label];
                 TriSample tri;
-                float t = sample_keep(A.v, p, tri);
+                float t = sample_keep<TAB>(A.v, s.terms, p, tri);
                 t = (t - aw.z) / aw.y + 0.5f;
                 t = t > 1.0f ? 1.0f : t;
                 const float4 col = lut_sample<LDS_LUT>(A, s.lut, lb.x, lb.y, t);
@@ -757,7 +850,7 @@ __device__ inline void vr_tail(const RenderArgs& A, const Smem& s, float* gobj) 
                 // stays there, and its samples never composite
                 const float held = MASK ? gobj[lb.z] : aobj0;
                 if (a > 0.0005f && held < aw.x) {
-                    const float k = vr_shade(A, gradient_kept(A, tri));
+                    const float k = vr_shade(A, gradient_kept<TAB>(A, tri, s.terms));
                     kr = col.x * k;
                     kg = col.y * k;
                     kb = col.z * k;
@@ -863,7 +956,7 @@ __device__ inline void vr_tail(const RenderArgs& A, const Smem& s, float* gobj) 
 // MIP's march over one ray, one lattice point at a time (d_renderMIP,
 // core/kernel.cu:266-364): a cell whose bound cannot exceed the running
 // maximum leaves it unchanged and is skipped.
-template <bool MASK, bool TOUCH>
+template <bool MASK, bool TOUCH, bool TAB = false>
 __device__ inline void mip_march_single(const RenderArgs& A, const Smem& s, float u, float v, const RayGeom& g,
                                         float acc, float a_out, float& aAcc) {
     const float L1 = A.l1;
@@ -893,7 +986,13 @@ __device__ inline void mip_march_single(const RenderArgs& A, const Smem& s, floa
                 const float4 aw = s.aw[label];
                 if (aw.x > 0.f) {
                     if (TOUCH) touch_footprint(A, p[0], p[1], p[2]);
-                    float t = mk_sample_hu(A.v, p[0], p[1], p[2]);
+                    float t;
+                    if constexpr (TAB) {
+                        TriSample tri;
+                        t = sample_keep<true>(A.v, s.terms, p, tri);
+                    } else {
+                        t = mk_sample_hu(A.v, p[0], p[1], p[2]);
+                    }
                     t = (t - aw.z) / aw.y + 0.5f;
                     if (aAcc < t) aAcc = t;
                 }
@@ -912,7 +1011,7 @@ __device__ inline void mip_march_single(const RenderArgs& A, const Smem& s, floa
 // then they are folded into the maximum. Deciding a skip against the maximum
 // from before the group only skips less, so the result is the same as one
 // point at a time.
-template <bool MASK, bool TOUCH>
+template <bool MASK, bool TOUCH, bool TAB = false>
 __device__ inline void mip_march_grouped(const RenderArgs& A, const Smem& s, float u, float v, const RayGeom& g,
                                          float acc, float a_out, float& aAcc) {
     constexpr int kMipGroup = 4;
@@ -971,7 +1070,8 @@ __device__ inline void mip_march_grouped(const RenderArgs& A, const Smem& s, flo
         for (int j = 0; j < kMipGroup; ++j) {
             if (want[j] && s.aw[lab[j]].x > 0.f) {
                 if (TOUCH) touch_footprint(A, pp[j][0], pp[j][1], pp[j][2]);
-                sample_fetch(A.v, pp[j], fr[j]);
+                if constexpr (TAB) sample_fetch(A.v, s.terms, pp[j], fr[j]);
+                else sample_fetch(A.v, pp[j], fr[j]);
             } else {
                 want[j] = false;
             }
@@ -998,7 +1098,7 @@ __device__ inline void mip_march_grouped(const RenderArgs& A, const Smem& s, flo
 // march takes four lattice points at a time (measured faster); without one a
 // group's fetches did not stay in flight together, and one point at a time is
 // faster.
-template <bool MASK, bool TOUCH>
+template <bool MASK, bool TOUCH, bool TAB = false>
 __device__ inline void mip_pixel(const RenderArgs& A, const Smem& s, int x, int y) {
     const float u = ((float)x - (float)A.width / 2.0f - A.xT) / (float)A.height;
     const float v = ((float)y - (float)A.height / 2.0f - A.yT) / (float)A.height;
@@ -1006,8 +1106,8 @@ __device__ inline void mip_pixel(const RenderArgs& A, const Smem& s, int x, int 
     float a_in, a_out;
     RayGeom g;
     if (ray_interval(A, u, v, g, a_in, a_out)) {
-        if constexpr (MASK) mip_march_grouped<MASK, TOUCH>(A, s, u, v, g, start_acc(A, a_in), a_out, aAcc);
-        else mip_march_single<MASK, TOUCH>(A, s, u, v, g, start_acc(A, a_in), a_out, aAcc);
+        if constexpr (MASK) mip_march_grouped<MASK, TOUCH, TAB>(A, s, u, v, g, start_acc(A, a_in), a_out, aAcc);
+        else mip_march_single<MASK, TOUCH, TAB>(A, s, u, v, g, start_acc(A, a_in), a_out, aAcc);
     }
     float4 c = aAcc <= 0.f ? bg4(A) : make_float4(aAcc, aAcc, aAcc, aAcc);
     store_rgb(A, x, y, c);

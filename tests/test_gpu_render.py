@@ -98,6 +98,34 @@ def test_build_mode_step_exit_4096_lut(gpu_ctx, small):
         assert frac >= MIN_FRAC, (ang, frac, dmax)
 
 
+def test_all_65_labels_vr_and_mip(gpu_ctx, small):
+    """Labels 0..64 all present and visible. VR keeps one accumulator per
+    label and thread in LDS, so its tiles shrink to 8x8; MIP keeps none and
+    must render every pixel with its own 16x16 tiles."""
+    vol, lut0 = small
+    z, y, x = np.meshgrid(np.arange(128), np.arange(128), np.arange(128), indexing="ij")
+    mask = (1 + (x // 9 + 3 * (y // 11) + 7 * (z // 13)) % 64).astype(np.uint8)
+    mask[:, :, :20] = 0
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_mask(mask)
+    luts, labels = {}, {}
+    for l in range(65):
+        luts[l] = lut0 if l == 0 else P.transfer_function(W.tf_points(200 + l))
+        labels[l] = P.LabelState(0.3 + 0.7 * ((l * 37) % 11) / 10.0, 1400.0 + 10.0 * l, -150.0 + 5.0 * l)
+    for l, a in luts.items():
+        gpu_ctx.set_lut(l, a)
+    try:
+        for rt in (MK_RENDER_VR, MK_RENDER_MIP):
+            for ang in [(0, 0), (35, 20)]:
+                p = _params(rt, ang, size=160, labels=labels)
+                got = gpu_ctx.render(p)
+                ref = O.render(p, vol, luts, mask_zyx=mask)
+                frac, dmax = frac_within(got, ref)
+                assert frac >= MIN_FRAC, (rt, ang, frac, dmax)
+    finally:
+        gpu_ctx.clear_mask()
+
+
 def test_multilabel_mask(gpu_ctx, small):
     """Four ellipsoid labels, each with its own LUT, window and alpha cap."""
     vol, lut0 = small
