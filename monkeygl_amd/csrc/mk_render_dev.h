@@ -642,6 +642,10 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
         uint32_t cur_brick = 0xFFFFFFFFu;
         int cur_macro_empty = 0, cur_occ = 1, cur_label = 0xFF;
         int trips = 0, it_queued = -1;
+        // finished first-pass waves as read at the previous poll: a poll's
+        // load is used 16 trips later, so its round trip overlaps the ray's
+        // own loads instead of stalling the wave
+        uint32_t seen = 0;
         for (int it = 0; it < A.max_iter && acc <= kAccEnd && acc <= a_out; ++it) {
             // a ray still marching after the budget, or when most of the
             // first pass's waves have finished (this wave is in its tail), is
@@ -649,10 +653,9 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
             // exit on the same trip
             if (!TOUCH && A.cont.cap > 0) {
                 if (trips == A.cont.budget) { it_queued = it; break; }
-                if ((trips & 15) == 15 &&
-                    __hip_atomic_load(&A.cont.counters[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= A.cont.tail_waves) {
-                    it_queued = it;
-                    break;
+                if ((trips & 15) == 15) {
+                    if (seen >= A.cont.tail_waves) { it_queued = it; break; }
+                    seen = __hip_atomic_load(&A.cont.counters[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 ++trips;
             }
@@ -678,6 +681,11 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
             // (kernel.cu:84-125) on their own: the sample's state (corners,
             // colour, label) then never has to meet the empty case's zeros.
             if (!empty_shift) {
+                // the corner loads are issued before the mask label is read,
+                // so that both are in flight together
+                TriFetch fr;
+                if constexpr (TAB) sample_fetch(A.v, s.terms, p, fr);
+                else sample_fetch(A.v, p, fr);
                 int label = 0;
                 if (MASK) label = (TOUCH || cur_label == 0xFF) ? A.v.mask[mk_mask_offset(A.v, cx, cy, cz)] : cur_label;
                 if (TOUCH) {
@@ -687,7 +695,7 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
                 const float4 aw = s.aw[label];
                 const int4 lb = s.lab[label];
                 TriSample tri;
-                float t = sample_keep<TAB>(A.v, s.terms, p, tri);
+                float t = sample_finish(fr, tri);
                 t = (t - aw.z) / aw.y + 0.5f;
                 t = t > 1.0f ? 1.0f : t;
                 const float4 col = lut_sample<LDS_LUT>(A, s.lut, lb.x, lb.y, t);
@@ -836,12 +844,16 @@ __device__ inline void vr_tail(const RenderArgs& A, const Smem& s, float* gobj) 
                 cur_label = MASK ? A.blabel[b] : 0;
             }
             if (!(cur_macro_empty || !cur_occ)) {
+                // corners first, then the mask label: both in flight together
+                TriFetch fr;
+                if constexpr (TAB) sample_fetch(A.v, s.terms, p, fr);
+                else sample_fetch(A.v, p, fr);
                 int label = 0;
                 if (MASK) label = cur_label == 0xFF ? A.v.mask[mk_mask_offset(A.v, cx, cy, cz)] : cur_label;
                 const float4 aw = s.aw[label];
                 const int4 lb = s.lab[label];
                 TriSample tri;
-                float t = sample_keep<TAB>(A.v, s.terms, p, tri);
+                float t = sample_finish(fr, tri);
                 t = (t - aw.z) / aw.y + 0.5f;
                 t = t > 1.0f ? 1.0f : t;
                 const float4 col = lut_sample<LDS_LUT>(A, s.lut, lb.x, lb.y, t);
