@@ -384,6 +384,7 @@ extern "C" mk_status mk_batch_submit(mk_batcher* b, int slot, const mk_batch_ite
             A.occ2 = sk.occ2;
             A.mipb2 = sk.mipb2;
             A.blabel = c->stats.blabel;
+            A.bskip = sk.bskip;
             A.nmx = (c->nbx + 3) / 4;
             A.nmy = (c->nby + 3) / 4;
             A.steps = c->d_steps;

@@ -46,13 +46,19 @@ constexpr int kVrResidentWavesPerCu = 12;   // 4 SIMDs x 3 waves of the first pa
 
 // Skip tables of one render type. occ/occ2: brick / macro cell (4x4x4 bricks)
 // may contribute (VR, Surface); mipb/mipb2: upper bound of the windowed value
-// (MIP). Valid for the (window/alpha palette, LUT version, volume version)
-// they were built from.
+// (MIP); bskip: per brick, what a ray entering it needs in one 16-bit load
+// (MK_BSKIP_*: its only label, or 0xFF when mixed; occ; its macro cell's occ2).
+// Valid for the (window/alpha palette, LUT version, volume version) they were
+// built from.
+#define MK_BSKIP_LABEL 0x00FFu
+#define MK_BSKIP_OCC 0x0100u
+#define MK_BSKIP_MACRO 0x0200u
 struct MkSkipTables {
     uint8_t* occ = nullptr;
     float* mipb = nullptr;
     uint8_t* occ2 = nullptr;
     float* mipb2 = nullptr;
+    uint16_t* bskip = nullptr;
     bool valid = false;
     uint32_t lut_version = 0;
     uint32_t vol_version = 0;

@@ -190,6 +190,18 @@ __global__ void __launch_bounds__(256) k_macro(const uint8_t* __restrict__ occ, 
     else occ2[m] = any;
 }
 
+// The one-load brick entries (MK_BSKIP_*) from the brick and macro-cell
+// occupancy and the brick labels.
+__global__ void __launch_bounds__(256) k_brick_skip(const uint8_t* __restrict__ occ, const uint8_t* __restrict__ occ2,
+                                                     const uint8_t* __restrict__ blabel, int nbx, int nby, int nmx,
+                                                     int nmy, uint32_t nb, uint16_t* __restrict__ bskip) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const uint32_t bx = b % (uint32_t)nbx, by = (b / (uint32_t)nbx) % (uint32_t)nby, bz = b / ((uint32_t)nbx * nby);
+    const uint32_t m = ((bz >> 2) * (uint32_t)nmy + (by >> 2)) * (uint32_t)nmx + (bx >> 2);
+    bskip[b] = (uint16_t)(blabel[b] | (occ[b] ? MK_BSKIP_OCC : 0u) | (occ2[m] ? MK_BSKIP_MACRO : 0u));
+}
+
 mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev_rgb, void* stream,
                         uint8_t* touch_v, uint8_t* touch_m) {
     mk_status st = mk_validate_render(ctx, p);
@@ -220,6 +232,7 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     A.mipb = sk.mipb;
     A.occ2 = sk.occ2;
     A.blabel = ctx->stats.blabel;
+    A.bskip = sk.bskip;
     A.mipb2 = sk.mipb2;
     A.nmx = (ctx->nbx + 3) / 4;
     A.nmy = (ctx->nby + 3) / 4;
@@ -403,6 +416,11 @@ mk_status mk_ensure_skip_tables(mk_context* ctx, int render_type, const mk_alpha
     hipLaunchKernelGGL(k_macro, dim3((nm + 255) / 256), dim3(256), 0, ctx->stream, sk.occ, sk.mipb, ctx->nbx,
                        ctx->nby, ctx->nbz, nmx, nmy, nmz, render_type, sk.occ2, sk.mipb2);
     MK_HIP(ctx, hipGetLastError());
+    if (render_type != MK_RENDER_MIP) {
+        hipLaunchKernelGGL(k_brick_skip, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, sk.occ, sk.occ2,
+                           ctx->stats.blabel, ctx->nbx, ctx->nby, nmx, nmy, nb, sk.bskip);
+        MK_HIP(ctx, hipGetLastError());
+    }
     sk.valid = true;
     sk.lut_version = ctx->lut_version;
     sk.vol_version = ctx->vol_version;

@@ -41,6 +41,7 @@ struct RenderArgs {
     const float* mipb;     // per 8^3 brick (MIP)
     const uint8_t* occ2;   // per 32^3 macro cell (4x4x4 bricks)
     const uint8_t* blabel; // per brick: its only label, or 0xFF when mixed
+    const uint16_t* bskip; // per brick: label, occ and macro occ2 (MK_BSKIP_*)
     const float* mipb2;
     int nmx, nmy;          // macro cell dims (x, y)
     const float* steps;
@@ -671,9 +672,10 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
                 const uint32_t b = mk_brick_index(A.v, cx, cy, cz);
                 if (b != cur_brick) {
                     cur_brick = b;
-                    cur_macro_empty = !A.occ2[macro_index(A, cx, cy, cz)];
-                    cur_occ = A.occ[b];
-                    cur_label = MASK ? A.blabel[b] : 0;
+                    const uint32_t e = A.bskip[b];
+                    cur_macro_empty = !(e & MK_BSKIP_MACRO);
+                    cur_occ = (e & MK_BSKIP_OCC) != 0;
+                    cur_label = MASK ? (int)(e & MK_BSKIP_LABEL) : 0;
                 }
                 empty_shift = cur_macro_empty ? 5 : (cur_occ ? 0 : 3);
             }
@@ -839,9 +841,10 @@ __device__ inline void vr_tail(const RenderArgs& A, const Smem& s, float* gobj) 
             const uint32_t b = mk_brick_index(A.v, cx, cy, cz);
             if (b != cur_brick) {
                 cur_brick = b;
-                cur_macro_empty = !A.occ2[macro_index(A, cx, cy, cz)];
-                cur_occ = A.occ[b];
-                cur_label = MASK ? A.blabel[b] : 0;
+                const uint32_t e = A.bskip[b];
+                cur_macro_empty = !(e & MK_BSKIP_MACRO);
+                cur_occ = (e & MK_BSKIP_OCC) != 0;
+                cur_label = MASK ? (int)(e & MK_BSKIP_LABEL) : 0;
             }
             if (!(cur_macro_empty || !cur_occ)) {
                 // corners first, then the mask label: both in flight together

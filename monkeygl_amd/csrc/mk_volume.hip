@@ -118,6 +118,7 @@ static void mk_free_volume(mk_context* ctx) {
         mk_free(t.mipb);
         mk_free(t.occ2);
         mk_free(t.mipb2);
+        mk_free(t.bskip);
         t.valid = false;
     }
     ctx->nx = ctx->ny = ctx->nz = 0;
@@ -479,6 +480,7 @@ static mk_status mk_volume_from_device(mk_context* ctx, const int16_t* d_lin, in
         MK_HIP(ctx, hipMalloc((void**)&t.mipb, nb * sizeof(float)));
         MK_HIP(ctx, hipMalloc((void**)&t.occ2, nm * sizeof(uint8_t)));
         MK_HIP(ctx, hipMalloc((void**)&t.mipb2, nm * sizeof(float)));
+        MK_HIP(ctx, hipMalloc((void**)&t.bskip, nb * sizeof(uint16_t)));
     }
     hipLaunchKernelGGL((k_brickify<int16_t, false>), dim3((uint32_t)nb), dim3(512), 0, ctx->stream, d_lin, ctx->d_vox, nx, ny, nz,
                        ctx->nbx, ctx->nby);
