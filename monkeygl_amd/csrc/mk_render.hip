@@ -63,12 +63,14 @@ constexpr size_t kTermsLdsMax = 32 * 1024;           // axis term tables staged 
 
 using namespace mkr;
 
-// With the LUTs in LDS the first pass fits 96 VGPRs: 5 waves per SIMD instead
-// of 4 (1% faster on the body views). TAB: the samplers read the axis term
-// tables staged after the accumulators (one dword spilled in the shading
-// branch at 96 VGPRs; still faster than composing the terms).
+// With the LUTs in LDS and the address terms composed, the first pass fits 96
+// VGPRs: 5 waves per SIMD instead of 4 (1% faster on the body views). TAB:
+// the samplers read the axis term tables staged after the accumulators; the
+// compiler then takes ~99 VGPRs (4 waves per SIMD), which measured faster than
+// holding it to 96 with two dwords spilled to scratch (whose write-backs
+// doubled the pass's HBM writes).
 template <int TW, bool MASK, bool LDS_LUT, bool TOUCH, bool TAB>
-__global__ void __launch_bounds__(TW * TW, (LDS_LUT && !TOUCH) ? 5 : 1) k_vr(RenderArgs A, AwTable T) {
+__global__ void __launch_bounds__(TW * TW, (LDS_LUT && !TOUCH && !TAB) ? 5 : 1) k_vr(RenderArgs A, AwTable T) {
     Smem s = stage_shared(A, T, LDS_LUT, A.nslot * TW * TW, TAB);
     int x, y;
     if (!tile_pixel<TW>(A, x, y)) return;
