@@ -98,6 +98,57 @@ def test_build_mode_step_exit_4096_lut(gpu_ctx, small):
         assert frac >= MIN_FRAC, (ang, frac, dmax)
 
 
+def test_long_volume_composes_address_terms(gpu_ctx):
+    """A 24x20x8200 volume (40 x 40 x 0.1 mm voxels, about cubic in space):
+    its per-axis address-term tables would take 33 KiB, above the 32 KiB the
+    kernels stage in LDS, so VR (with and without a mask) and MIP compose the
+    terms from bit fields instead. Both paths against the FP64 oracle."""
+    dims, spacing = (24, 20, 8200), (40.0, 40.0, 0.1)
+    nx, ny, nz = dims
+    zz, yy, xx = np.meshgrid(np.arange(nz) * spacing[2], np.arange(ny) * spacing[1], np.arange(nx) * spacing[0],
+                             indexing="ij")
+    rng = np.random.default_rng(11)
+    vol = np.full((nz, ny, nx), -1000.0, np.float32)
+    ext = np.array([nx * spacing[0], ny * spacing[1], nz * spacing[2]])
+    for _ in range(5):
+        c = rng.uniform(0.2, 0.8, 3) * ext
+        r = rng.uniform(0.12, 0.25) * ext.min()
+        d2 = ((xx - c[0]) ** 2 + (yy - c[1]) ** 2 + (zz - c[2]) ** 2) / r ** 2
+        vol += rng.uniform(800, 2000) * np.exp(-d2).astype(np.float32)
+    vol = np.clip(vol, -1024, 3000).astype(np.int16)
+    mask = np.zeros(vol.shape, np.uint8)
+    mask[: nz // 2, :, : nx // 2] = 1
+    mask[nz // 3:, ny // 2:, :] = 2
+    lut0 = P.transfer_function(W.tf_points(0))
+    luts = {0: lut0, 1: P.transfer_function(W.tf_points(31)), 2: P.transfer_function(W.tf_points(32))}
+    labels = {0: P.LabelState(1.0, 2000.0, 0.0), 1: P.LabelState(0.5, 1600.0, 100.0), 2: P.LabelState(0.8, 1800.0, -50.0)}
+    gpu_ctx.set_volume(vol)
+    for l, a in luts.items():
+        gpu_ctx.set_lut(l, a)
+
+    def check(rt, ang, lab, use_mask):
+        info = P.RenderInfo()
+        info.set_vr_size(160, 160)
+        info.rotate(*ang)
+        p = P.render_params(dims, spacing, info, lab, render_type=rt)
+        got = gpu_ctx.render(p)
+        ref = O.render(p, vol, {l: luts[l] for l in lab}, mask_zyx=mask if use_mask else None)
+        frac, dmax = frac_within(got, ref)
+        assert frac >= MIN_FRAC, (rt, ang, use_mask, frac, dmax)
+        assert (ref.sum(-1) > 0).mean() > 0.2, "the view should show the volume"
+
+    for ang in [(0, 0), (30, 40)]:
+        check(MK_RENDER_VR, ang, {0: labels[0]}, False)
+        check(MK_RENDER_MIP, ang, {0: labels[0]}, False)
+    gpu_ctx.set_mask(mask)
+    try:
+        for ang in [(0, 0), (30, 40)]:
+            check(MK_RENDER_VR, ang, labels, True)
+            check(MK_RENDER_MIP, ang, labels, True)
+    finally:
+        gpu_ctx.clear_mask()
+
+
 def test_all_65_labels_vr_and_mip(gpu_ctx, small):
     """Labels 0..64 all present and visible. VR keeps one accumulator per
     label and thread in LDS, so its tiles shrink to 8x8; MIP keeps none and
