@@ -446,8 +446,10 @@ const char* mk_batcher_last_error(const mk_batcher* b);
  * must be rebuilt first waits for every batch in flight. Within one batch,
  * the render items of one context share one step_scale, and those of one
  * context and render type one alpha_wwwl table (MK_ERR_ARG otherwise). A
- * context used by a batcher must not be rendered through the one-request
- * entry points while its batches run. */
+ * batch of one render item is rendered as mk_render_async renders it (VR in
+ * two passes), same image, after the same checks. A context used by a batcher
+ * must not be rendered through the one-request entry points while its
+ * batches run. */
 mk_status mk_batch_submit(mk_batcher* b, int slot, const mk_batch_item* items, int n, uint8_t* dev_arena,
                           uint64_t arena_bytes, void* stream);
 

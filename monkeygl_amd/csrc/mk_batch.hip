@@ -12,6 +12,10 @@
 // mk_render / mk_render_plane / mk_render_cpr. One kernel per variant, rather
 // than one kernel branching over variants, keeps each at the register count
 // of its one-request twin (a kernel holding all variants needs ~250 VGPRs).
+// A batch of one render request skips the graph and takes the one-request
+// launch path (VR in two passes, address-term tables in LDS): at the served
+// mix's arrival rate most batches hold one request, and that path renders
+// them faster with the same pixels.
 //
 // Launch: each slot owns a pinned host descriptor block, its device copy and
 // a hipGraph built once at creation: [copy the descriptor block] -> [variant 0
@@ -481,6 +485,21 @@ extern "C" mk_status mk_batch_submit(mk_batcher* b, int slot, const mk_batch_ite
     }
     H->n_items = n;
 
+    hipStream_t st = (hipStream_t)stream;
+    // A batch of one render request runs on the one-request launch path: VR in
+    // two passes (the batched kernel marches every ray to its end in one), VR
+    // and MIP sampling through the LDS address tables. The frame is the same;
+    // the tables it reads were made current above, after the batches in
+    // flight, so the path rebuilds nothing.
+    if (n == 1 && items[0].kind == MK_ITEM_RENDER) {
+        mk_context* c = b->ctxs[items[0].context];
+        mk_status r = mk_render_async(c, &items[0].render, dev_arena + items[0].out_offset, stream);
+        if (r) return bfail(b, r, std::string("item 0: ") + c->err);
+        B_HIP(b, hipEventRecord(s.done, st));
+        s.busy = true;
+        return MK_OK;
+    }
+
     B_HIP(b, hipGraphExecMemcpyNodeSetParams1D(s.exec, s.copy, s.d, s.h, b->lay.bytes(n), hipMemcpyHostToDevice));
     for (int v = 0; v < V_COUNT; ++v) {
         const bool used = H->var[v].tiles > 0;
@@ -493,7 +512,6 @@ extern "C" mk_status mk_batch_submit(mk_batcher* b, int slot, const mk_batch_ite
             s.enabled[v] = used;
         }
     }
-    hipStream_t st = (hipStream_t)stream;
     for (mk_context* c : b->ctxs) {
         mk_status r = mk_order_after_context(c, st);
         if (r) return bfail(b, r, c->err);
