@@ -21,6 +21,7 @@
 // it the adjoint first re-marches the ray for C. No sample is stored.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -115,7 +116,7 @@ __device__ inline Sample fit_sample(const FitArgs& A, const float4* s_lut, const
     return S;
 }
 
-__device__ inline bool fit_ray(const FitArgs& A, int x, int y, float& u, float& v, float& acc, float& a_out) {
+__device__ inline bool fit_ray(const FitArgs& A, int x, int y, float& u, float& v, int& k0, float& a_out) {
     u = ((float)x - (float)A.width / 2.0f - A.xT) / (float)A.height;
     v = ((float)y - (float)A.height / 2.0f - A.yT) / (float)A.height;
     float p0[3];
@@ -141,9 +142,8 @@ __device__ inline bool fit_ray(const FitArgs& A, int x, int y, float& u, float& 
     t0 = fmaxf(t0 - 4.0f * A.l1, 0.0f);
     a_out = t1 + 4.0f * A.l1;
     if (t0 > a_out) return false;
-    int k0 = (int)(t0 / A.l1) - 2;
+    k0 = (int)(t0 / A.l1) - 2;
     k0 = k0 < 0 ? 0 : (k0 >= A.nsteps ? A.nsteps - 1 : k0);
-    acc = A.steps[k0];
     return true;
 }
 
@@ -153,92 +153,267 @@ __device__ inline float3 fit_light(const FitArgs& A) {
     return make_float3(l.x * r, l.y * r, l.z * r);
 }
 
+// A workgroup renders a 16x16 tile as four waves of 8x8 rays.
+constexpr int kFitTile = 16;
+constexpr int kFitWaves = 4;
 __device__ inline void fit_tile(const FitArgs& A, int& x, int& y) {
-    const uint32_t tiles_x = (A.width + 15) / 16;
-    const uint32_t t = mk_xcd_row_tile(blockIdx.x, tiles_x, (A.height + 15) / 16);
+    const uint32_t tiles_x = (A.width + kFitTile - 1) / kFitTile;
+    const uint32_t t = mk_xcd_row_tile(blockIdx.x, tiles_x, (A.height + kFitTile - 1) / kFitTile);
     int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    x = (int)(t % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
-    y = (int)(t / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    x = (int)(t % tiles_x) * kFitTile + (wave & 1) * 8 + (lane & 7);
+    y = (int)(t / tiles_x) * kFitTile + (wave >> 1) * 8 + (lane >> 3);
 }
 
-// Lattice points marched per group: the group's positions do not depend on
-// its samples (fixed step), so all their corner and gradient loads are issued
-// before the first is used and share one round trip. One point already issues
-// its 8 corners and 24 gradient taps together; groups of 2, 3 or 4 measured
-// the same (the loads are bound by the L1 cache's access rate: 64-lane
-// gathers touch ~34 cache lines each, and 99.8% hit) and need more VGPRs.
-constexpr int kFitGroup = 1;
-// dynamic LDS of one fitting workgroup: the LUT (16 B per entry, up to 4096
-// entries) and, in the adjoint, its gradient sums when they fit beside it
-constexpr size_t kFitLdsBytes = 64 * 1024;
+// Orders this wave's LDS writes before its following reads (and its reads
+// before the next writes): the wave's lanes share its staged box, and no
+// other wave touches it.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
 
-// Visits the ray's in-VOI lattice points in order: visit(S, k) gets the
-// sample and its shading factor and returns false to end the march. The
-// points are those of the loop
+// dynamic LDS of one fitting workgroup: the LUT (16 B per entry, up to 4096
+// entries), in the adjoint its gradient sums when they fit beside it, and the
+// staged blocks of the volume
+constexpr size_t kFitLdsBytes = 64 * 1024;
+constexpr int kFitChunk = 8;           // lattice points per staged chunk
+constexpr int kFitBoxBlocks = 60;      // 4x4x4 blocks staged per chunk at most (7.5 KiB)
+
+// The staged box: the 4x4x4 blocks [b0, b0 + nb) (block units) copied from
+// the bricked volume in their own layout (128 B each, x-then-y-then-z), so a
+// voxel is read at the sum of one term per axis, as in HBM.
+struct FitBox {
+    const uint8_t* base;
+    int bx0, by0, bz0;
+    uint32_t sy, sz;   // bytes per block row, per block slab
+    // per-axis terms of the box offset for unclamped voxel indices from
+    // 4 * b0 - 4 on (clamping folded in), built with the box
+    const uint32_t* tx;
+    const uint32_t* ty;
+    const uint32_t* tz;
+};
+// table entries per axis for a box nb blocks long, and for any box within the
+// cap (the sum over axes of nb is at most cap + 2)
+__host__ __device__ constexpr int box_term_len(int nb) { return 4 * nb + 16; }
+__host__ __device__ constexpr int box_terms_max(int cap) { return 4 * (cap + 2) + 48; }
+__device__ inline uint32_t box_tx(const FitBox& B, int x) { return (uint32_t)(((x >> 2) - B.bx0) * 128 + (x & 3) * 2); }
+__device__ inline uint32_t box_ty(const FitBox& B, int y) { return (uint32_t)((y >> 2) - B.by0) * B.sy + (uint32_t)((y & 3) * 8); }
+__device__ inline uint32_t box_tz(const FitBox& B, int z) { return (uint32_t)((z >> 2) - B.bz0) * B.sz + (uint32_t)((z & 3) * 32); }
+__device__ inline int box_vox(const FitBox& B, uint32_t off) { return *(const int16_t*)(B.base + off); }
+
+// Fills the box's term tables (the wave's lanes; the caller orders them
+// before use). Entry e of an axis holds the term of voxel index
+// clamp(4 * b0 - 4 + e, 0, n - 1), held inside the box so that every entry
+// addresses it.
+__device__ inline void box_terms_build(const MkVolView& v, const FitBox& B, int nbx, int nby, int nbz, uint32_t* t) {
+    const int lx = box_term_len(nbx), ly = box_term_len(nby), lz = box_term_len(nbz);
+    for (int e = threadIdx.x & 63; e < lx + ly + lz; e += 64) {
+        uint32_t w;
+        if (e < lx) {
+            const int i = mk_clampi(mk_clampi(4 * B.bx0 - 4 + e, 0, v.nx - 1), 4 * B.bx0, 4 * (B.bx0 + nbx) - 1);
+            w = box_tx(B, i);
+        } else if (e < lx + ly) {
+            const int i = mk_clampi(mk_clampi(4 * B.by0 - 4 + (e - lx), 0, v.ny - 1), 4 * B.by0, 4 * (B.by0 + nby) - 1);
+            w = box_ty(B, i);
+        } else {
+            const int i =
+                mk_clampi(mk_clampi(4 * B.bz0 - 4 + (e - lx - ly), 0, v.nz - 1), 4 * B.bz0, 4 * (B.bz0 + nbz) - 1);
+            w = box_tz(B, i);
+        }
+        t[e] = w;
+    }
+}
+
+// sample_fetch and grad_taps_fetch from the box: the same clamped indices,
+// so the same voxel values. Floor indices lie in [-3, n + 2], so the entries
+// read, ix + 3 .. ix + 6 - 4 * b0, lie in the tables.
+__device__ inline void box_fetch(const MkVolView& v, const FitBox& B, const float p[3], mkr::TriFetch& F,
+                                 mkr::GradTaps& G) {
+    mkr::grad_axis_base(p[0], v.nx, F.ix, F.ax);
+    mkr::grad_axis_base(p[1], v.ny, F.iy, F.ay);
+    mkr::grad_axis_base(p[2], v.nz, F.iz, F.az);
+    const uint32_t* tx = B.tx + (F.ix + 3 - 4 * B.bx0);
+    const uint32_t* ty = B.ty + (F.iy + 3 - 4 * B.by0);
+    const uint32_t* tz = B.tz + (F.iz + 3 - 4 * B.bz0);
+    uint32_t OX[4], OY[4], OZ[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        OX[k] = tx[k];
+        OY[k] = ty[k];
+        OZ[k] = tz[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) F.r[k] = box_vox(B, OX[1 + (k & 1)] + OY[1 + ((k >> 1) & 1)] + OZ[1 + (k >> 2)]);
+#define MK_BV(a, b, c) ((float)max(box_vox(B, OX[a] + OY[b] + OZ[c]), -32767))
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            G.x[k][j][0] = MK_BV(0, 1 + j, 1 + k);
+            G.x[k][j][1] = MK_BV(3, 1 + j, 1 + k);
+            G.y[k][j][0] = MK_BV(1 + j, 0, 1 + k);
+            G.y[k][j][1] = MK_BV(1 + j, 3, 1 + k);
+            G.z[k][j][0] = MK_BV(1 + j, 1 + k, 0);
+            G.z[k][j][1] = MK_BV(1 + j, 1 + k, 3);
+        }
+#undef MK_BV
+}
+
+// LDS bytes of one wave's region: the box, then its term tables.
+__host__ __device__ constexpr int box_region(int cap) { return cap * 128 + 4 * box_terms_max(cap); }
+
+__device__ inline int wave_min(int x) {
+    for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
+    return x;
+}
+__device__ inline int wave_max(int x) {
+    for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o));
+    return x;
+}
+
+// Visits each active ray's in-VOI lattice points in order: visit(S, k) gets
+// the sample and its shading factor and returns false to end that ray's
+// march. A ray's points are those of the loop
 //   for (it = 0; it < max_iter && (double)acc < 1.732 && acc <= a_out; ++it)
 //     { p = pos(acc); acc += L1; if in VOI: visit }
-// taken kFitGroup at a time; loads of points past the end read clamped
-// in-volume voxels and are not used.
+// from acc = steps[k0]; the step lattice is shared, so point it of the ray is
+// lattice point k0 + it (acc = steps[k0 + it]). The wave's rays march the
+// lattice together, kFitChunk points at a time: the box of 4x4x4 blocks that
+// holds every corner and gradient tap of the chunk's points is copied to LDS
+// with coalesced loads, and the samples read it there, each 64-lane gather of
+// the HBM path (touching ~30 cache lines) becoming LDS reads. A chunk whose
+// box exceeds the LDS budget reads HBM. s_box is this wave's own region;
+// every lane of the wave calls this (active or not).
 template <typename Visit>
-__device__ inline void fit_march(const FitArgs& A, const float4* s_lut, float u, float v, float acc, float a_out,
-                                 float3 L, Visit&& visit) {
-    int it = 0;
-    bool more = true;
-    while (more) {
-        float P[kFitGroup][3];
-        bool use[kFitGroup];
+__device__ inline void fit_march(const FitArgs& A, const float4* s_lut, uint8_t* s_box, int box_cap, bool active,
+                                 float u, float v, int k0, float a_out, float3 L, Visit&& visit) {
+    const int lane = threadIdx.x & 63;
+    bool done = !active;
+    int c = wave_min(done ? 0x7FFFFFFF : k0);
+    while (__ballot(!done) != 0ull) {
+        c = __builtin_amdgcn_readfirstlane(c);
+        // this lane's points in [c, c + kFitChunk): k in [kb, ke], and whether
+        // its march ends in this chunk
+        int kb = 0x7FFFFFFF, ke = -1;
+        bool ends = false;
+        if (!done) {
 #pragma unroll
-        for (int j = 0; j < kFitGroup; ++j) {
-            const bool ok = more && it < A.max_iter && (double)acc < 1.732 && acc <= a_out;
-            fit_pos(A, u, v, acc, P[j]);
-            use[j] = ok && fit_voi(A, P[j]);
-            acc = ok ? acc + A.l1 : acc;
-            it += ok ? 1 : 0;
-            more = ok;
-        }
-        mkr::TriFetch F[kFitGroup];
-        mkr::GradTaps G[kFitGroup];
-#pragma unroll
-        for (int j = 0; j < kFitGroup; ++j) {
-            mkr::sample_fetch(A.v, P[j], F[j]);
-            mkr::grad_taps_fetch(A.v, F[j].ix, F[j].iy, F[j].iz, G[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < kFitGroup; ++j) {
-            if (!use[j]) continue;
-            mkr::TriSample tri;
-            const Sample S = fit_sample(A, s_lut, F[j], tri);
-            if (!visit(S, fit_shade(A, tri, G[j], L))) {
-                more = false;
-                break;
+            for (int m = 0; m < kFitChunk; ++m) {
+                const int k = c + m;
+                if (k < k0 || ends) continue;
+                const bool ok = k - k0 < A.max_iter && k < A.nsteps && (double)A.steps[k] < 1.732 && A.steps[k] <= a_out;
+                if (ok) {
+                    kb = min(kb, k);
+                    ke = k;
+                } else {
+                    ends = true;
+                }
             }
         }
+        // the box: positions move monotonically along a ray, so the first and
+        // last points bound its indices; taps reach one below, two above
+        int lo[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, hi[3] = {-1, -1, -1};
+        if (ke >= 0) {
+            float p0[3], p1[3];
+            fit_pos(A, u, v, A.steps[kb], p0);
+            fit_pos(A, u, v, A.steps[ke], p1);
+            const int n[3] = {A.v.nx, A.v.ny, A.v.nz};
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                int i0, i1;
+                float w;
+                mkr::grad_axis_base(p0[i], n[i], i0, w);
+                mkr::grad_axis_base(p1[i], n[i], i1, w);
+                lo[i] = mk_clampi(min(i0, i1) - 1, 0, n[i] - 1);
+                hi[i] = mk_clampi(max(i0, i1) + 2, 0, n[i] - 1);
+            }
+        }
+        FitBox B;
+        B.base = s_box;
+        B.bx0 = wave_min(lo[0]) >> 2;
+        B.by0 = wave_min(lo[1]) >> 2;
+        B.bz0 = wave_min(lo[2]) >> 2;
+        const int nbx = (wave_max(hi[0]) >> 2) - B.bx0 + 1;
+        const int nby = (wave_max(hi[1]) >> 2) - B.by0 + 1;
+        const int nbz = (wave_max(hi[2]) >> 2) - B.bz0 + 1;
+        const bool any = __ballot(ke >= 0) != 0ull;
+        const bool staged = any && nbx * nby * nbz <= box_cap;
+        if (staged) {
+            B.sy = (uint32_t)nbx * 128u;
+            B.sz = (uint32_t)(nbx * nby) * 128u;
+            const int nblk = nbx * nby * nbz;
+            uint32_t* t = (uint32_t*)(s_box + box_cap * 128);
+            B.tx = t;
+            B.ty = t + box_term_len(nbx);
+            B.tz = B.ty + box_term_len(nby);
+            box_terms_build(A.v, B, nbx, nby, nbz, t);
+            for (int i = lane; i < nblk * 8; i += 64) {
+                const int blk = i >> 3, part = i & 7;
+                const int X = B.bx0 + blk % nbx, Y = B.by0 + (blk / nbx) % nby, Z = B.bz0 + blk / (nbx * nby);
+                const uint32_t brick =
+                    __umul24(__umul24((uint32_t)(Z >> 1), (uint32_t)A.v.nby) + (uint32_t)(Y >> 1), (uint32_t)A.v.nbx) +
+                    (uint32_t)(X >> 1);
+                const uint32_t off = (brick << 10) | ((uint32_t)(Z & 1) << 9) | ((uint32_t)(Y & 1) << 8) |
+                                     ((uint32_t)(X & 1) << 7) | ((uint32_t)part << 4);
+                *(uint4*)(s_box + blk * 128 + part * 16) = *(const uint4*)((const uint8_t*)A.v.vox + off);
+            }
+            wave_lds_sync();
+        }
+        if (ke >= 0) {
+            for (int k = kb; k <= ke; ++k) {
+                float P[3];
+                fit_pos(A, u, v, A.steps[k], P);
+                if (!fit_voi(A, P)) continue;
+                mkr::TriFetch F;
+                mkr::GradTaps G;
+                if (staged) {
+                    box_fetch(A.v, B, P, F, G);
+                } else {
+                    mkr::sample_fetch(A.v, P, F);
+                    mkr::grad_taps_fetch(A.v, F.ix, F.iy, F.iz, G);
+                }
+                mkr::TriSample tri;
+                const Sample S = fit_sample(A, s_lut, F, tri);
+                if (!visit(S, fit_shade(A, tri, G, L))) {
+                    ends = true;
+                    break;
+                }
+            }
+        }
+        if (ends) done = true;
+        if (staged) wave_lds_sync();   // the next chunk's box overwrites this one
+        c += kFitChunk;
     }
 }
 
 // Forward: float RGB per pixel (width*height*3), and the same colours in
 // double into out64 unless it is null.
-__global__ void __launch_bounds__(256) k_fit_forward(FitArgs A, float* __restrict__ out, double* __restrict__ out64) {
+// box_cap: 4x4x4 blocks of the volume each wave's region of LDS (after the
+// LUT) holds.
+__global__ void __launch_bounds__(256) k_fit_forward(FitArgs A, float* __restrict__ out, double* __restrict__ out64,
+                                                     int box_cap) {
     extern __shared__ float4 s_lut[];
     for (int i = threadIdx.x; i < A.n; i += blockDim.x) s_lut[i] = A.lut[i];
     __syncthreads();
+    uint8_t* s_box = (uint8_t*)(s_lut + A.n) + (threadIdx.x >> 6) * box_region(box_cap);
     int x, y;
     fit_tile(A, x, y);
-    if (x >= A.width || y >= A.height) return;
     double C[3] = {0.0, 0.0, 0.0}, T = 1.0;
-    float u, v, acc, a_out;
-    if (fit_ray(A, x, y, u, v, acc, a_out)) {
-        fit_march(A, s_lut, u, v, acc, a_out, fit_light(A), [&](const Sample& S, float k) {
-            if (S.a > 0.0f) {
-                const double w = T * (double)S.a * (double)k;
-                C[0] += w * S.c.x;
-                C[1] += w * S.c.y;
-                C[2] += w * S.c.z;
-                T *= 1.0 - (double)S.a;
-            }
-            return true;
-        });
-    }
+    float u = 0.f, v = 0.f, a_out = 0.f;
+    int k0 = 0;
+    const bool inside = x < A.width && y < A.height;
+    const bool active = inside && fit_ray(A, x, y, u, v, k0, a_out);
+    fit_march(A, s_lut, s_box, box_cap, active, u, v, k0, a_out, fit_light(A), [&](const Sample& S, float k) {
+        if (S.a > 0.0f) {
+            const double w = T * (double)S.a * (double)k;
+            C[0] += w * S.c.x;
+            C[1] += w * S.c.y;
+            C[2] += w * S.c.z;
+            T *= 1.0 - (double)S.a;
+        }
+        return true;
+    });
+    if (!inside) return;
     float* o = out + 3 * ((size_t)y * A.width + x);
     o[0] = (float)C[0];
     o[1] = (float)C[1];
@@ -258,103 +433,141 @@ __global__ void __launch_bounds__(256) k_fit_forward(FitArgs A, float* __restric
 // it (LUTs too long for both tables in LDS) the lanes add to HBM directly.
 template <bool LDS_GRAD>
 __global__ void __launch_bounds__(256) k_fit_backward(FitArgs A, const float* __restrict__ grad_img,
-                                                      const double* __restrict__ color64,
-                                                      double* __restrict__ grad_lut, double* __restrict__ grad_scale) {
+                                                     const double* __restrict__ color64,
+                                                     double* __restrict__ grad_lut, double* __restrict__ grad_scale,
+                                                     int box_cap) {
     extern __shared__ float4 smem[];
     float4* s_lut = smem;
     double* s_g = LDS_GRAD ? (double*)(smem + A.n) : grad_lut;  // 4*n LUT entries, then the scale
     double* s_scale = LDS_GRAD ? s_g + 4 * A.n : grad_scale;
+    uint8_t* s_box = (LDS_GRAD ? (uint8_t*)(s_g + 4 * A.n + 2) : (uint8_t*)(smem + A.n)) +
+                     (threadIdx.x >> 6) * box_region(box_cap);
     for (int i = threadIdx.x; i < A.n; i += blockDim.x) s_lut[i] = A.lut[i];
     if (LDS_GRAD)
         for (int i = threadIdx.x; i < 4 * A.n + 1; i += blockDim.x) s_g[i] = 0.0;
     __syncthreads();
     int x, y;
     fit_tile(A, x, y);
-    float u, v, acc0, a_out;
-    if (x < A.width && y < A.height && fit_ray(A, x, y, u, v, acc0, a_out)) {
+    float u = 0.f, v = 0.f, a_out = 0.f;
+    int k0 = 0;
+    double g[3] = {0.0, 0.0, 0.0};
+    bool active = x < A.width && y < A.height && fit_ray(A, x, y, u, v, k0, a_out);
+    if (active) {
         const float* gp = grad_img + 3 * ((size_t)y * A.width + x);
-        const double g[3] = {gp[0], gp[1], gp[2]};
-        const float3 L = fit_light(A);
-        if (g[0] != 0.0 || g[1] != 0.0 || g[2] != 0.0) {
-            // the final colour: kept by the forward pass, else a first march
-            double C[3] = {0.0, 0.0, 0.0}, T = 1.0;
-            if (color64) {
-                const double* cp = color64 + 3 * ((size_t)y * A.width + x);
-                C[0] = cp[0];
-                C[1] = cp[1];
-                C[2] = cp[2];
-            } else {
-                fit_march(A, s_lut, u, v, acc0, a_out, L, [&](const Sample& S, float k) {
-                    if (S.a > 0.0f) {
-                        const double w = T * (double)S.a * (double)k;
-                        C[0] += w * S.c.x;
-                        C[1] += w * S.c.y;
-                        C[2] += w * S.c.z;
-                        T *= 1.0 - (double)S.a;
-                    }
-                    return true;
-                });
-            }
-            // gradients, un-compositing the remainder of the ray
-            double Cp[3] = {0.0, 0.0, 0.0};
-            T = 1.0;
-            double gs = 0.0;
-            // LUT gradients of consecutive samples that interpolate the same two
-            // entries are summed in registers and added to LDS when the pair changes
-            int p0 = -1, p1 = -1;
-            double q0[4] = {0.0, 0.0, 0.0, 0.0}, q1[4] = {0.0, 0.0, 0.0, 0.0};
-            auto flush = [&]() {
-                if (p0 < 0) return;
-                for (int ch = 0; ch < 4; ++ch) {
-                    if (q0[ch] != 0.0) atomicAdd(&s_g[4 * p0 + ch], q0[ch]);
-                    if (q1[ch] != 0.0) atomicAdd(&s_g[4 * p1 + ch], q1[ch]);
-                    q0[ch] = 0.0;
-                    q1[ch] = 0.0;
-                }
-            };
-            fit_march(A, s_lut, u, v, acc0, a_out, L, [&](const Sample& S, float kf) {
-                // samples with zero alpha still move the image when their LUT
-                // entries grow: da > 0 from 0 is a smooth onset
-                const double k = (double)kf;
-                const double a = S.a;
-                const double w = T * a * k;
-                const double c[3] = {S.c.x, S.c.y, S.c.z};
-                for (int ch = 0; ch < 3; ++ch) Cp[ch] += w * c[ch];
-                // dL/dc.rgb
-                const double drgb[3] = {g[0] * w, g[1] * w, g[2] * w};
-                // dL/da
-                // an opaque sample (alpha clamped at 1) hides the rest and passes no
-                // gradient to its alpha
-                double da = 0.0;
-                if (S.grad_ok)
-                    for (int ch = 0; ch < 3; ++ch) da += g[ch] * (T * k * c[ch] - (C[ch] - Cp[ch]) / (1.0 - a));
-                T *= 1.0 - a;
-                const double dw = da;
-                // alpha = s * c.w (unclamped region): dL/dc.w = s dL/da, dL/ds = c.w dL/da
-                const double dcw = dw * (double)A.scale;
-                gs += dw * (double)S.c.w;
-                const double f = S.f;
-                const double d4[4] = {drgb[0], drgb[1], drgb[2], dcw};
-                if (S.i0 != p0 || S.i1 != p1) {
-                    flush();
-                    p0 = S.i0;
-                    p1 = S.i1;
-                }
-                for (int ch = 0; ch < 4; ++ch) {
-                    q0[ch] += (1.0 - f) * d4[ch];
-                    q1[ch] += f * d4[ch];
-                }
-                return T != 0.0;
-            });
-            flush();
-            if (gs != 0.0) atomicAdd(s_scale, gs);
-        }
+        g[0] = gp[0];
+        g[1] = gp[1];
+        g[2] = gp[2];
+        active = g[0] != 0.0 || g[1] != 0.0 || g[2] != 0.0;
     }
+    const float3 L = fit_light(A);
+    // the final colour: kept by the forward pass, else a first march
+    double C[3] = {0.0, 0.0, 0.0}, T = 1.0;
+    if (color64) {
+        if (active) {
+            const double* cp = color64 + 3 * ((size_t)y * A.width + x);
+            C[0] = cp[0];
+            C[1] = cp[1];
+            C[2] = cp[2];
+        }
+    } else {
+        fit_march(A, s_lut, s_box, box_cap, active, u, v, k0, a_out, L, [&](const Sample& S, float k) {
+            if (S.a > 0.0f) {
+                const double w = T * (double)S.a * (double)k;
+                C[0] += w * S.c.x;
+                C[1] += w * S.c.y;
+                C[2] += w * S.c.z;
+                T *= 1.0 - (double)S.a;
+            }
+            return true;
+        });
+    }
+    // gradients, un-compositing the remainder of the ray
+    double Cp[3] = {0.0, 0.0, 0.0};
+    T = 1.0;
+    double gs = 0.0;
+    // LUT gradients of consecutive samples that interpolate the same two
+    // entries are summed in registers and added to LDS when the pair changes
+    int p0 = -1, p1 = -1;
+    double q0[4] = {0.0, 0.0, 0.0, 0.0}, q1[4] = {0.0, 0.0, 0.0, 0.0};
+    auto flush = [&]() {
+        if (p0 < 0) return;
+        for (int ch = 0; ch < 4; ++ch) {
+            if (q0[ch] != 0.0) atomicAdd(&s_g[4 * p0 + ch], q0[ch]);
+            if (q1[ch] != 0.0) atomicAdd(&s_g[4 * p1 + ch], q1[ch]);
+            q0[ch] = 0.0;
+            q1[ch] = 0.0;
+        }
+    };
+    fit_march(A, s_lut, s_box, box_cap, active, u, v, k0, a_out, L, [&](const Sample& S, float kf) {
+        // samples with zero alpha still move the image when their LUT
+        // entries grow: da > 0 from 0 is a smooth onset
+        const double k = (double)kf;
+        const double a = S.a;
+        const double w = T * a * k;
+        const double c[3] = {S.c.x, S.c.y, S.c.z};
+        for (int ch = 0; ch < 3; ++ch) Cp[ch] += w * c[ch];
+        // dL/dc.rgb
+        const double drgb[3] = {g[0] * w, g[1] * w, g[2] * w};
+        // dL/da
+        // an opaque sample (alpha clamped at 1) hides the rest and passes no
+        // gradient to its alpha
+        double da = 0.0;
+        if (S.grad_ok) {
+            // one double division per sample (1 - a > 0 here), not one per channel
+            const double r = 1.0 / (1.0 - a);
+            for (int ch = 0; ch < 3; ++ch) da += g[ch] * (T * k * c[ch] - (C[ch] - Cp[ch]) * r);
+        }
+        T *= 1.0 - a;
+        const double dw = da;
+        // alpha = s * c.w (unclamped region): dL/dc.w = s dL/da, dL/ds = c.w dL/da
+        const double dcw = dw * (double)A.scale;
+        gs += dw * (double)S.c.w;
+        const double f = S.f;
+        const double d4[4] = {drgb[0], drgb[1], drgb[2], dcw};
+        if (S.i0 != p0 || S.i1 != p1) {
+            // a held entry that the new pair also uses stays in its register
+            // (along a ray the pair mostly moves by one entry); the others
+            // go to LDS
+            double c0[4] = {0.0, 0.0, 0.0, 0.0}, c1[4] = {0.0, 0.0, 0.0, 0.0};
+            if (p0 >= 0) {
+                for (int ch = 0; ch < 4; ++ch) {
+                    if (p0 == S.i0) c0[ch] += q0[ch];
+                    else if (p0 == S.i1) c1[ch] += q0[ch];
+                    else if (q0[ch] != 0.0) atomicAdd(&s_g[4 * p0 + ch], q0[ch]);
+                    if (p1 == S.i0) c0[ch] += q1[ch];
+                    else if (p1 == S.i1) c1[ch] += q1[ch];
+                    else if (q1[ch] != 0.0) atomicAdd(&s_g[4 * p1 + ch], q1[ch]);
+                }
+            }
+            for (int ch = 0; ch < 4; ++ch) {
+                q0[ch] = c0[ch];
+                q1[ch] = c1[ch];
+            }
+            p0 = S.i0;
+            p1 = S.i1;
+        }
+        for (int ch = 0; ch < 4; ++ch) {
+            q0[ch] += (1.0 - f) * d4[ch];
+            q1[ch] += f * d4[ch];
+        }
+        return T != 0.0;
+    });
+    flush();
+    if (gs != 0.0) atomicAdd(s_scale, gs);
     if (!LDS_GRAD) return;
     __syncthreads();
     for (int i = threadIdx.x; i < 4 * A.n; i += blockDim.x)
         if (s_g[i] != 0.0) atomicAdd(&grad_lut[i], s_g[i]);
     if (threadIdx.x == 0 && s_g[4 * A.n] != 0.0) atomicAdd(grad_scale, s_g[4 * A.n]);
+}
+
+// Blocks of each wave's staged box that fit, with their term tables, in the
+// LDS left after `used` bytes.
+int box_blocks(size_t used) {
+    const size_t left = used < kFitLdsBytes ? kFitLdsBytes - used : 0;
+    int cap = kFitBoxBlocks;
+    while (cap > 0 && (size_t)box_region(cap) * kFitWaves > left) --cap;
+    return cap;
 }
 
 mk_status fit_prepare(mk_context* ctx, const mk_render_params* p, const float* dev_lut, int n, float scale, FitArgs& A) {
@@ -408,8 +621,11 @@ extern "C" mk_status mk_tf_forward(mk_context* ctx, const mk_render_params* p, c
     hipStream_t s = (hipStream_t)stream;
     st = mk_order_after_context(ctx, s);
     if (st) return st;
-    const uint32_t tiles = (uint32_t)(((p->width + 15) / 16) * ((p->height + 15) / 16));
-    hipLaunchKernelGGL(k_fit_forward, dim3(tiles), dim3(256), (size_t)n * sizeof(float4), s, A, dev_rgb, dev_rgb64);
+    const uint32_t tiles = (uint32_t)(((p->width + kFitTile - 1) / kFitTile) * ((p->height + kFitTile - 1) / kFitTile));
+    const size_t lds_lut = (size_t)n * sizeof(float4);
+    const int box_cap = box_blocks(lds_lut);
+    hipLaunchKernelGGL(k_fit_forward, dim3(tiles), dim3(256), lds_lut + (size_t)box_region(box_cap) * kFitWaves, s, A,
+                       dev_rgb, dev_rgb64, box_cap);
     MK_HIP(ctx, hipGetLastError());
     return mk_note_reader(ctx, s);
 }
@@ -426,16 +642,20 @@ extern "C" mk_status mk_tf_backward(mk_context* ctx, const mk_render_params* p, 
     hipStream_t s = (hipStream_t)stream;
     st = mk_order_after_context(ctx, s);
     if (st) return st;
-    const uint32_t tiles = (uint32_t)(((p->width + 15) / 16) * ((p->height + 15) / 16));
-    // gradient sums in LDS beside the LUT when both fit (n <= 1365), else in HBM
+    const uint32_t tiles = (uint32_t)(((p->width + kFitTile - 1) / kFitTile) * ((p->height + kFitTile - 1) / kFitTile));
+    // gradient sums in LDS beside the LUT when both fit (n <= 1365), else in
+    // HBM; the volume's staged blocks take what is left
     const size_t lds_lut = (size_t)n * sizeof(float4);
-    const size_t lds_all = lds_lut + (4 * (size_t)n + 1) * sizeof(double);
-    if (lds_all <= kFitLdsBytes)
-        hipLaunchKernelGGL(k_fit_backward<true>, dim3(tiles), dim3(256), lds_all, s, A, dev_grad_rgb, dev_rgb64,
-                           dev_grad_lut, dev_grad_scale);
-    else
-        hipLaunchKernelGGL(k_fit_backward<false>, dim3(tiles), dim3(256), lds_lut, s, A, dev_grad_rgb, dev_rgb64,
-                           dev_grad_lut, dev_grad_scale);
+    const size_t lds_all = lds_lut + (4 * (size_t)n + 2) * sizeof(double);
+    if (lds_all <= kFitLdsBytes) {
+        const int box_cap = box_blocks(lds_all);
+        hipLaunchKernelGGL(k_fit_backward<true>, dim3(tiles), dim3(256), lds_all + (size_t)box_region(box_cap) * kFitWaves,
+                           s, A, dev_grad_rgb, dev_rgb64, dev_grad_lut, dev_grad_scale, box_cap);
+    } else {
+        const int box_cap = box_blocks(lds_lut);
+        hipLaunchKernelGGL(k_fit_backward<false>, dim3(tiles), dim3(256), lds_lut + (size_t)box_region(box_cap) * kFitWaves,
+                           s, A, dev_grad_rgb, dev_rgb64, dev_grad_lut, dev_grad_scale, box_cap);
+    }
     MK_HIP(ctx, hipGetLastError());
     return mk_note_reader(ctx, s);
 }
