@@ -93,14 +93,14 @@ struct mk_context {
 
     // VR second pass: queue of the rays still marching after vr_budget loop
     // trips of the first pass (0: one pass); renders on other streams are
-    // ordered after the last second pass through cont_done
+    // ordered after the last second pass (on cont_stream)
     int vr_budget = kVrTripBudgetAuto;
     int n_cu = 0;   // compute units of the device (queried once)
     void* d_cont = nullptr;
     size_t cont_bytes = 0;
-    hipEvent_t cont_done = nullptr;
     hipStream_t cont_stream = nullptr;
     bool cont_used = false;
+    bool cont_clean = false;  // the queue counters are zero for the next frame
     uint32_t vol_version = 0;
 
     // exact float step lattice a_{k+1} = a_k + L1 from a_0 = 0
@@ -178,6 +178,9 @@ mk_status mk_order_after_context(mk_context* ctx, hipStream_t s);
 // Records that work just enqueued on stream s reads the context's device
 // data; no-op when s is the context's stream (its own work is in order).
 mk_status mk_note_reader(mk_context* ctx, hipStream_t s);
+// Makes stream s wait for the work recorded so far by mk_note_reader on
+// stream r (nothing when r's record was drained, which synchronized it).
+mk_status mk_order_after_reader(mk_context* ctx, hipStream_t r, hipStream_t s);
 // Makes the context's stream wait for all work recorded by mk_note_reader, so
 // that what it enqueues next may rewrite that data; a synchronize of the
 // context stream after this call also waits for those readers.

@@ -60,6 +60,11 @@ constexpr float kLongSteps = 256.0f;                 // second-pass rays claimed
 constexpr int kTailGroup = 8;                        // second-pass lanes per ray
 constexpr size_t kTailPercent = 95;                  // first-pass waves finished before the rest hand off
 constexpr size_t kTermsLdsMax = 32 * 1024;           // axis term tables staged in LDS up to this
+// Queue counters: 0 rays queued at the front, 1 rays claimed by the second
+// pass, 2 rays queued at the back, 3 first-pass waves finished, 4 second-pass
+// workgroups finished. All are zero when a frame starts.
+constexpr int kQueueCounters = 5;
+constexpr int kCounterGroupsDone = 4;
 
 using namespace mkr;
 
@@ -89,6 +94,14 @@ template <bool MASK, bool LDS_LUT, bool TAB>
 __global__ void __launch_bounds__(256) k_vr_tail(RenderArgs A, AwTable T) {
     Smem s = stage_shared(A, T, LDS_LUT, (256 / kTailGroup) * (MASK ? A.nslot : 0), TAB);
     vr_tail<MASK, LDS_LUT, kTailGroup, TAB>(A, s, s.aobj + (threadIdx.x / kTailGroup) * (MASK ? A.nslot : 0));
+    // The last workgroup to finish clears the queue counters for the next
+    // frame: every other one has made its last counter access by then.
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t* c = A.cont.counters;
+        if (atomicAdd(&c[kCounterGroupsDone], 1u) == gridDim.x - 1u)
+            for (int i = 0; i < kQueueCounters; ++i) __hip_atomic_store(&c[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 __global__ void k_zero_u32(uint32_t* p, int n) {
@@ -314,13 +327,19 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     }
     const bool two_pass = p->render_type == MK_RENDER_VR && !touch && budget > 0 && cont_bytes <= kContMaxBytes;
     if (two_pass) {
-        // a larger queue replaces the buffer: the last second pass must be done with it
-        if (ctx->cont_used && ctx->cont_bytes < cont_bytes) MK_HIP(ctx, hipEventSynchronize(ctx->cont_done));
+        // a larger queue replaces the buffer, after every reader of the
+        // context (the last second pass among them) is done with the old one
+        if (ctx->cont_bytes < cont_bytes) ctx->cont_clean = false;
         st = mk_ensure_buffer(ctx, &ctx->d_cont, &ctx->cont_bytes, cont_bytes);
         if (st) return st;
-        if (!ctx->cont_done) MK_HIP(ctx, hipEventCreateWithFlags(&ctx->cont_done, hipEventDisableTiming));
         // renders share the queue: one on another stream waits for the last
-        if (ctx->cont_used && ctx->cont_stream != s) MK_HIP(ctx, hipStreamWaitEvent(s, ctx->cont_done, 0));
+        // (one on the context's stream is already ordered by
+        // mk_order_after_context, one on a caller's stream by that stream's
+        // reader event, recorded after its second pass)
+        if (ctx->cont_used && ctx->cont_stream != s && ctx->cont_stream != ctx->stream) {
+            st = mk_order_after_reader(ctx, ctx->cont_stream, s);
+            if (st) return st;
+        }
         uint8_t* q = (uint8_t*)ctx->d_cont;
         VrCont& C = A.cont;
         C.counters = (uint32_t*)q;
@@ -347,7 +366,10 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
         // return at once and never count themselves
         const size_t live_waves = (size_t)((p->width + 7) / 8) * (size_t)((p->height + 7) / 8);
         C.tail_waves = (uint32_t)(live_waves * kTailPercent / 100);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, C.counters, 4);
+        // each second pass leaves the counters at zero; a new buffer, or a
+        // frame whose second pass was not enqueued, needs them cleared
+        if (!ctx->cont_clean) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, C.counters, kQueueCounters);
+        ctx->cont_clean = false;
     }
     dim3 grid(tiles), block(tw * tw);
 #define MK_LAUNCH(K) hipLaunchKernelGGL(K, grid, block, lds, s, A, T)
@@ -375,7 +397,8 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
                 else { if (lds_lut) MK_TAIL((k_vr_tail<false, true, false>)); else MK_TAIL((k_vr_tail<false, false, false>)); }
             }
 #undef MK_TAIL
-            MK_HIP(ctx, hipEventRecord(ctx->cont_done, s));
+            MK_HIP(ctx, hipGetLastError());
+            ctx->cont_clean = true;
             ctx->cont_stream = s;
             ctx->cont_used = true;
         }

@@ -137,7 +137,6 @@ extern "C" void mk_context_destroy(mk_context* ctx) {
     mk_free(ctx->d_lut);
     mk_free(ctx->d_lutnz);
     mk_free(ctx->d_cont);
-    if (ctx->cont_done) (void)hipEventDestroy(ctx->cont_done);
     if (ctx->mc_done) (void)hipEventDestroy(ctx->mc_done);
     if (ctx->order_ev) (void)hipEventDestroy(ctx->order_ev);
     mk_free(ctx->d_steps);
@@ -161,6 +160,10 @@ extern "C" void* mk_context_stream(mk_context* ctx) { return ctx ? (void*)ctx->s
 
 mk_status mk_order_after_context(mk_context* ctx, hipStream_t s) {
     if (s == ctx->stream) return MK_OK;
+    // an idle context stream has nothing left to order after
+    hipError_t q = hipStreamQuery(ctx->stream);
+    if (q == hipSuccess) return MK_OK;
+    if (q != hipErrorNotReady) return mk_hip_fail(ctx, q, "querying the context stream");
     // the wait takes the event's state at this call, so one event serves all
     hipError_t e = hipEventRecord(ctx->order_ev, ctx->stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(s, ctx->order_ev, 0);
@@ -194,6 +197,15 @@ mk_status mk_note_reader(mk_context* ctx, hipStream_t s) {
         return mk_hip_fail(ctx, e, "recording a reader of the context");
     }
     ctx->readers.push_back(r);
+    return MK_OK;
+}
+
+mk_status mk_order_after_reader(mk_context* ctx, hipStream_t r, hipStream_t s) {
+    for (auto& e : ctx->readers)
+        if (e.stream == r) {
+            MK_HIP(ctx, hipStreamWaitEvent(s, e.done, 0));
+            break;
+        }
     return MK_OK;
 }
 
