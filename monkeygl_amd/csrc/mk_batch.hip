@@ -408,6 +408,7 @@ extern "C" mk_status mk_batch_submit(mk_batcher* b, int slot, const mk_batch_ite
             A.voi = p.voi;
             memcpy(A.bg, p.background, sizeof(A.bg));
             A.l1 = c->steps_l1;
+            A.rl1 = 1.0f / c->steps_l1;
             A.exit_alpha = p.exit_alpha;
             A.max_iter = 24 * c->steps_n + 64;
             const bool mask = var == V_VR_MASK_LDS || var == V_VR_MASK_GLB || var == V_MIP_MASK;

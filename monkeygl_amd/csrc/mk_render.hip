@@ -270,6 +270,7 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     A.voi = p->voi;
     memcpy(A.bg, p->background, sizeof(A.bg));
     A.l1 = l1;
+    A.rl1 = 1.0f / l1;
     A.exit_alpha = p->exit_alpha;
     // each accepted sample advances by at least L1/8 and at most two rejections
     // precede an acceptance: 3 * 8 trips per base step bound every ray

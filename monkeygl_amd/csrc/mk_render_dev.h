@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "mk_context.h"
+#include "mk_lattice.h"
 
 namespace mkr {
 
@@ -62,6 +63,7 @@ struct RenderArgs {
     mk_voi voi;
     float bg[4];
     float l1;
+    float rl1;   // 1 / l1 (an estimate for lattice_skip)
     float exit_alpha;
     int max_iter;      // hard bound on loop trips per ray
     VrCont cont;       // VR second pass (cap 0 in every other use)
@@ -545,14 +547,12 @@ __device__ inline float start_acc(const RenderArgs& A, float a_in) {
     return A.steps[k0];
 }
 
-// Advances acc over the lattice points a_{k+1} = a_k + L1 that lie below
-// a_skip (all inside one cell that cannot change the result); it counts trips
-// against the same budget as the marching loop.
-__device__ inline void skip_lattice(float& acc, float l1, float a_skip, int& it, int max_iter) {
-    while (acc + l1 < a_skip && it < max_iter) {
-        acc += l1;
-        ++it;
-    }
+// skip_lattice: advances acc over the points acc + L1, acc + L1 + L1, ...
+// that lie below a_skip (all inside one cell that cannot change the result),
+// counting trips against the same budget as the marching loop; the same
+// floats as adding one L1 at a time (lattice_skip, mk_lattice.h).
+__device__ inline void skip_lattice(const RenderArgs& A, float& acc, float a_skip, int& it) {
+    lattice_skip(acc, A.l1, A.rl1, a_skip, it, A.max_iter);
 }
 
 // Alpha of a sample taken at a refined step, as getNextStep corrects it
@@ -720,7 +720,7 @@ __device__ inline void vr_pixel(const RenderArgs& A, const Smem& s, int x, int y
             if (aAcc > A.exit_alpha) break;
             // steady state in an empty cell: every further lattice point inside
             // it is accepted with alpha 0 and changes nothing but acc
-            if (empty_shift && step == L1) skip_lattice(acc, L1, cell_exit(A, g, cx, cy, cz, empty_shift), it, A.max_iter);
+            if (empty_shift && step == L1) skip_lattice(A, acc, cell_exit(A, g, cx, cy, cz, empty_shift), it);
         }
         if (!TOUCH && it_queued >= 0) {
             // one queue slot per lane, one atomic per wave and end; the queue
@@ -1013,7 +1013,7 @@ __device__ inline void mip_march_single(const RenderArgs& A, const Smem& s, floa
                 }
             }
             acc += L1;
-            if (skip_shift) skip_lattice(acc, L1, cell_exit(A, g, cx, cy, cz, skip_shift), it, A.max_iter);
+            if (skip_shift) skip_lattice(A, acc, cell_exit(A, g, cx, cy, cz, skip_shift), it);
         } else {
             acc += L1;
         }
@@ -1105,7 +1105,7 @@ __device__ inline void mip_march_grouped(const RenderArgs& A, const Smem& s, flo
         acc = acc_j;
         it += n;
         // the last point's cell, skipped as one point at a time would
-        if (last_shift) skip_lattice(acc, L1, cell_exit(A, g, lx, ly, lz, last_shift), it, A.max_iter);
+        if (last_shift) skip_lattice(A, acc, cell_exit(A, g, lx, ly, lz, last_shift), it);
     }
 }
 
@@ -1170,7 +1170,7 @@ __device__ inline void surface_pixel(const RenderArgs& A, int x, int y) {
                     }
                 }
                 acc += L1;
-                if (skip_shift) skip_lattice(acc, L1, cell_exit(A, g, cx, cy, cz, skip_shift), it, A.max_iter);
+                if (skip_shift) skip_lattice(A, acc, cell_exit(A, g, cx, cy, cz, skip_shift), it);
             } else {
                 acc += L1;
             }
