@@ -100,10 +100,11 @@ struct Sample {
     bool grad_ok; // the clamp is inactive: 0 <= s*alpha < 1 (one-sided at 0)
 };
 
-// A sample from its fetched corners; tri keeps them for fit_shade.
+// A sample from its fetched corners, already raised to -32767 at least;
+// tri keeps them for fit_shade.
 __device__ inline Sample fit_sample(const FitArgs& A, const float4* s_lut, const mkr::TriFetch& F, mkr::TriSample& tri) {
     Sample S;
-    float t = mkr::sample_finish(F, tri);
+    float t = mkr::sample_finish<false>(F, tri);
     t = (t - A.wl) / A.ww + 0.5f;
     t = t > 1.0f ? 1.0f : (t >= -1.0f ? t : -1.0f);
     mk_split(t, A.n, S.i0, S.i1, S.f);
@@ -225,8 +226,9 @@ __device__ inline void box_terms_build(const MkVolView& v, const FitBox& B, int 
 }
 
 // sample_fetch and grad_taps_fetch from the box: the same clamped indices,
-// so the same voxel values. Floor indices lie in [-3, n + 2], so the entries
-// read, ix + 3 .. ix + 6 - 4 * b0, lie in the tables.
+// so the same voxel values (raised to -32767 at least when the box was
+// staged, as the samplers raise them). Floor indices lie in [-3, n + 2], so
+// the entries read, ix + 3 .. ix + 6 - 4 * b0, lie in the tables.
 __device__ inline void box_fetch(const MkVolView& v, const FitBox& B, const float p[3], mkr::TriFetch& F,
                                  mkr::GradTaps& G) {
     mkr::grad_axis_base(p[0], v.nx, F.ix, F.ax);
@@ -244,7 +246,7 @@ __device__ inline void box_fetch(const MkVolView& v, const FitBox& B, const floa
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) F.r[k] = box_vox(B, OX[1 + (k & 1)] + OY[1 + ((k >> 1) & 1)] + OZ[1 + (k >> 2)]);
-#define MK_BV(a, b, c) ((float)max(box_vox(B, OX[a] + OY[b] + OZ[c]), -32767))
+#define MK_BV(a, b, c) ((float)box_vox(B, OX[a] + OY[b] + OZ[c]))
 #pragma unroll
     for (int k = 0; k < 2; ++k)
 #pragma unroll
@@ -257,6 +259,14 @@ __device__ inline void box_fetch(const MkVolView& v, const FitBox& B, const floa
             G.z[k][j][1] = MK_BV(1 + j, 1 + k, 3);
         }
 #undef MK_BV
+}
+
+// Two int16 voxels raised to -32767 at least (the samplers read -32768 as
+// -32767), with one packed max.
+__device__ inline uint32_t vox_floor2(uint32_t w) {
+    typedef short short2v __attribute__((ext_vector_type(2)));
+    const short2v v = __builtin_elementwise_max(__builtin_bit_cast(short2v, w), (short2v){-32767, -32767});
+    return __builtin_bit_cast(uint32_t, v);
 }
 
 // LDS bytes of one wave's region: the box, then its term tables.
@@ -355,7 +365,9 @@ __device__ inline void fit_march(const FitArgs& A, const float4* s_lut, uint8_t*
                     (uint32_t)(X >> 1);
                 const uint32_t off = (brick << 10) | ((uint32_t)(Z & 1) << 9) | ((uint32_t)(Y & 1) << 8) |
                                      ((uint32_t)(X & 1) << 7) | ((uint32_t)part << 4);
-                *(uint4*)(s_box + blk * 128 + part * 16) = *(const uint4*)((const uint8_t*)A.v.vox + off);
+                const uint4 w = *(const uint4*)((const uint8_t*)A.v.vox + off);
+                *(uint4*)(s_box + blk * 128 + part * 16) =
+                    make_uint4(vox_floor2(w.x), vox_floor2(w.y), vox_floor2(w.z), vox_floor2(w.w));
             }
             wave_lds_sync();
         }
@@ -371,6 +383,8 @@ __device__ inline void fit_march(const FitArgs& A, const float4* s_lut, uint8_t*
                 } else {
                     mkr::sample_fetch(A.v, P, F);
                     mkr::grad_taps_fetch(A.v, F.ix, F.iy, F.iz, G);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) F.r[q] = max(F.r[q], -32767);
                 }
                 mkr::TriSample tri;
                 const Sample S = fit_sample(A, s_lut, F, tri);

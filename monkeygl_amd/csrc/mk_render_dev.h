@@ -247,7 +247,9 @@ __device__ inline void sample_fetch(const MkVolView& v, const float p[3], TriFet
 }
 
 // The trilinear value of a fetched sample, keeping what the gradient reuses
-// in T (same arithmetic as mk_sample_hu).
+// in T (same arithmetic as mk_sample_hu). CLAMP = false: the fetch's voxels
+// are already at least -32767.
+template <bool CLAMP = true>
 __device__ inline float sample_finish(const TriFetch& F, TriSample& T) {
     T.ix = F.ix;
     T.iy = F.iy;
@@ -258,7 +260,7 @@ __device__ inline float sample_finish(const TriFetch& F, TriSample& T) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int q = F.r[k];
-        T.c[k >> 2][(k >> 1) & 1][k & 1] = (float)(q < -32767 ? -32767 : q);
+        T.c[k >> 2][(k >> 1) & 1][k & 1] = (float)(CLAMP && q < -32767 ? -32767 : q);
     }
     const float c00 = T.c[0][0][0] + T.ax * (T.c[0][0][1] - T.c[0][0][0]);
     const float c10 = T.c[0][1][0] + T.ax * (T.c[0][1][1] - T.c[0][1][0]);
