@@ -98,6 +98,24 @@ def test_build_mode_step_exit_4096_lut(gpu_ctx, small):
         assert frac >= MIN_FRAC, (ang, frac, dmax)
 
 
+@pytest.mark.parametrize("rt", [MK_RENDER_VR, MK_RENDER_MIP])
+@pytest.mark.parametrize("step", [0.05, 0.3, 3.0])
+def test_step_scale_sweep(gpu_ctx, small, rt, step):
+    """Fine and coarse sampling steps: the empty-cell walk jumps over the float
+    accumulation acc += L1 by per-binade closed forms whose size and caps
+    depend on L1 (core/kernel.cu:133-264 and 266-364 add L1 once per lattice
+    point), so frames at 1/20, 3/10 and 3 times the voxel step against FP64."""
+    vol, lut = small
+    gpu_ctx.set_volume(vol)
+    gpu_ctx.set_lut(0, lut)
+    p = _params(rt, (20, 35), size=128, step_scale=step)
+    got = gpu_ctx.render(p)
+    ref = O.render(p, vol, {0: lut}, precision="f64")
+    frac, dmax = frac_within(got, ref)
+    print(f"rt={rt} step={step} frac={frac:.6f} max={dmax}")
+    assert frac >= MIN_FRAC, (step, frac, dmax)
+
+
 def test_long_volume_composes_address_terms(gpu_ctx):
     """A 24x20x8200 volume (40 x 40 x 0.1 mm voxels, about cubic in space):
     its per-axis address-term tables would take 33 KiB, above the 32 KiB the
