@@ -229,8 +229,9 @@ mk_status mk_render_async(mk_context* ctx, const mk_render_params* p, uint8_t* d
  * of a ray at once and applies the reference's loop (core/kernel.cu:133-264)
  * to them in order. The frame is the same for every value; 0 renders in one
  * pass; -1 (the default) chooses per frame: 64 when all of the frame's waves
- * fit on the device at once, else 1024. Frames whose queue would exceed 2 GiB
- * render in one pass. */
+ * fit on the device at once, 256 when they fill it at most one and a half
+ * times, else 1024. Frames whose queue would exceed 2 GiB render in one
+ * pass. */
 mk_status mk_set_vr_trip_budget(mk_context* ctx, int trips);
 
 /* Measurement variant of mk_render_async (no reference counterpart): renders

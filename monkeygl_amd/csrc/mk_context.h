@@ -38,9 +38,11 @@ struct MkLutTable {
 // Loop trips of the first VR pass after which a ray moves to the second (the
 // rays of its last waves move earlier), chosen per frame unless set: a frame
 // whose waves all fit on the device at once (few rays, idle SIMDs) hands off
-// early, a larger one late.
+// early, one of up to one and a half such rounds after a middle budget, a
+// larger one late.
 constexpr int kVrTripBudgetAuto = -1;
 constexpr int kVrTripBudgetSmall = 64;
+constexpr int kVrTripBudgetMid = 256;
 constexpr int kVrTripBudgetLarge = 1024;
 constexpr int kVrResidentWavesPerCu = 12;   // 4 SIMDs x 3 waves of the first pass
 

@@ -324,7 +324,8 @@ mk_status launch_render(mk_context* ctx, const mk_render_params* p, uint8_t* dev
     if (ctx->n_cu <= 0) MK_HIP(ctx, hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     if (budget == kVrTripBudgetAuto) {
         const size_t waves = (size_t)tiles * (size_t)(tw * tw / 64);
-        budget = waves <= (size_t)ctx->n_cu * kVrResidentWavesPerCu ? kVrTripBudgetSmall : kVrTripBudgetLarge;
+        const size_t round = (size_t)ctx->n_cu * kVrResidentWavesPerCu;
+        budget = waves <= round ? kVrTripBudgetSmall : (2 * waves <= 3 * round ? kVrTripBudgetMid : kVrTripBudgetLarge);
     }
     const bool two_pass = p->render_type == MK_RENDER_VR && !touch && budget > 0 && cont_bytes <= kContMaxBytes;
     if (two_pass) {
